@@ -1,0 +1,11 @@
+"""MI355X-native (gfx950) YASHE polynomial-ring hot path.
+
+The compute lives in ``lib/libyashe_hip.so`` (hand-written HIP kernels behind
+the C ABI of include/yashe_hip.h); this package only binds it for Python
+callers (tests, benchmarks, the serving loop).
+"""
+from ._lib import (  # noqa: F401
+    Context, Ring, Evk, DeviceBuffer, YashError, lib, LIB_PATH,
+    ints_to_array, array_to_ints, int_to_limbs, ring_plan,
+    OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD,
+)

@@ -1,0 +1,316 @@
+"""ctypes binding of ``libyashe_hip.so`` (the C ABI in include/yashe_hip.h).
+
+There is no fallback: if the gfx950 library is not built, or no GPU is
+visible, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libyashe_hip.so")
+
+YH_OK = 0
+YH_FORWARD = 1
+YH_INVERSE = 0
+YH_ADD = 0
+YH_SUB = 1
+
+OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD = range(5)
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+
+
+class YashError(RuntimeError):
+    pass
+
+
+class RingInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("q_limbs", ctypes.c_int), ("q_bits", ctypes.c_int), ("R", ctypes.c_int),
+        ("R_ks", ctypes.c_int), ("w_bits", ctypes.c_int), ("lw", ctypes.c_int), ("work_limbs", ctypes.c_int),
+        ("m_bits", ctypes.c_int), ("m_ks_bits", ctypes.c_int), ("t", ctypes.c_uint64), ("prime_max", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "yh_last_error": (ctypes.c_char_p, []),
+    "yh_version": (ctypes.c_int, []),
+    "yh_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "yh_ctx_destroy": (ctypes.c_int, [_vp]),
+    "yh_ctx_stream": (_vp, [_vp]),
+    "yh_ctx_sync": (ctypes.c_int, [_vp]),
+    "yh_malloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "yh_free": (ctypes.c_int, [_vp, _vp]),
+    "yh_memcpy_htod": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "yh_memcpy_dtoh": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "yh_memcpy_dtod": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "yh_memset": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_size_t]),
+    "yh_ring_create": (ctypes.c_int, [_vp, ctypes.c_int, _u64p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
+                                      ctypes.POINTER(_vp)]),
+    "yh_ring_destroy": (ctypes.c_int, [_vp]),
+    "yh_ring_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(RingInfo)]),
+    "yh_ring_plan": (ctypes.c_int, [ctypes.c_int, _u64p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.POINTER(RingInfo), _u64p, ctypes.c_int]),
+    "yh_ring_get_primes": (ctypes.c_int, [_vp, _u64p, ctypes.c_int]),
+    "yh_ring_get_modulus": (ctypes.c_int, [_vp, ctypes.c_int, _u64p, ctypes.c_int]),
+    "yh_crt": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
+    "yh_icrt": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int]),
+    "yh_ntt": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "yh_pointwise_mulmod": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "yh_residue_addsub": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "yh_poly_mul_plain": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int)]),
+    "yh_ring_mul": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_ring_addsub": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "yh_ring_reduce": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int]),
+    "yh_yashe_mul": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_evk_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "yh_evk_destroy": (ctypes.c_int, [_vp]),
+    "yh_keyswitch": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_yashe_mul_keyswitch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_yashe_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_mem_footprint": (ctypes.c_size_t, [_vp, ctypes.c_int]),
+}
+
+
+def exported_symbols() -> List[str]:
+    return list(_SIGS)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise YashError(f"{LIB_PATH} is not built; run `python -m cuyashe_amd.build` "
+                            "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != YH_OK:
+        msg = lib().yh_last_error()
+        raise YashError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+# ------------------------------------------------------------------ helpers
+
+def int_to_limbs(v: int, L: int) -> List[int]:
+    mask = (1 << 64) - 1
+    return [(v >> (64 * l)) & mask for l in range(L)]
+
+
+def ints_to_array(values: Sequence[int], L: int) -> np.ndarray:
+    """[L][n] uint64 limb-major array."""
+    vals = [int(v) for v in values]
+    out = np.empty((L, len(vals)), dtype=np.uint64)
+    mask = (1 << 64) - 1
+    for l in range(L):
+        sh = 64 * l
+        out[l] = np.fromiter(((v >> sh) & mask for v in vals), dtype=np.uint64, count=len(vals))
+    return out
+
+
+def array_to_ints(arr: np.ndarray) -> List[int]:
+    arr = np.asarray(arr, dtype=np.uint64)
+    L = arr.shape[0]
+    vals = [int(x) for x in arr[L - 1]]
+    for l in range(L - 2, -1, -1):
+        row = arr[l]
+        vals = [(v << 64) | int(x) for v, x in zip(vals, row)]
+    return vals
+
+
+def ring_plan(n: int, q: int, t: int = 0, w_bits: int = 0):
+    """Host-only parameter planning: (RingInfo, primes)."""
+    q = int(q)
+    L = max(1, (q.bit_length() + 63) // 64)
+    ql = (ctypes.c_uint64 * L)(*int_to_limbs(q, L))
+    info = RingInfo()
+    buf = (ctypes.c_uint64 * 512)()
+    check(lib().yh_ring_plan(n, ql, L, t, w_bits, ctypes.byref(info), buf, 512), "yh_ring_plan")
+    return info, [int(buf[i]) for i in range(info.R)]
+
+
+class DeviceBuffer:
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(lib().yh_malloc(ctx.h, self.nbytes, ctypes.byref(p)), "yh_malloc")
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            check(lib().yh_free(self.ctx.h, self.ptr), "yh_free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                lib().yh_free(self.ctx.h, self.ptr)
+        except Exception:
+            pass
+
+    def upload(self, arr: np.ndarray):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(lib().yh_memcpy_htod(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "yh_memcpy_htod")
+
+    def download(self, shape, dtype=np.uint64) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().yh_memcpy_dtoh(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "yh_memcpy_dtoh")
+        return out
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = _vp()
+        check(lib().yh_ctx_create(device, ctypes.byref(h)), "yh_ctx_create")
+        self.h = h.value
+
+    @property
+    def stream(self) -> int:
+        return lib().yh_ctx_stream(self.h)
+
+    def sync(self):
+        check(lib().yh_ctx_sync(self.h), "yh_ctx_sync")
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr: np.ndarray) -> DeviceBuffer:
+        a = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, max(a.nbytes, 1))
+        b.upload(a)
+        return b
+
+    def ntt(self, buf: DeviceBuffer, N: int, count: int, inverse: bool = False):
+        check(lib().yh_ntt(self.h, buf.ptr, N, count, YH_INVERSE if inverse else YH_FORWARD), "yh_ntt")
+
+    def pointwise(self, a: DeviceBuffer, b: DeviceBuffer, count: int):
+        check(lib().yh_pointwise_mulmod(self.h, a.ptr, b.ptr, count), "yh_pointwise_mulmod")
+
+    def close(self):
+        if self.h:
+            lib().yh_ctx_destroy(self.h)
+            self.h = None
+
+
+class Ring:
+    """One ring Z_q[x]/(x^n + 1) with its CRT primes and tables."""
+
+    def __init__(self, ctx: Context, n: int, q: int, t: int = 0, w_bits: int = 0):
+        self.ctx = ctx
+        self.q = int(q)
+        self.n = int(n)
+        self.t = int(t)
+        L = max(1, (self.q.bit_length() + 63) // 64)
+        ql = (ctypes.c_uint64 * L)(*int_to_limbs(self.q, L))
+        h = _vp()
+        check(lib().yh_ring_create(ctx.h, n, ql, L, t, w_bits, ctypes.byref(h)), "yh_ring_create")
+        self.h = h.value
+        info = RingInfo()
+        check(lib().yh_ring_get_info(self.h, ctypes.byref(info)), "yh_ring_get_info")
+        self.info = info
+        self.L = info.q_limbs
+        self.R = info.R
+        self.R_ks = info.R_ks
+        self.lw = info.lw
+        self.w_bits = info.w_bits
+        self.LW = info.work_limbs
+
+    def primes(self) -> List[int]:
+        buf = (ctypes.c_uint64 * self.R)()
+        check(lib().yh_ring_get_primes(self.h, buf, self.R), "yh_ring_get_primes")
+        return [int(v) for v in buf]
+
+    def modulus(self, ks: bool = False) -> int:
+        buf = (ctypes.c_uint64 * self.LW)()
+        check(lib().yh_ring_get_modulus(self.h, 1 if ks else 0, buf, self.LW), "yh_ring_get_modulus")
+        return sum(int(v) << (64 * i) for i, v in enumerate(buf))
+
+    # -- array helpers: a batch of ring elements is [batch][L][n] uint64
+    def pack(self, polys: Sequence[Sequence[int]]) -> np.ndarray:
+        return np.stack([ints_to_array(list(p) + [0] * (self.n - len(p)), self.L) for p in polys])
+
+    def unpack(self, arr: np.ndarray) -> List[List[int]]:
+        return [array_to_ints(a) for a in arr]
+
+    def footprint(self, op: int) -> int:
+        return int(lib().yh_mem_footprint(self.h, op))
+
+    # -- operations on device buffers
+    def crt(self, coefs: DeviceBuffer, limbs: int, ncoef: int, spacing: int, out: DeviceBuffer):
+        check(lib().yh_crt(self.h, coefs.ptr, limbs, ncoef, spacing, out.ptr), "yh_crt")
+
+    def icrt(self, res: DeviceBuffer, spacing: int, ncoef: int, out: DeviceBuffer, out_limbs: int):
+        check(lib().yh_icrt(self.h, res.ptr, spacing, ncoef, out.ptr, out_limbs), "yh_icrt")
+
+    def residue_addsub(self, c: DeviceBuffer, a: DeviceBuffer, b: DeviceBuffer, spacing: int, sub: bool):
+        check(lib().yh_residue_addsub(self.h, c.ptr, a.ptr, b.ptr, spacing, YH_SUB if sub else YH_ADD),
+              "yh_residue_addsub")
+
+    def poly_mul_plain(self, a: DeviceBuffer, deg_a: int, b: DeviceBuffer, deg_b: int, out: DeviceBuffer,
+                       out_limbs: int, min_N: int = 1) -> int:
+        N = ctypes.c_int(0)
+        check(lib().yh_poly_mul_plain(self.h, a.ptr, deg_a, b.ptr, deg_b, min_N, out.ptr, out_limbs, ctypes.byref(N)),
+              "yh_poly_mul_plain")
+        return N.value
+
+    def ring_mul(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
+        check(lib().yh_ring_mul(self.h, a.ptr, b.ptr, out.ptr, batch), "yh_ring_mul")
+
+    def ring_addsub(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1, sub: bool = False):
+        check(lib().yh_ring_addsub(self.h, a.ptr, b.ptr, out.ptr, batch, YH_SUB if sub else YH_ADD), "yh_ring_addsub")
+
+    def ring_reduce(self, c: DeviceBuffer, in_limbs: int, out: DeviceBuffer, batch: int = 1):
+        check(lib().yh_ring_reduce(self.h, c.ptr, in_limbs, out.ptr, batch), "yh_ring_reduce")
+
+    def yashe_mul(self, c1: DeviceBuffer, c2: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
+        check(lib().yh_yashe_mul(self.h, c1.ptr, c2.ptr, out.ptr, batch), "yh_yashe_mul")
+
+    def evk(self, gamma: DeviceBuffer) -> "Evk":
+        return Evk(self, gamma)
+
+    def keyswitch(self, evk: "Evk", c: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
+        check(lib().yh_keyswitch(self.h, evk.h, c.ptr, out.ptr, batch), "yh_keyswitch")
+
+    def mul_keyswitch(self, evk: "Evk", c1: DeviceBuffer, c2: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
+        check(lib().yh_yashe_mul_keyswitch(self.h, evk.h, c1.ptr, c2.ptr, out.ptr, batch), "yh_yashe_mul_keyswitch")
+
+    def decrypt(self, f: DeviceBuffer, c: DeviceBuffer, m_out: DeviceBuffer, batch: int = 1):
+        check(lib().yh_yashe_decrypt(self.h, f.ptr, c.ptr, m_out.ptr, batch), "yh_yashe_decrypt")
+
+    def close(self):
+        if self.h:
+            lib().yh_ring_destroy(self.h)
+            self.h = None
+
+
+class Evk:
+    def __init__(self, ring: Ring, gamma: DeviceBuffer):
+        h = _vp()
+        check(lib().yh_evk_create(ring.h, gamma.ptr, ctypes.byref(h)), "yh_evk_create")
+        self.h = h.value
+        self.ring = ring
+
+    def close(self):
+        if self.h:
+            lib().yh_evk_destroy(self.h)
+            self.h = None

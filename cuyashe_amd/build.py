@@ -1,0 +1,49 @@
+"""Build recipe for the gfx950 shared library (and the host C++ API).
+
+Runs hipcc directly (no CMake, no JIT cache) and writes the libraries into
+``cuyashe_amd/lib`` so they travel with the source tree.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+ARCH = "gfx950"
+# Packed FP32 forms are kept out of every HIP object (the kernels are
+# integer-only, the flag guards any float code the compiler might vectorise).
+HIP_FLAGS = [
+    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+LIB_HIP = os.path.join(LIBDIR, "libyashe_hip.so")
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_hip(verbose: bool = True) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    src = os.path.join(CSRC, "yashe_hip.hip")
+    tmp = LIB_HIP + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, src]
+    _run(cmd)
+    os.replace(tmp, LIB_HIP)
+    return LIB_HIP
+
+
+def build_all():
+    return [build_hip()]
+
+
+if __name__ == "__main__":
+    build_all()

@@ -1,0 +1,880 @@
+// HIP kernels for gfx950 (CDNA4): CRT, Goldilocks NTT, pointwise products,
+// ICRT and the YASHE epilogues.
+//
+// Reference kernels these replace (behaviour, not structure):
+//  * cuCRT                src/cuda_bn.cu:803-835     -> crt_residue()
+//  * NTT / NTTIteration   src/cuda_functions.cu:404-458 (one radix-2 stage per
+//                         launch)                    -> ntt_fwd()/ntt_inv(): a
+//                         whole transform per workgroup, LDS-resident, four
+//                         radix-2 stages per register pass
+//  * polynomialNTTMul     src/cuda_functions.cu:460-474 -> fused pointwise step
+//  * NTTScale             src/cuda_functions.cu:397-402 -> multiply by N^-1 mod P
+//  * cuPreICRT + cuICRT   src/cuda_bn.cu:854-968     -> k_icrt (fixed-point
+//                         quotient, no per-coefficient Barrett by M)
+//  * Polynomial::reduce   src/cuda_functions.cu:1040-1160 (host fold + mod q)
+//                         -> negacyclic transforms make the fold implicit; the
+//                         mod-q / scale-and-round epilogues of k_icrt
+//  * Ciphertext::operator* rounding  src/ciphertext.cpp:55-63 -> EPI_SCALE
+//  * Ciphertext::worddecomp          src/ciphertext.cpp:109-130 -> digit loads
+//                         inside k_keyswitch (no digit polynomials in HBM)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gl64.hpp"
+
+namespace yhk {
+
+constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stages)
+
+// Device view of one ring's tables (all pointers are device memory).
+struct RingDev {
+  int n, logn, LQ, LW;
+  int R, w_bits, lw;
+  uint64_t t;
+  uint64_t n_inv;
+  const uint64_t* psi_rev;
+  const uint64_t* psiinv_rev;
+  const uint64_t* primes;
+  const uint64_t* barrett;  // floor(2^64 / p_i)
+  const uint64_t* pow32;    // [R][2*LW]
+  const uint64_t* q;        // [LQ]
+  const uint64_t* mu;       // [LW+2-LQ]
+  // reconstruction sets: 0 = all R primes, 1 = keyswitch prefix
+  int set_R[2];
+  const uint64_t* Mi[2];    // [R][LW]
+  const uint64_t* invMi[2]; // [R]
+  const uint64_t* M[2];     // [LW]
+};
+
+// ---------------------------------------------------------------- small mod p
+
+// x mod p for x < 2^64 with b = floor(2^64/p) (p < 2^40).
+__device__ __forceinline__ uint64_t mod_p(uint64_t x, uint64_t p, uint64_t b) {
+  uint64_t qh = gl::umulhi(x, b);
+  uint64_t r = x - qh * p;
+  return r >= p ? r - p : r;
+}
+
+// Residue r in [0, p) as a Goldilocks element of its centred lift in
+// (-p/2, p/2]; centring halves the magnitude entering the convolution.
+__device__ __forceinline__ uint64_t center(uint64_t r, uint64_t p) {
+  return r > (p >> 1) ? gl::P - (p - r) : r;
+}
+
+// CRT of one coefficient (LIN 64-bit limbs, limb-major with stride `ls`):
+// sum_k word_k * (2^(32k) mod p), then one Barrett reduction.
+// Matches bn_mod1_low (src/cuda_bn.cu:290-317): the residue x mod p.
+__device__ __forceinline__ uint64_t crt_residue(const uint64_t* a, int lin, size_t ls,
+                                                const uint64_t* pw, uint64_t p, uint64_t b) {
+  uint64_t acc = 0;
+  for (int l = 0; l < lin; ++l) {
+    uint64_t v = a[(size_t)l * ls];
+    acc += (v & 0xFFFFFFFFull) * pw[2 * l] + (v >> 32) * pw[2 * l + 1];
+    // each product < 2^32 * 2^28; reducing every 4 limbs keeps the sum < 2^64
+    if ((l & 3) == 3) acc = mod_p(acc, p, b);
+  }
+  return mod_p(acc, p, b);
+}
+
+// ---------------------------------------------------------------- NTT passes
+//
+// Transform of length N = 2^logN held by N/16 threads. A pass covers up to four
+// consecutive radix-2 stages; thread g holds the 16 elements
+//   j = base(g) + (c << b0),  c = 0..15,
+// base(g) = ((g >> b0) << (b0 + 4)) | (g & (2^b0 - 1)).
+// Forward (Cooley-Tukey, natural in -> bit-reversed out) stage s toggles bit
+// logN-1-s and uses T[2^s + (j >> (logN - s))]; a pass starting at stage s0
+// has b0 = logN - 4 - s0 (first pass short when logN % 4 != 0).
+// Inverse (Gentleman-Sande, bit-reversed in -> natural out) stage u toggles
+// bit u and uses Ti[(N >> (u+1)) + (j >> (u+1))]; passes start at u0 = 0, 4,
+// ... with b0 = min(u0, logN - 4) (last pass short).
+// LDS rows are padded by one element every 16 to spread the b0 = 0 pass over
+// banks.
+
+__device__ __forceinline__ int lds_idx(int j) { return j + (j >> 4); }
+__device__ __forceinline__ int pass_base(int g, int b0) {
+  return ((g >> b0) << (b0 + 4)) | (g & ((1 << b0) - 1));
+}
+
+__device__ __forceinline__ void fwd_stages(uint64_t* x, int base, int b0, int logN, int s0, int ns,
+                                           const uint64_t* __restrict__ T) {
+  for (int s = s0; s < s0 + ns; ++s) {
+    int cb = logN - 1 - s - b0;  // toggled bit within c
+    int m = 1 << cb;
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      if (c & m) continue;
+      int j = base + (c << b0);
+      uint64_t w = T[(1 << s) + (j >> (logN - s))];
+      uint64_t U = x[c];
+      uint64_t V = gl::mul(x[c | m], w);
+      x[c] = gl::add(U, V);
+      x[c | m] = gl::sub(U, V);
+    }
+  }
+}
+
+__device__ __forceinline__ void inv_stages(uint64_t* x, int base, int b0, int logN, int u0, int ns,
+                                           const uint64_t* __restrict__ Ti) {
+  int N = 1 << logN;
+  for (int u = u0; u < u0 + ns; ++u) {
+    int cb = u - b0;
+    int m = 1 << cb;
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      if (c & m) continue;
+      int j = base + (c << b0);
+      uint64_t w = Ti[(N >> (u + 1)) + (j >> (u + 1))];
+      uint64_t U = x[c];
+      uint64_t V = x[c | m];
+      x[c] = gl::add(U, V);
+      x[c | m] = gl::mul(gl::sub(U, V), w);
+    }
+  }
+}
+
+__device__ __forceinline__ void lds_store(uint64_t* lds, const uint64_t* x, int base, int b0) {
+#pragma unroll
+  for (int c = 0; c < E; ++c) lds[lds_idx(base + (c << b0))] = x[c];
+}
+__device__ __forceinline__ void lds_load(const uint64_t* lds, uint64_t* x, int base, int b0) {
+#pragma unroll
+  for (int c = 0; c < E; ++c) x[c] = lds[lds_idx(base + (c << b0))];
+}
+
+// Forward transform. On entry x holds the elements of the FIRST pass layout
+// (b0 = logN - 4, i.e. j = g + c*N/16); on exit x holds the LAST pass layout
+// (b0 = 0, j = 16*g + c), ready for a thread-local pointwise step.
+__device__ void ntt_fwd(uint64_t* x, uint64_t* lds, int logN, const uint64_t* __restrict__ T) {
+  int g = threadIdx.x;
+  int first = logN & 3;
+  if (first == 0) first = 4;
+  int s0 = 0;
+  int ns = first;
+  while (true) {
+    int b0 = logN - 4 - s0;
+    int base = pass_base(g, b0);
+    fwd_stages(x, base, b0, logN, s0, ns, T);
+    s0 += ns;
+    if (s0 >= logN) break;
+    lds_store(lds, x, base, b0);
+    __syncthreads();
+    int nb0 = logN - 4 - s0;
+    lds_load(lds, x, pass_base(g, nb0), nb0);
+    __syncthreads();
+    ns = 4;
+  }
+}
+
+// Inverse transform. On entry x holds the b0 = 0 layout (as left by ntt_fwd);
+// on exit the b0 = logN - 4 layout (j = g + c*N/16). Not scaled by N^-1.
+__device__ void ntt_inv(uint64_t* x, uint64_t* lds, int logN, const uint64_t* __restrict__ Ti) {
+  int g = threadIdx.x;
+  int u0 = 0;
+  while (true) {
+    int ns = logN - u0 < 4 ? logN - u0 : 4;
+    int b0 = u0 < logN - 4 ? u0 : logN - 4;
+    int base = pass_base(g, b0);
+    inv_stages(x, base, b0, logN, u0, ns, Ti);
+    u0 += ns;
+    if (u0 >= logN) break;
+    lds_store(lds, x, base, b0);
+    __syncthreads();
+    int nb0 = u0 < logN - 4 ? u0 : logN - 4;
+    lds_load(lds, x, pass_base(g, nb0), nb0);
+    __syncthreads();
+  }
+}
+
+// After the inverse transform: value v = x * N^-1 mod P is the exact signed
+// convolution coefficient; reduce it mod p and fold in (M/p)^-1 mod p.
+__device__ __forceinline__ uint32_t finish_residue(uint64_t x, uint64_t ninv, uint64_t p, uint64_t b,
+                                                   uint64_t invM) {
+  int64_t v = gl::lift(gl::mul(x, ninv));
+  uint64_t mag = v < 0 ? (uint64_t)(-v) : (uint64_t)v;
+  uint64_t r = mod_p(mag, p, b);
+  if (v < 0 && r) r = p - r;
+  return (uint32_t)mod_p(r * invM, p, b);
+}
+
+// ---------------------------------------------------------------- fused products
+//
+// k_prod: one workgroup per (pair, prime). CRT both operands on load, forward
+// transforms, pointwise product in registers, inverse transform, and the
+// residue y = (c mod p) * (M/p)^-1 mod p written as u32 for the ICRT.
+// Negacyclic (ring product mod x^n + 1) when T/Ti are the psi tables, cyclic
+// (plain product mod x^N - 1) with the omega tables.
+// Operand layout: limb-major [lin][lstride] per operand, operand `pair` at
+// offset pair*ostride; coefficients j >= ncoef read as zero.
+struct ProdArgs {
+  const uint64_t* A; const uint64_t* B;
+  int lin_a, lin_b;
+  int ncoef_a, ncoef_b;
+  size_t lstride_a, lstride_b;
+  size_t ostride_a, ostride_b;
+  uint32_t* Y;          // [pair][R][N]
+  int logN;
+  const uint64_t* T; const uint64_t* Ti;
+  uint64_t n_inv;
+  int set;
+};
+
+__device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
+                                         int logN, const uint64_t* pw, uint64_t p, uint64_t b) {
+  int g = threadIdx.x;
+  int stride = 1 << (logN - 4);
+#pragma unroll
+  for (int c = 0; c < E; ++c) {
+    int j = g + c * stride;
+    uint64_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, p, b) : 0;
+    x[c] = center(r, p);
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_prod(RingDev rd, ProdArgs pa) {
+  extern __shared__ uint64_t lds[];
+  const int i = blockIdx.x;
+  const int pair = blockIdx.y;
+  const uint64_t p = rd.primes[i];
+  const uint64_t b = rd.barrett[i];
+  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  uint64_t xa[E], xb[E];
+  load_crt(xa, pa.A + pair * pa.ostride_a, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.logN, pw, p, b);
+  ntt_fwd(xa, lds, pa.logN, pa.T);
+  __syncthreads();
+  load_crt(xb, pa.B + pair * pa.ostride_b, pa.lin_b, pa.ncoef_b, pa.lstride_b, pa.logN, pw, p, b);
+  ntt_fwd(xb, lds, pa.logN, pa.T);
+#pragma unroll
+  for (int c = 0; c < E; ++c) xa[c] = gl::mul(xa[c], xb[c]);
+  __syncthreads();
+  ntt_inv(xa, lds, pa.logN, pa.Ti);
+  const uint64_t invM = rd.invMi[pa.set][i];
+  const int N = 1 << pa.logN;
+  uint32_t* y = pa.Y + ((size_t)pair * rd.set_R[pa.set] + i) * N;
+  const int stride = N >> 4;
+#pragma unroll
+  for (int c = 0; c < E; ++c) y[threadIdx.x + c * stride] = finish_residue(xa[c], pa.n_inv, p, b, invM);
+}
+
+// Small transforms (N < 64): one thread per (pair, prime) computes the same
+// Goldilocks convolution of centred residues directly.
+__global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
+  const int i = blockIdx.x;
+  const int pair = blockIdx.y;
+  if (threadIdx.x != 0) return;
+  const int N = 1 << pa.logN;
+  const uint64_t p = rd.primes[i];
+  const uint64_t b = rd.barrett[i];
+  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  uint64_t ra[64], rb[64], rc[64];
+  for (int j = 0; j < N; ++j) {
+    const uint64_t* a = pa.A + pair * pa.ostride_a + j;
+    const uint64_t* bb = pa.B + pair * pa.ostride_b + j;
+    ra[j] = j < pa.ncoef_a ? center(crt_residue(a, pa.lin_a, pa.lstride_a, pw, p, b), p) : 0;
+    rb[j] = j < pa.ncoef_b ? center(crt_residue(bb, pa.lin_b, pa.lstride_b, pw, p, b), p) : 0;
+    rc[j] = 0;
+  }
+  for (int u = 0; u < N; ++u)
+    for (int v = 0; v < N; ++v) {
+      uint64_t prod = gl::mul(ra[u], rb[v]);
+      int k = u + v;
+      if (k >= N) {
+        k -= N;
+        if (negacyclic) prod = gl::neg(prod);
+      }
+      rc[k] = gl::add(rc[k], prod);
+    }
+  const uint64_t invM = rd.invMi[pa.set][i];
+  uint32_t* y = pa.Y + ((size_t)pair * rd.set_R[pa.set] + i) * N;
+  for (int j = 0; j < N; ++j) y[j] = finish_residue(rc[j], 1, p, b, invM);
+}
+
+// ---------------------------------------------------------------- keyswitch
+//
+// One workgroup per (ciphertext, prime of the keyswitch set): digit k of each
+// coefficient (bits [k*w, (k+1)*w) of c in [0, q)) is read straight from the
+// ciphertext limbs, reduced mod p, transformed, multiplied by the resident
+// NTT(gamma_k mod p) and accumulated; one inverse transform per prime.
+struct KsArgs {
+  const uint64_t* C;        // [ct][LQ][n]
+  size_t ostride;
+  const uint64_t* evk;      // [lw][R_ks][n] transform domain
+  uint32_t* Y;              // [ct][R_ks][n]
+};
+
+__device__ __forceinline__ uint64_t digit_of(const uint64_t* a, int n, int lq, int k, int wb) {
+  int bit = k * wb;
+  int l = bit >> 6, off = bit & 63;
+  uint64_t v = a[(size_t)l * n] >> off;
+  if (off + wb > 64 && l + 1 < lq) v |= a[(size_t)(l + 1) * n] << (64 - off);
+  return v & ((1ull << wb) - 1);
+}
+
+__global__ void __launch_bounds__(1024) k_keyswitch(RingDev rd, KsArgs ka) {
+  extern __shared__ uint64_t lds[];
+  const int i = blockIdx.x;
+  const int ct = blockIdx.y;
+  const uint64_t p = rd.primes[i];
+  const uint64_t b = rd.barrett[i];
+  const int logN = rd.logn, N = rd.n;
+  const int Rks = rd.set_R[1];
+  const uint64_t* c0 = ka.C + ct * ka.ostride;
+  const int stride = N >> 4;
+  uint64_t acc[E], x[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) acc[c] = 0;
+  for (int k = 0; k < rd.lw; ++k) {
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int j = threadIdx.x + c * stride;
+      x[c] = center(mod_p(digit_of(c0 + j, N, rd.LQ, k, rd.w_bits), p, b), p);
+    }
+    ntt_fwd(x, lds, logN, rd.psi_rev);
+    const uint64_t* ev = ka.evk + ((size_t)k * Rks + i) * N + threadIdx.x * E;
+#pragma unroll
+    for (int c = 0; c < E; ++c) acc[c] = gl::add(acc[c], gl::mul(x[c], ev[c]));
+    __syncthreads();
+  }
+  ntt_inv(acc, lds, logN, rd.psiinv_rev);
+  const uint64_t invM = rd.invMi[1][i];
+  uint32_t* y = ka.Y + ((size_t)ct * Rks + i) * N;
+#pragma unroll
+  for (int c = 0; c < E; ++c) y[threadIdx.x + c * stride] = finish_residue(acc[c], rd.n_inv, p, b, invM);
+}
+
+// Keyswitch for small rings (n < 64): direct negacyclic convolution per thread.
+__global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* gamma_res /*[lw][Rks][n] centred residues*/) {
+  const int i = blockIdx.x;
+  const int ct = blockIdx.y;
+  if (threadIdx.x != 0) return;
+  const int N = rd.n;
+  const int Rks = rd.set_R[1];
+  const uint64_t p = rd.primes[i];
+  const uint64_t b = rd.barrett[i];
+  const uint64_t* c0 = ka.C + ct * ka.ostride;
+  uint64_t rc[64];
+  for (int j = 0; j < N; ++j) rc[j] = 0;
+  for (int k = 0; k < rd.lw; ++k) {
+    uint64_t d[64];
+    for (int j = 0; j < N; ++j) d[j] = center(mod_p(digit_of(c0 + j, N, rd.LQ, k, rd.w_bits), p, b), p);
+    const uint64_t* gk = gamma_res + ((size_t)k * Rks + i) * N;
+    for (int u = 0; u < N; ++u)
+      for (int v = 0; v < N; ++v) {
+        uint64_t prod = gl::mul(d[u], gk[v]);
+        int kk = u + v;
+        if (kk >= N) { kk -= N; prod = gl::neg(prod); }
+        rc[kk] = gl::add(rc[kk], prod);
+      }
+  }
+  const uint64_t invM = rd.invMi[1][i];
+  uint32_t* y = ka.Y + ((size_t)ct * Rks + i) * N;
+  for (int j = 0; j < N; ++j) y[j] = finish_residue(rc[j], 1, p, b, invM);
+}
+
+// Pre-transform of the keyswitch keys: gamma_k (coefficients, [lw][LQ][n]) ->
+// NTT(gamma_k mod p_i, centred) for every prime of the keyswitch set.
+__global__ void __launch_bounds__(1024) k_evk_prep(RingDev rd, const uint64_t* G, uint64_t* out, int transform) {
+  extern __shared__ uint64_t lds[];
+  const int i = blockIdx.x;
+  const int k = blockIdx.y;
+  const int Rks = rd.set_R[1];
+  const uint64_t p = rd.primes[i];
+  const uint64_t b = rd.barrett[i];
+  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  const int N = rd.n;
+  const uint64_t* g = G + (size_t)k * rd.LQ * N;
+  uint64_t* o = out + ((size_t)k * Rks + i) * N;
+  if (!transform) {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) o[j] = center(crt_residue(g + j, rd.LQ, N, pw, p, b), p);
+    return;
+  }
+  uint64_t x[E];
+  load_crt(x, g, rd.LQ, N, N, rd.logn, pw, p, b);
+  ntt_fwd(x, lds, rd.logn, rd.psi_rev);
+#pragma unroll
+  for (int c = 0; c < E; ++c) o[threadIdx.x * E + c] = x[c];
+}
+
+// ---------------------------------------------------------------- ICRT + epilogues
+
+enum Epilogue { EPI_RAW = 0, EPI_MODQ = 1, EPI_SCALE = 2, EPI_DECRYPT = 3 };
+
+template <int NL>
+__device__ __forceinline__ void mac32(uint64_t* acc, const uint64_t* m, uint64_t y) {
+  // acc += m * y (mod 2^(64 NL)), y < 2^32
+  uint64_t carry = 0;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    uint64_t lo = m[l] * y;
+    uint64_t hi = gl::umulhi(m[l], y);
+    uint64_t s = acc[l] + lo;
+    hi += (s < lo);
+    uint64_t s2 = s + carry;
+    hi += (s2 < s);
+    acc[l] = s2;
+    carry = hi;
+  }
+}
+
+template <int NL>
+__device__ __forceinline__ void sub_n(uint64_t* a, const uint64_t* b) {
+  uint64_t br = 0;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    uint64_t d = a[l] - b[l];
+    uint64_t b1 = a[l] < b[l];
+    uint64_t d2 = d - br;
+    uint64_t b2 = d < br;
+    a[l] = d2;
+    br = b1 | b2;
+  }
+}
+template <int NL>
+__device__ __forceinline__ void add_n(uint64_t* a, const uint64_t* b) {
+  uint64_t c = 0;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    uint64_t s = a[l] + b[l];
+    uint64_t c1 = s < a[l];
+    uint64_t s2 = s + c;
+    uint64_t c2 = s2 < s;
+    a[l] = s2;
+    c = c1 | c2;
+  }
+}
+template <int NL>
+__device__ __forceinline__ void neg_n(uint64_t* a) {
+  uint64_t c = 1;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    uint64_t v = ~a[l] + c;
+    c = (c && v == 0) ? 1 : 0;
+    a[l] = v;
+  }
+}
+// a >= b ?
+template <int NL>
+__device__ __forceinline__ bool geq_n(const uint64_t* a, const uint64_t* b) {
+#pragma unroll
+  for (int l = NL - 1; l >= 0; --l) {
+    if (a[l] != b[l]) return a[l] > b[l];
+  }
+  return true;
+}
+
+// Quotient and remainder of a (LA = LW+1 limbs) by q (LQ limbs) with the
+// generalised Barrett reciprocal mu = floor(2^(64 LA) / q):
+// qhat = floor(a*mu / 2^(64 LA)) is q_true or q_true - 1.
+template <int LQ>
+__device__ void divmod_q(const uint64_t* a, const RingDev& rd, uint64_t* quo, uint64_t* rem) {
+  constexpr int LW = 2 * LQ + 1;
+  constexpr int LA = LW + 1;
+  constexpr int LMU = LW + 2 - LQ;
+  uint64_t prod[LA + LMU];
+#pragma unroll
+  for (int l = 0; l < LA + LMU; ++l) prod[l] = 0;
+  for (int i = 0; i < LA; ++i) {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < LMU; ++k) {
+      uint64_t lo = a[i] * rd.mu[k];
+      uint64_t hi = gl::umulhi(a[i], rd.mu[k]);
+      uint64_t s = prod[i + k] + lo;
+      hi += (s < lo);
+      uint64_t s2 = s + carry;
+      hi += (s2 < s);
+      prod[i + k] = s2;
+      carry = hi;
+    }
+    prod[i + LMU] = carry;
+  }
+#pragma unroll
+  for (int k = 0; k < LMU; ++k) quo[k] = prod[LA + k];
+  // r = a - quo*q over LQ+1 limbs (the true remainder is < 2q)
+  uint64_t qq[LQ + 1];
+#pragma unroll
+  for (int l = 0; l <= LQ; ++l) qq[l] = 0;
+  for (int k = 0; k < LMU; ++k) {
+    uint64_t carry = 0;
+    for (int l = 0; k + l <= LQ; ++l) {
+      uint64_t ql = l < LQ ? rd.q[l] : 0;
+      uint64_t lo = quo[k] * ql;
+      uint64_t hi = gl::umulhi(quo[k], ql);
+      uint64_t s = qq[k + l] + lo;
+      hi += (s < lo);
+      uint64_t s2 = s + carry;
+      hi += (s2 < s);
+      qq[k + l] = s2;
+      carry = hi;
+    }
+  }
+  uint64_t r[LQ + 1];
+#pragma unroll
+  for (int l = 0; l <= LQ; ++l) r[l] = a[l];
+  sub_n<LQ + 1>(r, qq);
+  uint64_t qe[LQ + 1];
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) qe[l] = rd.q[l];
+  qe[LQ] = 0;
+  if (geq_n<LQ + 1>(r, qe)) {
+    sub_n<LQ + 1>(r, qe);
+    uint64_t one[LMU];
+#pragma unroll
+    for (int k = 0; k < LMU; ++k) one[k] = k == 0;
+    add_n<LMU>(quo, one);
+  }
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) rem[l] = r[l];
+}
+
+// 2r > q ?
+template <int LQ>
+__device__ __forceinline__ bool twice_gt_q(const uint64_t* r, const uint64_t* q) {
+  uint64_t d[LQ + 1];
+  uint64_t c = 0;
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) {
+    d[l] = (r[l] << 1) | c;
+    c = r[l] >> 63;
+  }
+  d[LQ] = c;
+  uint64_t qe[LQ + 1];
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) qe[l] = q[l];
+  qe[LQ] = 0;
+  // d > q  <=>  !(q >= d)
+  return !geq_n<LQ + 1>(qe, d);
+}
+
+struct IcrtArgs {
+  const uint32_t* Y;     // [pair][R_S][N]
+  int N;                 // row length of Y
+  int ncoef;             // coefficients reconstructed per pair
+  int set;
+  int mode;              // Epilogue
+  uint64_t* out;         // [pair][out_limbs][out_stride]
+  int out_limbs;
+  size_t out_stride;     // elements between limbs
+  size_t out_pstride;    // elements between pairs
+  int positive;          // 1: result in [0, M) (plain product); 0: centred
+};
+
+template <int LQ>
+__global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
+  constexpr int LW = 2 * LQ + 1;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pair = blockIdx.y;
+  if (j >= ia.ncoef) return;
+  const int RS = rd.set_R[ia.set];
+  const uint32_t* y = ia.Y + (size_t)pair * RS * ia.N + j;
+  const uint64_t* Mi = rd.Mi[ia.set];
+  uint64_t acc[LW];
+#pragma unroll
+  for (int l = 0; l < LW; ++l) acc[l] = 0;
+  uint64_t flo = 0, fhi = 0;
+  for (int i = 0; i < RS; ++i) {
+    uint64_t yi = y[(size_t)i * ia.N];
+    // fixed-point y_i/p_i with 64 fractional bits (y_i < p_i keeps it < 2^64)
+    uint64_t f = yi * rd.barrett[i];
+    flo += f;
+    fhi += (flo < f);
+    mac32<LW>(acc, Mi + (size_t)i * LW, yi);
+  }
+  uint64_t k;
+  if (ia.positive) {
+    // bias covers the truncation error so that floor() never undershoots;
+    // a one-too-large k leaves a negative value that the check below fixes.
+    uint64_t bias = (uint64_t)RS << 41;
+    uint64_t nl = flo + bias;
+    fhi += (nl < flo);
+    k = fhi;
+  } else {
+    k = fhi + (flo >> 63);  // round to nearest: |g| <= M/4
+  }
+  uint64_t kM[LW];
+#pragma unroll
+  for (int l = 0; l < LW; ++l) kM[l] = 0;
+  mac32<LW>(kM, rd.M[ia.set], k);
+  sub_n<LW>(acc, kM);
+  bool neg = acc[LW - 1] >> 63;
+  if (ia.positive) {
+    if (neg) add_n<LW>(acc, rd.M[ia.set]);
+    neg = false;
+  } else if (neg) {
+    neg_n<LW>(acc);
+  }
+  uint64_t* o = ia.out + pair * ia.out_pstride + j;
+  if (ia.mode == EPI_RAW) {
+    for (int l = 0; l < ia.out_limbs; ++l) o[(size_t)l * ia.out_stride] = l < LW ? acc[l] : 0;
+    return;
+  }
+  constexpr int LA = LW + 1;
+  constexpr int LMU = LW + 2 - LQ;
+  uint64_t a[LA], quo[LMU], rem[LQ];
+#pragma unroll
+  for (int l = 0; l < LW; ++l) a[l] = acc[l];
+  a[LW] = 0;
+  if (ia.mode == EPI_MODQ || ia.mode == EPI_DECRYPT) {
+    divmod_q<LQ>(a, rd, quo, rem);
+    bool nz = false;
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) nz |= rem[l] != 0;
+    if (neg && nz) {
+      uint64_t qv[LQ];
+#pragma unroll
+      for (int l = 0; l < LQ; ++l) qv[l] = rd.q[l];
+      sub_n<LQ>(qv, rem);
+#pragma unroll
+      for (int l = 0; l < LQ; ++l) rem[l] = qv[l];
+    }
+    if (ia.mode == EPI_MODQ) {
+#pragma unroll
+      for (int l = 0; l < LQ; ++l) o[(size_t)l * ia.out_stride] = rem[l];
+      return;
+    }
+    // decrypt: m = round(t * [g]_q / q) mod t, [g]_q in [0, q)
+    // (src/yashe.cpp:167-197 on every coefficient)
+#pragma unroll
+    for (int l = 0; l < LA; ++l) a[l] = 0;
+    uint64_t carry = 0;
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) {
+      uint64_t lo = rem[l] * rd.t, hi = gl::umulhi(rem[l], rd.t);
+      uint64_t s = lo + carry;
+      hi += (s < lo);
+      a[l] = s;
+      carry = hi;
+    }
+    a[LQ] = carry;
+    divmod_q<LQ>(a, rd, quo, rem);
+    if (twice_gt_q<LQ>(rem, rd.q)) {
+      uint64_t one[LMU];
+#pragma unroll
+      for (int k2 = 0; k2 < LMU; ++k2) one[k2] = k2 == 0;
+      add_n<LMU>(quo, one);
+    }
+    // quo <= t here
+    o[0] = quo[0] == rd.t ? 0 : quo[0];
+    return;
+  }
+  // EPI_SCALE: round(t * g / q) mod q for the signed g (src/ciphertext.cpp:55-63
+  // semantics applied to t*g: floor division, non-negative remainder, ties down).
+  {
+    uint64_t carry = 0;
+#pragma unroll
+    for (int l = 0; l < LW; ++l) {
+      uint64_t lo = acc[l] * rd.t, hi = gl::umulhi(acc[l], rd.t);
+      uint64_t s = lo + carry;
+      hi += (s < lo);
+      a[l] = s;
+      carry = hi;
+    }
+    a[LW] = carry;
+  }
+  divmod_q<LQ>(a, rd, quo, rem);  // |t g| = quo*q + rem
+  // signed floor quotient Qs and remainder rs in [0, q)
+  bool nz = false;
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) nz |= rem[l] != 0;
+  uint64_t one[LMU];
+#pragma unroll
+  for (int k2 = 0; k2 < LMU; ++k2) one[k2] = k2 == 0;
+  bool qneg = false;
+  if (neg) {
+    qneg = true;  // Qs = -(quo) or -(quo+1)
+    if (nz) {
+      add_n<LMU>(quo, one);
+      uint64_t qv[LQ];
+#pragma unroll
+      for (int l = 0; l < LQ; ++l) qv[l] = rd.q[l];
+      sub_n<LQ>(qv, rem);
+#pragma unroll
+      for (int l = 0; l < LQ; ++l) rem[l] = qv[l];
+    }
+  }
+  if (twice_gt_q<LQ>(rem, rd.q)) {
+    // Qs += 1
+    if (!qneg) {
+      add_n<LMU>(quo, one);
+    } else {
+      sub_n<LMU>(quo, one);  // -(m) + 1 = -(m - 1); m >= 1 here
+    }
+  }
+  // result = Qs mod q
+  uint64_t a2[LA];
+#pragma unroll
+  for (int l = 0; l < LA; ++l) a2[l] = l < LMU ? quo[l] : 0;
+  uint64_t quo2[LMU], r2[LQ];
+  divmod_q<LQ>(a2, rd, quo2, r2);
+  bool nz2 = false;
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) nz2 |= r2[l] != 0;
+  if (qneg && nz2) {
+    uint64_t qv[LQ];
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) qv[l] = rd.q[l];
+    sub_n<LQ>(qv, r2);
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) r2[l] = qv[l];
+  }
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) o[(size_t)l * ia.out_stride] = r2[l];
+}
+
+// ---------------------------------------------------------------- primitives
+
+// CRT primitive: residues r[i][j] = coef_j mod p_i in [0, p_i), layout
+// [R][spacing] like the reference's d_polyCRT (src/polynomial.h:1435).
+__global__ void k_crt(RingDev rd, const uint64_t* A, int lin, int ncoef, size_t lstride, uint64_t* out,
+                      int spacing) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= spacing) return;
+  const uint64_t p = rd.primes[i];
+  uint64_t r = j < ncoef ? crt_residue(A + j, lin, lstride, rd.pow32 + (size_t)i * 2 * rd.LW, p, rd.barrett[i]) : 0;
+  out[(size_t)i * spacing + j] = r;
+}
+
+// y_i = r_i * (M/p_i)^-1 mod p_i (the cuPreICRT inner step, src/cuda_bn.cu:875-879).
+__global__ void k_residue_to_y(RingDev rd, const uint64_t* res, int spacing, uint32_t* Y, int set) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= spacing) return;
+  const uint64_t p = rd.primes[i];
+  uint64_t r = res[(size_t)i * spacing + j];
+  r = mod_p(r, p, rd.barrett[i]);
+  Y[(size_t)i * spacing + j] = (uint32_t)mod_p(r * rd.invMi[set][i], p, rd.barrett[i]);
+}
+
+// Residue-domain add/sub mod p_i (the reference's polynomialAddSub,
+// src/cuda_functions.cu:99-126, with the mod p_i it omits).
+__global__ void k_residue_addsub(RingDev rd, uint64_t* C, const uint64_t* A, const uint64_t* B, int spacing, int sub) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= spacing) return;
+  const uint64_t p = rd.primes[i];
+  size_t o = (size_t)i * spacing + j;
+  uint64_t a = A[o], b = B[o];
+  uint64_t r;
+  if (sub) r = a >= b ? a - b : a + p - b;
+  else { r = a + b; if (r >= p) r -= p; }
+  C[o] = r;
+}
+
+// Pointwise Goldilocks product (polynomialNTTMul, src/cuda_functions.cu:460-474).
+__global__ void k_pointwise(uint64_t* A, const uint64_t* B, size_t count) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) A[t] = gl::mul(A[t], B[t]);
+}
+
+// Standalone cyclic transform over global data, natural order in and out
+// (callNTT, src/cuda_functions.cu:447-458, with N^-1 scaling on the inverse).
+__device__ __forceinline__ int bitrev(int v, int bits) { return (int)(__brev((unsigned)v) >> (32 - bits)); }
+
+__global__ void __launch_bounds__(1024) k_ntt_cyclic(uint64_t* data, int logN, const uint64_t* T, const uint64_t* Ti,
+                                                     uint64_t Ninv, int inverse) {
+  extern __shared__ uint64_t lds[];
+  const int N = 1 << logN;
+  uint64_t* d = data + (size_t)blockIdx.x * N;
+  const int g = threadIdx.x;
+  const int stride = N >> 4;
+  uint64_t x[E];
+  if (!inverse) {
+#pragma unroll
+    for (int c = 0; c < E; ++c) x[c] = d[g + c * stride];
+    ntt_fwd(x, lds, logN, T);
+    // bit-reversed -> natural
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < E; ++c) lds[lds_idx(g * E + c)] = x[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int k = g + c * stride;
+      d[k] = lds[lds_idx(bitrev(k, logN))];
+    }
+  } else {
+    // natural -> bit-reversed into the b0 = 0 layout
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      int k = g + c * stride;
+      lds[lds_idx(bitrev(k, logN))] = d[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < E; ++c) x[c] = lds[lds_idx(g * E + c)];
+    __syncthreads();
+    ntt_inv(x, lds, logN, Ti);
+#pragma unroll
+    for (int c = 0; c < E; ++c) d[g + c * stride] = gl::mul(x[c], Ninv);
+  }
+}
+
+// Coefficient-domain add/sub mod q for ring elements in [0, q) (homomorphic
+// addition on resident ciphertexts).
+template <int LQ>
+__global__ void k_addsub_q(RingDev rd, const uint64_t* A, const uint64_t* B, uint64_t* C, int n, size_t count, int sub) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  size_t pr = t / n, j = t % n;
+  const uint64_t* a = A + pr * LQ * n + j;
+  const uint64_t* b = B + pr * LQ * n + j;
+  uint64_t* c = C + pr * LQ * n + j;
+  uint64_t x[LQ + 1], y[LQ + 1], qe[LQ + 1];
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) { x[l] = a[(size_t)l * n]; y[l] = b[(size_t)l * n]; qe[l] = rd.q[l]; }
+  x[LQ] = y[LQ] = qe[LQ] = 0;
+  if (!sub) {
+    add_n<LQ + 1>(x, y);
+    if (geq_n<LQ + 1>(x, qe)) sub_n<LQ + 1>(x, qe);
+  } else {
+    if (!geq_n<LQ + 1>(x, y)) add_n<LQ + 1>(x, qe);
+    sub_n<LQ + 1>(x, y);
+  }
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) c[(size_t)l * n] = x[l];
+}
+
+// Reduction mod (x^n + 1, q) of a length-2n coefficient vector
+// (Polynomial::reduce, src/cuda_functions.cu:1061-1078, and the % q of
+// src/polynomial.h:593-610): out_j = (c_j - c_{j+n}) mod q in [0, q).
+// Input [pair][lin][2n], lin <= LW; output [pair][LQ][n].
+template <int LQ>
+__global__ void k_reduce(RingDev rd, const uint64_t* C, int lin, uint64_t* out) {
+  constexpr int LW = 2 * LQ + 1;
+  constexpr int LA = LW + 1;
+  constexpr int LMU = LW + 2 - LQ;
+  const int n = rd.n;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pair = blockIdx.y;
+  if (j >= n) return;
+  const uint64_t* c = C + (size_t)pair * lin * 2 * n;
+  uint64_t a[LA], b[LA];
+#pragma unroll
+  for (int l = 0; l < LA; ++l) {
+    a[l] = l < lin ? c[(size_t)l * 2 * n + j] : 0;
+    b[l] = l < lin ? c[(size_t)l * 2 * n + j + n] : 0;
+  }
+  sub_n<LA>(a, b);
+  bool neg = a[LA - 1] >> 63;
+  if (neg) neg_n<LA>(a);
+  uint64_t quo[LMU], rem[LQ];
+  divmod_q<LQ>(a, rd, quo, rem);
+  bool nz = false;
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) nz |= rem[l] != 0;
+  if (neg && nz) {
+    uint64_t qv[LQ];
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) qv[l] = rd.q[l];
+    sub_n<LQ>(qv, rem);
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) rem[l] = qv[l];
+  }
+  uint64_t* o = out + (size_t)pair * LQ * n + j;
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) o[(size_t)l * n] = rem[l];
+}
+
+}  // namespace yhk

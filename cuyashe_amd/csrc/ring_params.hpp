@@ -1,0 +1,244 @@
+// Host-side construction of one ring's constant tables.
+//
+// Reference counterparts:
+//  * prime choice      src/polynomial.h:898-955 (gen_crt_primes: primes taken
+//                      in descending order until M > 2*degree*q^2) over the
+//                      bucket at src/polynomial.cpp:22;
+//  * ICRT constants    src/polynomial.h:939-943 (M/p_i and (M/p_i)^-1 mod p_i),
+//                      sent to the device by src/cuda_bn.cu:1047-1147;
+//  * twiddles          src/cuda_functions.cu:878-918 (omega = 7^((P-1)/N)).
+//
+// Differences that keep every result identical while fixing the reference's
+// exactness holes (its 29-bit primes overflow the Goldilocks convolution for
+// n > 64):
+//  * The prime size is the largest that keeps the Goldilocks convolution exact:
+//    with residues centred in (-p/2, p/2], K*n*((p-1)/2)^2 <= (P-1)/2, where K
+//    is the number of products summed before the inverse transform (1 for a
+//    ring product, l_w for the keyswitch multiply-accumulate).
+//  * The prime count makes M >= 4*B, B the largest |coefficient| the operation
+//    can produce, so the ICRT quotient k = round(sum y_i/p_i) can be found
+//    with 64-bit fixed point and no correction step.
+#pragma once
+#include <stdint.h>
+
+#include <stdexcept>
+#include <vector>
+
+#include "bigint.hpp"
+#include "gl64.hpp"
+
+namespace yh {
+
+inline bool is_prime_u64(uint64_t x) {
+  if (x < 2) return false;
+  for (uint64_t p : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 17ull, 19ull, 23ull, 29ull, 31ull, 37ull}) {
+    if (x % p == 0) return x == p;
+  }
+  uint64_t d = x - 1;
+  int s = 0;
+  while ((d & 1) == 0) { d >>= 1; ++s; }
+  auto mulm = [x](uint64_t a, uint64_t b) { return (uint64_t)((unsigned __int128)a * b % x); };
+  auto powm = [&](uint64_t b, uint64_t e) {
+    uint64_t r = 1;
+    b %= x;
+    while (e) { if (e & 1) r = mulm(r, b); b = mulm(b, b); e >>= 1; }
+    return r;
+  };
+  for (uint64_t a : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 17ull, 19ull, 23ull, 29ull, 31ull, 37ull}) {
+    uint64_t v = powm(a, d);
+    if (v == 1 || v == x - 1) continue;
+    bool comp = true;
+    for (int r = 1; r < s; ++r) {
+      v = mulm(v, v);
+      if (v == x - 1) { comp = false; break; }
+    }
+    if (comp) return false;
+  }
+  return true;
+}
+
+inline uint64_t inv_mod_u64(uint64_t a, uint64_t m) {
+  int64_t t = 0, nt = 1;
+  int64_t r = (int64_t)m, nr = (int64_t)(a % m);
+  while (nr) {
+    int64_t q = r / nr;
+    int64_t tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  if (r != 1) throw std::domain_error("inverse does not exist");
+  if (t < 0) t += (int64_t)m;
+  return (uint64_t)t;
+}
+
+// Largest prime size allowed by the exactness bound of the Goldilocks
+// convolution: K * n * ((p-1)/2)^2 <= (P-1)/2.
+// Primes are also kept below 2^28 so that the CRT word sums of the device
+// (products of a 32-bit word and a residue, eight per partial sum) fit 64 bits.
+inline uint64_t max_prime_for(int n, int K) {
+  unsigned __int128 lim = (unsigned __int128)((gl::P - 1) / 2);
+  uint64_t lo = 3, hi = (1ull << 28) - 1;
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo + 1) / 2;
+    unsigned __int128 h = (mid - 1) / 2;
+    unsigned __int128 v = (unsigned __int128)K * (unsigned __int128)n * h * h;
+    if (v <= lim) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Descending primes <= pmax until their product reaches `need`.
+inline std::vector<uint64_t> take_primes(uint64_t pmax, const BigInt& need) {
+  std::vector<uint64_t> out;
+  BigInt M(1);
+  uint64_t c = pmax | 1;
+  if (c > pmax) c -= 2;
+  while (M < need) {
+    while (!is_prime_u64(c)) c -= 2;
+    out.push_back(c);
+    M *= BigInt(c);
+    c -= 2;
+    if (out.size() > 255) throw std::runtime_error("too many CRT primes");
+  }
+  return out;
+}
+
+// One reconstruction set: a prefix of the ring primes.
+struct IcrtSet {
+  int R = 0;
+  BigInt M;
+  std::vector<uint64_t> Mi_limbs;   // [R][LW]
+  std::vector<uint64_t> invMi;      // [R]  (M/p_i)^-1 mod p_i
+  std::vector<uint64_t> M_limbs;    // [LW]
+};
+
+struct RingParams {
+  int n = 0, logn = 0;
+  BigInt q;
+  int LQ = 0;       // limbs of q
+  int LW = 0;       // working limbs of the reconstruction (2*LQ + 1)
+  uint64_t t = 0;   // plaintext modulus
+  int w_bits = 0;   // keyswitch base 2^w_bits (0: no keyswitch)
+  int lw = 0;       // number of keyswitch digits
+  uint64_t pmax = 0;
+  std::vector<uint64_t> primes;     // R primes, descending
+  std::vector<uint64_t> barrett;    // floor(2^64 / p_i)
+  std::vector<uint64_t> fixinv;     // floor(2^64 / p_i) used by the fixed-point quotient
+  std::vector<uint64_t> pow32;      // [R][2*LW]: 2^(32k) mod p_i
+  IcrtSet full, ks;
+  std::vector<uint64_t> q_limbs;    // [LQ]
+  std::vector<uint64_t> mu_limbs;   // [LW+2-LQ]: floor(2^(64*(LW+1)) / q)
+  std::vector<uint64_t> psi_rev, psiinv_rev;  // [n] negacyclic twiddles
+  uint64_t n_inv = 0;
+
+  static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
+
+  void build_set(IcrtSet& s, int R) {
+    s.R = R;
+    s.M = BigInt(1);
+    for (int i = 0; i < R; ++i) s.M *= BigInt(primes[i]);
+    if (s.M.bits() + 2 > 64 * LW) throw std::runtime_error("CRT modulus exceeds working width");
+    s.Mi_limbs.assign((size_t)R * LW, 0);
+    s.invMi.assign(R, 0);
+    s.M_limbs.assign(LW, 0);
+    s.M.to_limbs(s.M_limbs.data(), LW);
+    for (int i = 0; i < R; ++i) {
+      BigInt Mi = s.M / BigInt(primes[i]);
+      Mi.to_limbs(&s.Mi_limbs[(size_t)i * LW], LW);
+      s.invMi[i] = inv_mod_u64(Mi.mod_u64(primes[i]), primes[i]);
+    }
+  }
+
+  // n: ring degree (power of two, x^n + 1); q: modulus (odd, > 1);
+  // t: plaintext modulus (0 allowed when unused); w_bits: keyswitch digit
+  // width in bits (0: ring has no keyswitch).
+  RingParams(int n_, const BigInt& q_, uint64_t t_, int w_bits_) {
+    if (n_ < 2 || (n_ & (n_ - 1))) throw std::invalid_argument("n must be a power of two >= 2");
+    if (n_ > 16384) throw std::invalid_argument("n > 16384 is not supported");
+    if (q_ <= BigInt(1)) throw std::invalid_argument("q must be > 1");
+    if (w_bits_ < 0 || w_bits_ > 32) throw std::invalid_argument("w_bits must be in [0, 32]");
+    n = n_;
+    logn = ilog2(n);
+    q = q_;
+    t = t_;
+    w_bits = w_bits_;
+    LQ = (q.bits() + 63) / 64;
+    if (LQ > 8) throw std::invalid_argument("q wider than 512 bits is not supported");
+    LW = 2 * LQ + 1;
+    lw = w_bits ? (q.bits() + w_bits - 1) / w_bits : 0;
+    int K = lw > 1 ? lw : 1;
+    pmax = max_prime_for(n, K);
+    BigInt qm1 = q - BigInt(1);
+    // Ring products: |c_j| <= n (q-1)^2 for operands in [0, q).
+    BigInt B_mul = BigInt(n) * qm1 * qm1;
+    // Keyswitch: |c_j| <= l_w n (w-1)(q-1).
+    BigInt B_ks = w_bits ? BigInt(lw) * BigInt(n) * (BigInt::pow2(w_bits) - BigInt(1)) * qm1 : BigInt(0);
+    BigInt need = BigInt(4) * (B_mul > B_ks ? B_mul : B_ks);
+    if (need < BigInt(4)) need = BigInt(4);
+    primes = take_primes(pmax, need);
+    int R = (int)primes.size();
+    barrett.resize(R);
+    fixinv.resize(R);
+    pow32.assign((size_t)R * 2 * LW, 0);
+    for (int i = 0; i < R; ++i) {
+      uint64_t p = primes[i];
+      barrett[i] = (uint64_t)((((unsigned __int128)1) << 64) / p);
+      fixinv[i] = barrett[i];
+      uint64_t v = 1;
+      for (int k = 0; k < 2 * LW; ++k) {
+        pow32[(size_t)i * 2 * LW + k] = v;
+        v = (uint64_t)(((unsigned __int128)v << 32) % p);
+      }
+    }
+    build_set(full, R);
+    int Rks = 0;
+    if (w_bits) {
+      BigInt need_ks = BigInt(4) * B_ks, M(1);
+      while (M < need_ks) M *= BigInt(primes[Rks++]);
+    }
+    build_set(ks, Rks > 0 ? Rks : R);
+    q_limbs.assign(LQ, 0);
+    q.to_limbs(q_limbs.data(), LQ);
+    int LMU = LW + 2 - LQ;
+    BigInt mu = BigInt::pow2(64 * (LW + 1)) / q;
+    mu_limbs.assign(LMU, 0);
+    mu.to_limbs(mu_limbs.data(), LMU);
+    // Negacyclic twiddles: psi a primitive 2n-th root of unity mod P built
+    // from the reference generator 7; psi_rev[k] = psi^bitrev(k).
+    uint64_t psi = gl::pow(7, (gl::P - 1) / (2 * (uint64_t)n));
+    uint64_t psiinv = gl::inv(psi);
+    psi_rev.resize(n);
+    psiinv_rev.resize(n);
+    for (int k = 0; k < n; ++k) {
+      int r = 0;
+      for (int b = 0; b < logn; ++b) if (k >> b & 1) r |= 1 << (logn - 1 - b);
+      psi_rev[k] = gl::pow(psi, (uint64_t)r);
+      psiinv_rev[k] = gl::pow(psiinv, (uint64_t)r);
+    }
+    n_inv = gl::inv((uint64_t)n);
+  }
+};
+
+// Twiddle tables of the cyclic length-N transform in the bit-reversed layout
+// consumed by the Cooley-Tukey / Gentleman-Sande passes:
+// T[2^s + i] = omega_{2^(s+1)}^{bitrev_s(i)}, omega_N = 7^((P-1)/N).
+inline void cyclic_tables(int N, std::vector<uint64_t>& fwd, std::vector<uint64_t>& inv) {
+  int logN = RingParams::ilog2(N);
+  uint64_t w = gl::pow(7, (gl::P - 1) / (uint64_t)N);
+  uint64_t wi = gl::inv(w);
+  fwd.assign(N, 1);
+  inv.assign(N, 1);
+  for (int s = 0; s < logN; ++s) {
+    int m = 1 << s;
+    // primitive 2m-th root: omega_N^(N/(2m))
+    uint64_t r = gl::pow(w, (uint64_t)(N / (2 * m)));
+    uint64_t ri = gl::pow(wi, (uint64_t)(N / (2 * m)));
+    for (int i = 0; i < m; ++i) {
+      int b = 0;
+      for (int k = 0; k < s; ++k) if (i >> k & 1) b |= 1 << (s - 1 - k);
+      fwd[m + i] = gl::pow(r, (uint64_t)b);
+      inv[m + i] = gl::pow(ri, (uint64_t)b);
+    }
+  }
+}
+
+}  // namespace yh

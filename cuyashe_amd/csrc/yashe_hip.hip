@@ -1,0 +1,640 @@
+// C-ABI implementation of the YASHE polynomial-ring hot path for gfx950.
+// Declarations and reference citations: include/yashe_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/yashe_hip.h"
+#include "bigint.hpp"
+#include "kernels.hpp"
+#include "ring_params.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define YH_HIP(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(YH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define YH_TRY_BEGIN try {
+#define YH_TRY_END                                                   \
+  }                                                                  \
+  catch (const std::invalid_argument& e) { return fail(YH_ERR_ARG, e.what()); } \
+  catch (const std::exception& e) { return fail(YH_ERR_INTERNAL, e.what()); }
+
+struct DevTables {
+  uint64_t* fwd = nullptr;
+  uint64_t* inv = nullptr;
+};
+
+}  // namespace
+
+struct yh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<int, DevTables> cyclic;  // N -> twiddles of the cyclic transform
+  std::mutex mu;
+};
+
+struct yh_ring {
+  yh_ctx* ctx = nullptr;
+  std::unique_ptr<yh::RingParams> P;
+  uint64_t* d_tables = nullptr;
+  yhk::RingDev rd{};
+  uint32_t* d_y = nullptr;     // ICRT input scratch
+  size_t y_bytes = 0;
+  uint64_t* d_tmp = nullptr;   // ring-element scratch (mul+keyswitch intermediate)
+  size_t tmp_bytes = 0;
+};
+
+struct yh_evk {
+  yh_ring* ring = nullptr;
+  uint64_t* d = nullptr;  // [lw][R_ks][n]: transform domain (n >= 16) or centred residues
+};
+
+namespace {
+
+int set_dev(yh_ctx* ctx) {
+  YH_HIP(hipSetDevice(ctx->device));
+  return YH_OK;
+}
+
+int ensure(yh_ctx* ctx, void** buf, size_t* have, size_t need) {
+  if (*have >= need) return YH_OK;
+  if (*buf) {
+    YH_HIP(hipStreamSynchronize(ctx->stream));
+    YH_HIP(hipFree(*buf));
+    *buf = nullptr;
+    *have = 0;
+  }
+  YH_HIP(hipMalloc(buf, need));
+  *have = need;
+  return YH_OK;
+}
+
+int get_cyclic(yh_ctx* ctx, int N, DevTables* out) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  auto it = ctx->cyclic.find(N);
+  if (it != ctx->cyclic.end()) { *out = it->second; return YH_OK; }
+  std::vector<uint64_t> f, iv;
+  yh::cyclic_tables(N, f, iv);
+  DevTables t;
+  YH_HIP(hipMalloc(&t.fwd, N * sizeof(uint64_t)));
+  YH_HIP(hipMalloc(&t.inv, N * sizeof(uint64_t)));
+  YH_HIP(hipMemcpy(t.fwd, f.data(), N * sizeof(uint64_t), hipMemcpyHostToDevice));
+  YH_HIP(hipMemcpy(t.inv, iv.data(), N * sizeof(uint64_t), hipMemcpyHostToDevice));
+  ctx->cyclic[N] = t;
+  *out = t;
+  return YH_OK;
+}
+
+int ilog2i(long v) { int l = 0; while ((1L << l) < v) ++l; return l; }
+
+size_t lds_bytes(int N) { return (size_t)(N + N / 16) * sizeof(uint64_t); }
+
+// Launch the fused product kernel for `batch` pairs on N-point transforms.
+int launch_prod(yh_ring* r, const yhk::ProdArgs& pa, int N, int batch, bool negacyclic) {
+  hipStream_t s = r->ctx->stream;
+  int R = r->rd.set_R[pa.set];
+  if (N >= 16) {
+    dim3 grid(R, batch), block(N / 16);
+    hipLaunchKernelGGL(yhk::k_prod, grid, block, lds_bytes(N), s, r->rd, pa);
+  } else {
+    dim3 grid(R, batch), block(64);
+    hipLaunchKernelGGL(yhk::k_prod_small, grid, block, 0, s, r->rd, pa, negacyclic ? 1 : 0);
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+template <int LQ>
+void launch_icrt_t(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
+  dim3 block(256), grid((ia.ncoef + 255) / 256, batch);
+  hipLaunchKernelGGL(yhk::k_icrt<LQ>, grid, block, 0, r->ctx->stream, r->rd, ia);
+}
+
+int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
+  switch (r->rd.LQ) {
+    case 1: launch_icrt_t<1>(r, ia, batch); break;
+    case 2: launch_icrt_t<2>(r, ia, batch); break;
+    case 3: launch_icrt_t<3>(r, ia, batch); break;
+    case 4: launch_icrt_t<4>(r, ia, batch); break;
+    case 5: launch_icrt_t<5>(r, ia, batch); break;
+    case 6: launch_icrt_t<6>(r, ia, batch); break;
+    case 7: launch_icrt_t<7>(r, ia, batch); break;
+    case 8: launch_icrt_t<8>(r, ia, batch); break;
+    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+template <int LQ>
+void launch_addsub_t(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* c, int batch, int sub) {
+  size_t count = (size_t)batch * r->rd.n;
+  dim3 block(256), grid((unsigned)((count + 255) / 256));
+  hipLaunchKernelGGL(yhk::k_addsub_q<LQ>, grid, block, 0, r->ctx->stream, r->rd, a, b, c, r->rd.n, count, sub);
+}
+
+yhk::IcrtArgs icrt_args(yh_ring* r, int N, int ncoef, int set, int mode, uint64_t* out, int out_limbs,
+                        size_t out_stride, size_t out_pstride, int positive) {
+  yhk::IcrtArgs ia{};
+  ia.Y = r->d_y;
+  ia.N = N;
+  ia.ncoef = ncoef;
+  ia.set = set;
+  ia.mode = mode;
+  ia.out = out;
+  ia.out_limbs = out_limbs;
+  ia.out_stride = out_stride;
+  ia.out_pstride = out_pstride;
+  ia.positive = positive;
+  return ia;
+}
+
+int ring_product(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* out, int batch, int mode) {
+  const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
+  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  if (st) return st;
+  yhk::ProdArgs pa{};
+  pa.A = a; pa.B = b;
+  pa.lin_a = pa.lin_b = LQ;
+  pa.ncoef_a = pa.ncoef_b = n;
+  pa.lstride_a = pa.lstride_b = n;
+  pa.ostride_a = pa.ostride_b = (size_t)LQ * n;
+  pa.Y = r->d_y;
+  pa.logN = r->rd.logn;
+  pa.T = r->rd.psi_rev;
+  pa.Ti = r->rd.psiinv_rev;
+  pa.n_inv = r->rd.n_inv;
+  pa.set = 0;
+  st = launch_prod(r, pa, n, batch, true);
+  if (st) return st;
+  return launch_icrt(r, icrt_args(r, n, n, 0, mode, out, LQ, n, (size_t)LQ * n, 0), batch);
+}
+
+template <int LQ>
+void launch_reduce_t(yh_ring* r, const uint64_t* c, int lin, uint64_t* out, int batch) {
+  dim3 block(256), grid((r->rd.n + 255) / 256, batch);
+  hipLaunchKernelGGL(yhk::k_reduce<LQ>, grid, block, 0, r->ctx->stream, r->rd, c, lin, out);
+}
+
+int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out, int batch) {
+  const int n = r->rd.n, LQ = r->rd.LQ, Rks = r->rd.set_R[1];
+  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)batch * Rks * n * sizeof(uint32_t));
+  if (st) return st;
+  yhk::KsArgs ka{};
+  ka.C = c;
+  ka.ostride = (size_t)LQ * n;
+  ka.evk = k->d;
+  ka.Y = r->d_y;
+  dim3 grid(Rks, batch);
+  if (n >= 16) {
+    hipLaunchKernelGGL(yhk::k_keyswitch, grid, dim3(n / 16), lds_bytes(n), r->ctx->stream, r->rd, ka);
+  } else {
+    hipLaunchKernelGGL(yhk::k_keyswitch_small, grid, dim3(64), 0, r->ctx->stream, r->rd, ka, k->d);
+  }
+  YH_HIP(hipGetLastError());
+  return launch_icrt(r, icrt_args(r, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0), batch);
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* yh_last_error(void) { return g_err.c_str(); }
+int yh_version(void) { return 100; }
+
+int yh_ctx_create(int device, yh_ctx** out) {
+  if (!out) return fail(YH_ERR_ARG, "null out");
+  int count = 0;
+  YH_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(YH_ERR_ARG, "no such device");
+  YH_HIP(hipSetDevice(device));
+  // Transforms keep a whole length-16384 vector (+1/16 padding) in LDS:
+  // 139264 bytes, within gfx950's 160 KiB per workgroup.
+  const int max_lds = (int)lds_bytes(16384);
+  YH_HIP(hipFuncSetAttribute((const void*)yhk::k_prod, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  YH_HIP(hipFuncSetAttribute((const void*)yhk::k_keyswitch, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  YH_HIP(hipFuncSetAttribute((const void*)yhk::k_evk_prep, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  YH_HIP(hipFuncSetAttribute((const void*)yhk::k_ntt_cyclic, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  auto* c = new yh_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return fail(YH_ERR_HIP, hipGetErrorString(e)); }
+  *out = c;
+  return YH_OK;
+}
+
+int yh_ctx_destroy(yh_ctx* ctx) {
+  if (!ctx) return YH_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->cyclic) { (void)hipFree(kv.second.fwd); (void)hipFree(kv.second.inv); }
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return YH_OK;
+}
+
+void* yh_ctx_stream(yh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int yh_ctx_sync(yh_ctx* ctx) {
+  if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (int s = set_dev(ctx)) return s;
+  YH_HIP(hipStreamSynchronize(ctx->stream));
+  return YH_OK;
+}
+
+int yh_malloc(yh_ctx* ctx, size_t bytes, void** dptr) {
+  if (!ctx || !dptr) return fail(YH_ERR_ARG, "null argument");
+  if (int s = set_dev(ctx)) return s;
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+  if (e == hipErrorOutOfMemory) return fail(YH_ERR_NOMEM, "out of device memory");
+  if (e != hipSuccess) return fail(YH_ERR_HIP, hipGetErrorString(e));
+  return YH_OK;
+}
+
+int yh_free(yh_ctx* ctx, void* dptr) {
+  if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (!dptr) return YH_OK;
+  if (int s = set_dev(ctx)) return s;
+  YH_HIP(hipStreamSynchronize(ctx->stream));
+  YH_HIP(hipFree(dptr));
+  return YH_OK;
+}
+
+static int memcpy_kind(yh_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (bytes == 0) return YH_OK;
+  if (!dst || !src) return fail(YH_ERR_ARG, "null pointer");
+  if (int s = set_dev(ctx)) return s;
+  YH_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+  YH_HIP(hipStreamSynchronize(ctx->stream));
+  return YH_OK;
+}
+int yh_memcpy_htod(yh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return memcpy_kind(ctx, dst, src, bytes, hipMemcpyHostToDevice);
+}
+int yh_memcpy_dtoh(yh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return memcpy_kind(ctx, dst, src, bytes, hipMemcpyDeviceToHost);
+}
+int yh_memcpy_dtod(yh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return memcpy_kind(ctx, dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+int yh_memset(yh_ctx* ctx, void* dst, int value, size_t bytes) {
+  if (!ctx || !dst) return fail(YH_ERR_ARG, "null argument");
+  if (int s = set_dev(ctx)) return s;
+  YH_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  return YH_OK;
+}
+
+int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, yh_ring** out) {
+  if (!ctx || !q || !out || q_limbs <= 0) return fail(YH_ERR_ARG, "null or empty argument");
+  YH_TRY_BEGIN
+  if (int s = set_dev(ctx)) return s;
+  auto r = std::make_unique<yh_ring>();
+  r->ctx = ctx;
+  r->P = std::make_unique<yh::RingParams>(n, yh::BigInt::from_limbs(q, q_limbs), t, w_bits);
+  const yh::RingParams& P = *r->P;
+  // one table buffer: pack every array, remember offsets
+  std::vector<uint64_t> h;
+  auto put = [&h](const std::vector<uint64_t>& v) { size_t o = h.size(); h.insert(h.end(), v.begin(), v.end()); return o; };
+  size_t o_psi = put(P.psi_rev), o_psii = put(P.psiinv_rev), o_pr = put(P.primes), o_b = put(P.barrett);
+  size_t o_pw = put(P.pow32), o_q = put(P.q_limbs), o_mu = put(P.mu_limbs);
+  size_t o_mi0 = put(P.full.Mi_limbs), o_inv0 = put(P.full.invMi), o_m0 = put(P.full.M_limbs);
+  size_t o_mi1 = put(P.ks.Mi_limbs), o_inv1 = put(P.ks.invMi), o_m1 = put(P.ks.M_limbs);
+  YH_HIP(hipMalloc(&r->d_tables, h.size() * sizeof(uint64_t)));
+  YH_HIP(hipMemcpy(r->d_tables, h.data(), h.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  uint64_t* d = r->d_tables;
+  yhk::RingDev& rd = r->rd;
+  rd.n = P.n; rd.logn = P.logn; rd.LQ = P.LQ; rd.LW = P.LW;
+  rd.R = (int)P.primes.size(); rd.w_bits = P.w_bits; rd.lw = P.lw;
+  rd.t = P.t; rd.n_inv = P.n_inv;
+  rd.psi_rev = d + o_psi; rd.psiinv_rev = d + o_psii; rd.primes = d + o_pr; rd.barrett = d + o_b;
+  rd.pow32 = d + o_pw; rd.q = d + o_q; rd.mu = d + o_mu;
+  rd.set_R[0] = P.full.R; rd.Mi[0] = d + o_mi0; rd.invMi[0] = d + o_inv0; rd.M[0] = d + o_m0;
+  rd.set_R[1] = P.ks.R; rd.Mi[1] = d + o_mi1; rd.invMi[1] = d + o_inv1; rd.M[1] = d + o_m1;
+  *out = r.release();
+  return YH_OK;
+  YH_TRY_END
+}
+
+int yh_ring_destroy(yh_ring* r) {
+  if (!r) return YH_OK;
+  (void)hipSetDevice(r->ctx->device);
+  (void)hipStreamSynchronize(r->ctx->stream);
+  (void)hipFree(r->d_tables);
+  if (r->d_y) (void)hipFree(r->d_y);
+  if (r->d_tmp) (void)hipFree(r->d_tmp);
+  delete r;
+  return YH_OK;
+}
+
+static void fill_info(const yh::RingParams& P, yh_ring_info* info) {
+  info->n = P.n;
+  info->q_limbs = P.LQ;
+  info->q_bits = P.q.bits();
+  info->R = (int)P.primes.size();
+  info->R_ks = P.w_bits ? P.ks.R : 0;
+  info->w_bits = P.w_bits;
+  info->lw = P.lw;
+  info->work_limbs = P.LW;
+  info->m_bits = P.full.M.bits();
+  info->m_ks_bits = P.ks.M.bits();
+  info->t = P.t;
+  info->prime_max = P.pmax;
+}
+
+int yh_ring_get_info(const yh_ring* r, yh_ring_info* info) {
+  if (!r || !info) return fail(YH_ERR_ARG, "null argument");
+  fill_info(*r->P, info);
+  return YH_OK;
+}
+
+int yh_ring_plan(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, yh_ring_info* info,
+                 uint64_t* primes, int cap) {
+  if (!q || !info || q_limbs <= 0) return fail(YH_ERR_ARG, "null or empty argument");
+  YH_TRY_BEGIN
+  yh::RingParams P(n, yh::BigInt::from_limbs(q, q_limbs), t, w_bits);
+  fill_info(P, info);
+  if (primes) {
+    if (cap < (int)P.primes.size()) return fail(YH_ERR_ARG, "capacity too small");
+    for (size_t i = 0; i < P.primes.size(); ++i) primes[i] = P.primes[i];
+  }
+  return YH_OK;
+  YH_TRY_END
+}
+
+int yh_ring_get_primes(const yh_ring* r, uint64_t* out, int cap) {
+  if (!r || !out) return fail(YH_ERR_ARG, "null argument");
+  int R = (int)r->P->primes.size();
+  if (cap < R) return fail(YH_ERR_ARG, "capacity too small");
+  for (int i = 0; i < R; ++i) out[i] = r->P->primes[i];
+  return YH_OK;
+}
+
+int yh_ring_get_modulus(const yh_ring* r, int set, uint64_t* out, int cap) {
+  if (!r || !out || (set != 0 && set != 1)) return fail(YH_ERR_ARG, "bad argument");
+  const yh::IcrtSet& s = set ? r->P->ks : r->P->full;
+  if (cap < r->P->LW) return fail(YH_ERR_ARG, "capacity too small");
+  for (int l = 0; l < r->P->LW; ++l) out[l] = s.M_limbs[l];
+  return YH_OK;
+}
+
+int yh_crt(yh_ring* r, const uint64_t* d_coefs, int limbs, int ncoef, int spacing, uint64_t* d_res) {
+  if (!r || !d_res || (ncoef > 0 && !d_coefs)) return fail(YH_ERR_ARG, "null argument");
+  if (limbs < 1 || limbs > r->rd.LW) return fail(YH_ERR_ARG, "limbs out of range");
+  if (ncoef < 0 || spacing < ncoef || spacing <= 0) return fail(YH_ERR_ARG, "bad spacing");
+  if (int s = set_dev(r->ctx)) return s;
+  dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
+  hipLaunchKernelGGL(yhk::k_crt, grid, block, 0, r->ctx->stream, r->rd, d_coefs, limbs, ncoef, (size_t)ncoef,
+                     d_res, spacing);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int yh_icrt(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, uint64_t* d_coefs, int out_limbs) {
+  if (!r || !d_res || !d_coefs) return fail(YH_ERR_ARG, "null argument");
+  if (ncoef <= 0 || ncoef > spacing || out_limbs < 1) return fail(YH_ERR_ARG, "bad sizes");
+  if (int s = set_dev(r->ctx)) return s;
+  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)r->rd.R * spacing * sizeof(uint32_t));
+  if (st) return st;
+  dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
+  hipLaunchKernelGGL(yhk::k_residue_to_y, grid, block, 0, r->ctx->stream, r->rd, d_res, spacing, r->d_y, 0);
+  YH_HIP(hipGetLastError());
+  return launch_icrt(r, icrt_args(r, spacing, ncoef, 0, yhk::EPI_RAW, d_coefs, out_limbs, ncoef, 0, 1), 1);
+}
+
+int yh_ntt(yh_ctx* ctx, uint64_t* d_data, int N, int count, int direction) {
+  if (!ctx || !d_data) return fail(YH_ERR_ARG, "null argument");
+  if (N < 16 || N > 16384 || (N & (N - 1))) return fail(YH_ERR_UNSUPPORTED, "N must be a power of two in [16, 16384]");
+  if (count <= 0) return fail(YH_ERR_ARG, "count must be positive");
+  if (int s = set_dev(ctx)) return s;
+  DevTables t;
+  if (int s = get_cyclic(ctx, N, &t)) return s;
+  uint64_t Ninv = gl::inv((uint64_t)N);
+  hipLaunchKernelGGL(yhk::k_ntt_cyclic, dim3(count), dim3(N / 16), lds_bytes(N), ctx->stream, d_data, ilog2i(N),
+                     t.fwd, t.inv, Ninv, direction == YH_INVERSE ? 1 : 0);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int yh_pointwise_mulmod(yh_ctx* ctx, uint64_t* d_a, const uint64_t* d_b, size_t count) {
+  if (!ctx || !d_a || !d_b) return fail(YH_ERR_ARG, "null argument");
+  if (count == 0) return YH_OK;
+  if (int s = set_dev(ctx)) return s;
+  hipLaunchKernelGGL(yhk::k_pointwise, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, d_a, d_b,
+                     count);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int yh_residue_addsub(yh_ring* r, uint64_t* d_c, const uint64_t* d_a, const uint64_t* d_b, int spacing, int op) {
+  if (!r || !d_c || !d_a || !d_b || spacing <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (int s = set_dev(r->ctx)) return s;
+  dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
+  hipLaunchKernelGGL(yhk::k_residue_addsub, grid, block, 0, r->ctx->stream, r->rd, d_c, d_a, d_b, spacing,
+                     op == YH_SUB ? 1 : 0);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t* d_b, int deg_b, int min_N,
+                      uint64_t* d_out, int out_limbs, int* N_out) {
+  if (!r || !d_a || !d_b || !d_out) return fail(YH_ERR_ARG, "null argument");
+  const int n = r->rd.n;
+  if (deg_a < 0 || deg_b < 0 || deg_a >= n || deg_b >= n) return fail(YH_ERR_ARG, "operand degrees must be in [0, n)");
+  if (out_limbs < 1) return fail(YH_ERR_ARG, "out_limbs must be positive");
+  // N = 2^ceil(log2(deg a + deg b)) as in src/common.h:139, at least min_N
+  int sum = deg_a + deg_b;
+  long N = 1;
+  while (N < sum) N <<= 1;
+  if (N < min_N) N = min_N;
+  if (N < 2) N = 2;
+  // every operand coefficient needs a slot (deg_b = 0 with deg_a a power of two)
+  while (N < deg_a + 1 || N < deg_b + 1) N <<= 1;
+  if (N > 16384) return fail(YH_ERR_UNSUPPORTED, "plain product longer than 16384");
+  if (N & (N - 1)) return fail(YH_ERR_ARG, "min_N must be a power of two");
+  if (int s = set_dev(r->ctx)) return s;
+  const int R = r->rd.R, LQ = r->rd.LQ;
+  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)R * N * sizeof(uint32_t));
+  if (st) return st;
+  DevTables tw{};
+  if (N >= 16) {
+    if (int s = get_cyclic(r->ctx, (int)N, &tw)) return s;
+  }
+  yhk::ProdArgs pa{};
+  pa.A = d_a; pa.B = d_b;
+  pa.lin_a = pa.lin_b = LQ;
+  pa.ncoef_a = deg_a + 1; pa.ncoef_b = deg_b + 1;
+  pa.lstride_a = deg_a + 1; pa.lstride_b = deg_b + 1;
+  pa.ostride_a = pa.ostride_b = 0;
+  pa.Y = r->d_y;
+  pa.logN = ilog2i(N);
+  pa.T = tw.fwd; pa.Ti = tw.inv;
+  pa.n_inv = gl::inv((uint64_t)N);
+  pa.set = 0;
+  st = launch_prod(r, pa, (int)N, 1, false);
+  if (st) return st;
+  if (N_out) *N_out = (int)N;
+  return launch_icrt(r, icrt_args(r, (int)N, (int)N, 0, yhk::EPI_RAW, d_out, out_limbs, N, 0, 1), 1);
+}
+
+int yh_ring_mul(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch) {
+  if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (int s = set_dev(r->ctx)) return s;
+  return ring_product(r, d_a, d_b, d_out, batch, yhk::EPI_MODQ);
+}
+
+int yh_ring_addsub(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch, int op) {
+  if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (int s = set_dev(r->ctx)) return s;
+  int sub = op == YH_SUB ? 1 : 0;
+  switch (r->rd.LQ) {
+    case 1: launch_addsub_t<1>(r, d_a, d_b, d_out, batch, sub); break;
+    case 2: launch_addsub_t<2>(r, d_a, d_b, d_out, batch, sub); break;
+    case 3: launch_addsub_t<3>(r, d_a, d_b, d_out, batch, sub); break;
+    case 4: launch_addsub_t<4>(r, d_a, d_b, d_out, batch, sub); break;
+    case 5: launch_addsub_t<5>(r, d_a, d_b, d_out, batch, sub); break;
+    case 6: launch_addsub_t<6>(r, d_a, d_b, d_out, batch, sub); break;
+    case 7: launch_addsub_t<7>(r, d_a, d_b, d_out, batch, sub); break;
+    case 8: launch_addsub_t<8>(r, d_a, d_b, d_out, batch, sub); break;
+    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int yh_ring_reduce(yh_ring* r, const uint64_t* d_c, int in_limbs, uint64_t* d_out, int batch) {
+  if (!r || !d_c || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (in_limbs < 1 || in_limbs > r->rd.LW) return fail(YH_ERR_ARG, "in_limbs out of range");
+  if (int s = set_dev(r->ctx)) return s;
+  switch (r->rd.LQ) {
+    case 1: launch_reduce_t<1>(r, d_c, in_limbs, d_out, batch); break;
+    case 2: launch_reduce_t<2>(r, d_c, in_limbs, d_out, batch); break;
+    case 3: launch_reduce_t<3>(r, d_c, in_limbs, d_out, batch); break;
+    case 4: launch_reduce_t<4>(r, d_c, in_limbs, d_out, batch); break;
+    case 5: launch_reduce_t<5>(r, d_c, in_limbs, d_out, batch); break;
+    case 6: launch_reduce_t<6>(r, d_c, in_limbs, d_out, batch); break;
+    case 7: launch_reduce_t<7>(r, d_c, in_limbs, d_out, batch); break;
+    case 8: launch_reduce_t<8>(r, d_c, in_limbs, d_out, batch); break;
+    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int yh_yashe_mul(yh_ring* r, const uint64_t* d_c1, const uint64_t* d_c2, uint64_t* d_out, int batch) {
+  if (!r || !d_c1 || !d_c2 || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
+  if (int s = set_dev(r->ctx)) return s;
+  return ring_product(r, d_c1, d_c2, d_out, batch, yhk::EPI_SCALE);
+}
+
+int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
+  if (!r || !d_gamma || !out) return fail(YH_ERR_ARG, "null argument");
+  if (r->rd.w_bits == 0) return fail(YH_ERR_ARG, "ring has no keyswitch base");
+  if (int s = set_dev(r->ctx)) return s;
+  auto* k = new yh_evk();
+  k->ring = r;
+  size_t bytes = (size_t)r->rd.lw * r->rd.set_R[1] * r->rd.n * sizeof(uint64_t);
+  hipError_t e = hipMalloc(&k->d, bytes);
+  if (e != hipSuccess) { delete k; return fail(YH_ERR_NOMEM, hipGetErrorString(e)); }
+  dim3 grid(r->rd.set_R[1], r->rd.lw);
+  int transform = r->rd.n >= 16 ? 1 : 0;
+  dim3 block(transform ? r->rd.n / 16 : 64);
+  hipLaunchKernelGGL(yhk::k_evk_prep, grid, block, transform ? lds_bytes(r->rd.n) : 0, r->ctx->stream, r->rd,
+                     d_gamma, k->d, transform);
+  e = hipGetLastError();
+  if (e != hipSuccess) { (void)hipFree(k->d); delete k; return fail(YH_ERR_HIP, hipGetErrorString(e)); }
+  *out = k;
+  return YH_OK;
+}
+
+int yh_evk_destroy(yh_evk* k) {
+  if (!k) return YH_OK;
+  (void)hipSetDevice(k->ring->ctx->device);
+  (void)hipStreamSynchronize(k->ring->ctx->stream);
+  (void)hipFree(k->d);
+  delete k;
+  return YH_OK;
+}
+
+int yh_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c, uint64_t* d_out, int batch) {
+  if (!r || !k || !d_c || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (k->ring != r) return fail(YH_ERR_ARG, "key belongs to another ring");
+  if (d_c == d_out) return fail(YH_ERR_ARG, "keyswitch cannot run in place");
+  if (int s = set_dev(r->ctx)) return s;
+  return keyswitch_impl(r, k, d_c, d_out, batch);
+}
+
+int yh_yashe_mul_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, const uint64_t* d_c2,
+                           uint64_t* d_out, int batch) {
+  if (!r || !k || !d_c1 || !d_c2 || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (k->ring != r) return fail(YH_ERR_ARG, "key belongs to another ring");
+  if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
+  if (int s = set_dev(r->ctx)) return s;
+  size_t need = (size_t)batch * r->rd.LQ * r->rd.n * sizeof(uint64_t);
+  int st = ensure(r->ctx, (void**)&r->d_tmp, &r->tmp_bytes, need);
+  if (st) return st;
+  st = ring_product(r, d_c1, d_c2, r->d_tmp, batch, yhk::EPI_SCALE);
+  if (st) return st;
+  return keyswitch_impl(r, k, r->d_tmp, d_out, batch);
+}
+
+int yh_yashe_decrypt(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch) {
+  if (!r || !d_f || !d_c || !d_m || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
+  if (int s = set_dev(r->ctx)) return s;
+  const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
+  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  if (st) return st;
+  yhk::ProdArgs pa{};
+  pa.A = d_f; pa.B = d_c;
+  pa.lin_a = pa.lin_b = LQ;
+  pa.ncoef_a = pa.ncoef_b = n;
+  pa.lstride_a = pa.lstride_b = n;
+  pa.ostride_a = 0;
+  pa.ostride_b = (size_t)LQ * n;
+  pa.Y = r->d_y;
+  pa.logN = r->rd.logn;
+  pa.T = r->rd.psi_rev;
+  pa.Ti = r->rd.psiinv_rev;
+  pa.n_inv = r->rd.n_inv;
+  pa.set = 0;
+  st = launch_prod(r, pa, n, batch, true);
+  if (st) return st;
+  return launch_icrt(r, icrt_args(r, n, n, 0, yhk::EPI_DECRYPT, d_m, 1, n, n, 0), batch);
+}
+
+size_t yh_mem_footprint(const yh_ring* r, int op) {
+  if (!r) return 0;
+  size_t n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R, Rks = r->rd.set_R[1];
+  size_t ct = LQ * n * sizeof(uint64_t);
+  size_t y = R * n * sizeof(uint32_t), yks = Rks * n * sizeof(uint32_t);
+  switch (op) {
+    case 0: return 3 * ct + y;
+    case 1: return 3 * ct + y;
+    case 2: return 2 * ct + yks;
+    case 3: return 4 * ct + (y > yks ? y : yks);
+    case 4: return 3 * ct;
+    default: return 0;
+  }
+}
+
+}  // extern "C"

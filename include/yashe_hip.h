@@ -1,0 +1,197 @@
+/*
+ * yashe_hip.h - C ABI of the MI355X (gfx950) YASHE polynomial-ring hot path.
+ *
+ * This is the lower boundary the host data model (Polynomial / Ciphertext /
+ * Yashe) calls. It replaces the reference's CUDAFunctions statics
+ * (src/cuda_functions.h:19-84) and the free CRT/ICRT functions
+ * (src/cuda_bn.h:76-87). Differences in convention, applied to every call:
+ *  - every entry point returns an int status (YH_OK == 0); yh_last_error()
+ *    gives a message for the calling thread. The reference asserts.
+ *  - device buffers are owned by the caller (yh_malloc / yh_free); no call
+ *    returns a freshly allocated buffer (the reference's callPolynomialMul
+ *    and callPolynomialOPInteger do, and leak them).
+ *  - all work is ordered on the context's stream (yh_ctx_stream), and calls
+ *    return without synchronising unless documented. The reference uses the
+ *    legacy default stream with cudaDeviceSynchronize after each operator.
+ *  - ring parameters live in a yh_ring handle instead of global statics
+ *    (Polynomial::CRTPrimes, CUDAFunctions::N, Yashe::q ...).
+ *  - coefficient polynomials are limb-major: limb l of coefficient j of
+ *    polynomial b sits at ptr[(b*L + l)*n + j] (L 64-bit limbs, least
+ *    significant first). The reference's bn_t {alloc, used, sign, dp*}
+ *    per coefficient is not used.
+ *  - CRT residues are u64 [R][spacing], as the reference's d_polyCRT.
+ */
+#ifndef YASHE_HIP_H
+#define YASHE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YH_OK 0
+#define YH_ERR_ARG 1
+#define YH_ERR_HIP 2
+#define YH_ERR_UNSUPPORTED 3
+#define YH_ERR_NOMEM 4
+#define YH_ERR_INTERNAL 5
+
+#define YH_FORWARD 1 /* ntt_mode_t FORWARD, src/settings.h:38 */
+#define YH_INVERSE 0 /* ntt_mode_t INVERSE */
+#define YH_ADD 0     /* add_mode_t ADD, src/settings.h:37 */
+#define YH_SUB 1     /* add_mode_t SUB */
+
+typedef struct yh_ctx yh_ctx;
+typedef struct yh_ring yh_ring;
+typedef struct yh_evk yh_evk;
+
+typedef struct yh_ring_info {
+  int n;          /* ring degree: x^n + 1 */
+  int q_limbs;    /* 64-bit limbs of q */
+  int q_bits;
+  int R;          /* CRT primes of ring products */
+  int R_ks;       /* leading primes used by the keyswitch */
+  int w_bits;     /* keyswitch base w = 2^w_bits (0: none) */
+  int lw;         /* keyswitch digits */
+  int work_limbs; /* limbs of the reconstruction (ICRT) output */
+  int m_bits;     /* bits of the CRT modulus M (all R primes) */
+  int m_ks_bits;  /* bits of the keyswitch CRT modulus */
+  uint64_t t;     /* plaintext modulus */
+  uint64_t prime_max;
+} yh_ring_info;
+
+/* ------------------------------------------------------------ errors */
+const char* yh_last_error(void);
+int yh_version(void);
+
+/* ------------------------------------------------------------ context
+ * Replaces the implicit CUDA context + cudaSetDevice(0)
+ * (src/benchmark_polynomial.cpp:25) and the per-object streams of
+ * src/polynomial.h:48. */
+int yh_ctx_create(int device, yh_ctx** out);
+int yh_ctx_destroy(yh_ctx* ctx);
+/* hipStream_t of the context, as void*. */
+void* yh_ctx_stream(yh_ctx* ctx);
+/* Blocks until the context stream is idle (cudaDeviceSynchronize in the
+ * reference, src/common.h:223). */
+int yh_ctx_sync(yh_ctx* ctx);
+
+/* ------------------------------------------------------------ memory
+ * Replaces the cudaMalloc / cudaMemcpy calls scattered through
+ * src/polynomial.h:1127-1314 (update_crt_spacing) and
+ * src/polynomial.cpp:265-417 (update_device_data / update_host_data). */
+int yh_malloc(yh_ctx* ctx, size_t bytes, void** dptr);
+int yh_free(yh_ctx* ctx, void* dptr);
+/* Stream-ordered copies; they return after the copy completed. */
+int yh_memcpy_htod(yh_ctx* ctx, void* dst, const void* src, size_t bytes);
+int yh_memcpy_dtoh(yh_ctx* ctx, void* dst, const void* src, size_t bytes);
+int yh_memcpy_dtod(yh_ctx* ctx, void* dst, const void* src, size_t bytes);
+int yh_memset(yh_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* ------------------------------------------------------------ ring set-up
+ * Replaces Polynomial::gen_crt_primes (src/polynomial.h:898-955),
+ * CUDAFunctions::write_crt_primes (src/cuda_bn.cu:1047-1147) and
+ * CUDAFunctions::init (src/cuda_functions.cu:878-956).
+ * n: power of two in [2, 16384]; q: q_limbs little-endian limbs (q > 1,
+ * at most 8 limbs); t: plaintext modulus (0 if unused); w_bits: keyswitch
+ * digit width in bits, 0..32 (0 = no keyswitch). Primes are chosen so that
+ * every product of the ring is exact (see the ring_params.hpp header). */
+int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits,
+                   yh_ring** out);
+int yh_ring_destroy(yh_ring* ring);
+/* Host-only planning of the same parameters (no device needed): fills info
+ * and up to `cap` primes. Used for admission control and CPU-side checks. */
+int yh_ring_plan(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, yh_ring_info* info,
+                 uint64_t* primes, int cap);
+int yh_ring_get_info(const yh_ring* ring, yh_ring_info* info);
+/* The R CRT primes, descending (Polynomial::CRTPrimes). */
+int yh_ring_get_primes(const yh_ring* ring, uint64_t* out, int cap);
+/* M as limbs: set 0 = all R primes (Polynomial::CRTProduct), 1 = keyswitch. */
+int yh_ring_get_modulus(const yh_ring* ring, int set, uint64_t* out, int cap);
+
+/* ------------------------------------------------------------ primitives */
+/* CRT: res[i][j] = coef_j mod p_i for j < ncoef, 0 for ncoef <= j < spacing.
+ * coefs: limb-major [limbs][ncoef] (limbs <= 2*q_limbs+1).
+ * Replaces callCRT / cuCRT (src/cuda_bn.cu:970-1002, :803-835). */
+int yh_crt(yh_ring* ring, const uint64_t* d_coefs, int limbs, int ncoef, int spacing, uint64_t* d_res);
+/* ICRT: coefs_j = sum_i M_i ((r_ij (M_i^-1 mod p_i)) mod p_i) mod M, in
+ * [0, M), written as out_limbs limbs [out_limbs][ncoef]. r_ij < p_i.
+ * Replaces callICRT / cuPreICRT / cuICRT (src/cuda_bn.cu:1004-1045). */
+int yh_icrt(yh_ring* ring, const uint64_t* d_res, int spacing, int ncoef, uint64_t* d_coefs, int out_limbs);
+/* Cyclic NTT over P = 2^64 - 2^32 + 1 of `count` contiguous vectors of
+ * length N (power of two, 16 <= N <= 16384), in place, natural order:
+ * X_k = sum_j x_j w^(jk), w = 7^((P-1)/N); YH_INVERSE uses w^-1 and scales
+ * by N^-1 mod P. Replaces CUDAFunctions::callNTT
+ * (src/cuda_functions.cu:447-458, inverse scaling NTTScale :397-402). */
+int yh_ntt(yh_ctx* ctx, uint64_t* d_data, int N, int count, int direction);
+/* a[k] = a[k] * b[k] mod P. Replaces polynomialNTTMul
+ * (src/cuda_functions.cu:460-474). */
+int yh_pointwise_mulmod(yh_ctx* ctx, uint64_t* d_a, const uint64_t* d_b, size_t count);
+/* c = a +- b mod p_i on residues [R][spacing] (op YH_ADD / YH_SUB).
+ * Replaces callPolynomialAddSub (src/cuda_functions.cu:115-126), adding
+ * the mod p_i that the reference omits. c may alias a. */
+int yh_residue_addsub(yh_ring* ring, uint64_t* d_c, const uint64_t* d_a, const uint64_t* d_b, int spacing,
+                      int op);
+
+/* ------------------------------------------------------------ fused operations
+ * All take `batch` independent operands laid out back to back. */
+
+/* Plain product in Z[x]/(x^N - 1), N = max(min_N, 2^ceil(log2(deg_a+deg_b)))
+ * (the NTT-length rule of common_multiplication, src/common.h:136-226;
+ * power-of-two degree sums wrap cyclically exactly like the reference).
+ * a: [q_limbs][deg_a+1], b: [q_limbs][deg_b+1], coefficients in [0, q),
+ * deg_a, deg_b < n. out: [out_limbs][N] with the exact coefficients in
+ * [0, M). *N_out receives N. Replaces Polynomial::operator*
+ * (src/polynomial.cpp:52-60) -> callPolynomialMul
+ * (src/cuda_functions.cu:617-738) -> icrt. */
+int yh_poly_mul_plain(yh_ring* ring, const uint64_t* d_a, int deg_a, const uint64_t* d_b, int deg_b,
+                      int min_N, uint64_t* d_out, int out_limbs, int* N_out);
+/* c = a * b mod (x^n + 1, q), operands [batch][q_limbs][n] in [0, q).
+ * Replaces the sequence c = a*b; c.reduce(); c %= q
+ * (src/polynomial.cpp:52-60, src/cuda_functions.cu:1040-1078,
+ * src/polynomial.h:593-610). */
+int yh_ring_mul(yh_ring* ring, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch);
+/* c = a +- b mod q (op YH_ADD / YH_SUB) on [batch][q_limbs][n] in [0, q).
+ * Replaces common_addition (src/common.h:24-80) followed by % q. */
+int yh_ring_addsub(yh_ring* ring, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch, int op);
+/* Reduction of [batch][in_limbs][2n] coefficients (any value < 2^(64 in_limbs))
+ * mod (x^n + 1) and q: out_j = (c_j - c_{j+n}) mod q in [0, q).
+ * Replaces Polynomial::reduce (src/cuda_functions.cu:1040-1078). */
+int yh_ring_reduce(yh_ring* ring, const uint64_t* d_c, int in_limbs, uint64_t* d_out, int batch);
+/* YASHE multiply: c = [round(t * c1*c2 / q)]_q, the product taken mod
+ * x^n + 1 over Z (no reduction mod q before rounding); ties round down.
+ * Replaces Ciphertext::operator* (src/ciphertext.cpp:38-73) with the
+ * t factor and rounding order that the reference intends. */
+int yh_yashe_mul(yh_ring* ring, const uint64_t* d_c1, const uint64_t* d_c2, uint64_t* d_out, int batch);
+/* Keyswitch key: gamma [lw][q_limbs][n] (device, coefficients in [0, q))
+ * pre-transformed and kept resident. Replaces the per-call
+ * P[k] * gamma[k] products of Ciphertext::keyswitch (src/ciphertext.cpp:81-107). */
+int yh_evk_create(yh_ring* ring, const uint64_t* d_gamma, yh_evk** out);
+int yh_evk_destroy(yh_evk* evk);
+/* c' = [sum_k D_k(c) gamma_k]_(x^n+1, q), D_k(c) = floor(c / w^k) mod w.
+ * Replaces Ciphertext::convert/keyswitch/worddecomp
+ * (src/ciphertext.cpp:75-130) with the base, mask and digit count fixed
+ * as the reference intends (w a base 2^w_bits, l_w = ceil(bits(q)/w_bits)). */
+int yh_keyswitch(yh_ring* ring, const yh_evk* evk, const uint64_t* d_c, uint64_t* d_out, int batch);
+/* yh_yashe_mul followed by yh_keyswitch (the lazy convert() of the
+ * reference, src/ciphertext.cpp:44-47, made eager). */
+int yh_yashe_mul_keyswitch(yh_ring* ring, const yh_evk* evk, const uint64_t* d_c1, const uint64_t* d_c2,
+                           uint64_t* d_out, int batch);
+/* Decryption core: m_j = round(t * [f*c]_q,j / q) mod t for every coefficient
+ * ([.]_q in [0, q)). f: [q_limbs][n] (one key), c: [batch][q_limbs][n];
+ * m: [batch][n] u64. Replaces Yashe::decrypt (src/yashe.cpp:143-201), which
+ * decodes coefficient 0 only. */
+int yh_yashe_decrypt(yh_ring* ring, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch);
+
+/* ------------------------------------------------------------ batching helpers */
+/* Device bytes one operation of `op` (0 ring_mul, 1 yashe_mul, 2 keyswitch,
+ * 3 mul+keyswitch, 4 add) needs per batch element, inputs and outputs
+ * included: the admission unit of a serving loop. */
+size_t yh_mem_footprint(const yh_ring* ring, int op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,92 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol,
+and its host-only parameter planning follows the exactness rules."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import cuyashe_amd as ya
+from cuyashe_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "yashe_hip.h")
+P = (1 << 64) - (1 << 32) + 1
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(yh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert set(declared_functions()) == set(_lib.exported_symbols())
+
+
+def test_no_gpu_fails_loudly():
+    # Without a device the context cannot be created; there is no CPU fallback.
+    try:
+        import subprocess
+        has_gpu = subprocess.run(["rocminfo"], capture_output=True, timeout=20).returncode == 0 and \
+            b"gfx9" in subprocess.run(["rocminfo"], capture_output=True, timeout=20).stdout
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    with pytest.raises(ya.YashError):
+        ya.Context(0)
+
+
+@pytest.mark.parametrize("n,q,t,wb", [
+    (4096, (1 << 127) - 1, 1024, 32),
+    (8192, (1 << 255) - 19, 1 << 16, 32),
+    (1024, (1 << 127) - 1, 0, 0),
+    (16384, (1 << 511) - 187, 1024, 32),
+    (8, 77287149995192912462927307869, 35951, 32),
+    (2, 1171313591017775093490277364417, 0, 0),
+])
+def test_ring_plan_bounds(n, q, t, wb):
+    info, primes = ya.ring_plan(n, q, t, wb)
+    assert info.R == len(primes) and info.n == n
+    assert primes == sorted(primes, reverse=True) and len(set(primes)) == len(primes)
+    K = max(1, info.lw)
+    for p in primes:
+        assert all(p % d for d in range(2, int(p ** 0.5) + 1))
+        # centred residues keep the Goldilocks convolution exact
+        assert K * n * ((p - 1) // 2) ** 2 <= (P - 1) // 2
+        assert p < (1 << 28)
+    M = 1
+    for p in primes:
+        M *= p
+    assert M >= 4 * n * (q - 1) ** 2
+    assert M.bit_length() == info.m_bits
+    assert M.bit_length() + 2 <= 64 * info.work_limbs
+    if wb:
+        lw = (q.bit_length() + wb - 1) // wb
+        assert info.lw == lw
+        Mk = 1
+        for p in primes[:info.R_ks]:
+            Mk *= p
+        assert Mk >= 4 * lw * n * ((1 << wb) - 1) * (q - 1)
+        Mk_short = Mk // primes[info.R_ks - 1]
+        assert Mk_short < 4 * lw * n * ((1 << wb) - 1) * (q - 1)
+
+
+def test_ring_plan_rejects_bad_parameters():
+    with pytest.raises(ya.YashError):
+        ya.ring_plan(3, (1 << 127) - 1)
+    with pytest.raises(ya.YashError):
+        ya.ring_plan(32768, (1 << 127) - 1)
+    with pytest.raises(ya.YashError):
+        ya.ring_plan(64, 1)
+    with pytest.raises(ya.YashError):
+        ya.ring_plan(64, (1 << 600) + 1)
