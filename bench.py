@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark: bit-exact YASHE multiply + keyswitch at n=4096, q=2^127-1,
+t=1024, w=2^32 on one MI355X.
+
+A step is one batched multiply+keyswitch over `--batch` resident ciphertext
+pairs (hot path: CRT -> negacyclic Goldilocks NTTs -> pointwise -> INTT ->
+ICRT with t/q rounding, then digit decomposition -> NTT -> multiply-
+accumulate with the resident keys -> INTT -> ICRT mod q). Inputs sit in HBM
+before the timed region. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import random
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "bit-exact YASHE mul+keyswitch ops/sec (n=4096,q~2^127); batched vs plain; p99 ms"
+N_RING = 4096
+Q = (1 << 127) - 1
+T = 1024
+W_BITS = 32
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--plain-ops", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def uniform_poly(rng, n, q):
+    return [rng.randrange(q) for _ in range(n)]
+
+
+def percentile(xs, p):
+    xs = sorted(xs)
+    k = (len(xs) - 1) * p / 100.0
+    lo = int(k)
+    hi = min(lo + 1, len(xs) - 1)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path shards by request, "
+              "with no collective; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import cuyashe_amd as ya
+
+    rng = random.Random(20240613)
+    ctx = ya.Context(0)
+    ring = ya.Ring(ctx, N_RING, Q, t=T, w_bits=W_BITS)
+    n, L, B = ring.n, ring.L, args.batch
+
+    # Synthetic resident data: keyswitch keys and ciphertext pairs uniform mod q.
+    gammas = [uniform_poly(rng, n, Q) for _ in range(ring.lw)]
+    evk = ring.evk(ctx.to_device(ring.pack(gammas)))
+    uniq = 16  # distinct pairs, tiled over the batch
+    C1 = [uniform_poly(rng, n, Q) for _ in range(uniq)]
+    C2 = [uniform_poly(rng, n, Q) for _ in range(uniq)]
+    import numpy as np
+    A1 = ring.pack(C1)
+    A2 = ring.pack(C2)
+    reps = (B + uniq - 1) // uniq
+    d1 = ctx.to_device(np.ascontiguousarray(np.tile(A1, (reps, 1, 1))[:B]))
+    d2 = ctx.to_device(np.ascontiguousarray(np.tile(A2, (reps, 1, 1))[:B]))
+    out = ctx.alloc(B * L * n * 8)
+
+    # ---- batched: warmup, then exactly K steps between two synchronisations
+    for _ in range(args.warmup):
+        ring.mul_keyswitch(evk, d1, d2, out, B)
+    ctx.sync()
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ring.mul_keyswitch(evk, d1, d2, out, B)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    prod_ms, prod_n = ctx.kernel_time(ya.KERNEL_PROD)
+    ks_ms, ks_n = ctx.kernel_time(ya.KERNEL_KEYSWITCH)
+    icrt_ms, icrt_n = ctx.kernel_time(ya.KERNEL_ICRT)
+    ctx.set_timing(False)
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = B * args.steps / elapsed
+
+    # per-step latency (each step synchronised)
+    lat = []
+    for _ in range(max(10, args.steps)):
+        s = time.perf_counter()
+        ring.mul_keyswitch(evk, d1, d2, out, B)
+        ctx.sync()
+        lat.append(1000.0 * (time.perf_counter() - s))
+
+    # ---- plain: one op at a time, synchronised after each (reference style)
+    one = ctx.alloc(L * n * 8)
+    for _ in range(3):
+        ring.mul_keyswitch(evk, d1, d2, one, 1)
+    ctx.sync()
+    plain_lat = []
+    tp0 = time.perf_counter()
+    for _ in range(args.plain_ops):
+        s = time.perf_counter()
+        ring.mul_keyswitch(evk, d1, d2, one, 1)
+        ctx.sync()
+        plain_lat.append(1000.0 * (time.perf_counter() - s))
+    plain_value = args.plain_ops / (time.perf_counter() - tp0)
+
+    # ---- roofline of the dominant fused kernel (algorithmic bytes: 16*n per
+    # length-n transform of one residue vector; fused transforms each count)
+    R, Rks, lw = ring.R, ring.R_ks, ring.lw
+    kernels = {
+        "k_prod": {"total_ms": prod_ms, "launches": prod_n,
+                   "bytes_per_launch": 3 * R * B * 16 * n},
+        "k_keyswitch": {"total_ms": ks_ms, "launches": ks_n,
+                        "bytes_per_launch": (lw + 1) * Rks * B * 16 * n},
+        "k_icrt": {"total_ms": icrt_ms, "launches": icrt_n, "bytes_per_launch": None},
+    }
+    for k in kernels.values():
+        k["avg_ms"] = k["total_ms"] / k["launches"] if k["launches"] else None
+    dom_name = max(("k_prod", "k_keyswitch"), key=lambda k: kernels[k]["total_ms"])
+    dom = kernels[dom_name]
+    achieved = dom["bytes_per_launch"] / (dom["avg_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name}
+    ntt_bytes = kernels["k_prod"]["bytes_per_launch"] * prod_n + kernels["k_keyswitch"]["bytes_per_launch"] * ks_n
+    ntt_ms = prod_ms + ks_ms
+
+    # ---- CPU baseline: the oracle on a bounded sample of the same workload,
+    # which also checks those GPU results bit for bit.
+    cpu = None
+    exact = None
+    if not args.no_cpu:
+        from oracle import cpuref
+        got = ring.unpack(out.download((min(B, uniq), L, n)))
+        done = 0
+        match = True
+        c0 = time.perf_counter()
+        while done < uniq and (done == 0 or time.perf_counter() - c0 < args.cpu_seconds):
+            want = cpuref.keyswitch(cpuref.yashe_mul(C1[done], C2[done], n, Q, T), gammas, n, Q, W_BITS)
+            match &= want == got[done]
+            done += 1
+        cpu_s = time.perf_counter() - c0
+        cpu = {"value": round(done / cpu_s, 4), "unit": "ops/s", "cores": 1, "kind": "port",
+               "sample": f"{done} mul+keyswitch ops at n=4096, q=2^127-1 through the oracle (oracle/cpuref.c "
+                         f"Stockham NTT + Python ICRT glue), single thread, {cpu_s:.1f} s"}
+        exact = bool(match)
+
+    res = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "ops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic: seeded ciphertext pairs and keyswitch keys uniform mod q, resident in HBM",
+        "config": {"workload": "YASHE mul+keyswitch, n=4096, q=2^127-1, t=1024, w=2^32 (4 digits)",
+                   "batch": B, "crt_primes": R, "keyswitch_primes": Rks, "digits": lw},
+        "p50_ms": round(percentile(lat, 50), 4),
+        "p99_ms": round(percentile(lat, 99), 4),
+        "plain": {"value": round(plain_value, 1), "unit": "ops/s",
+                  "p50_ms": round(percentile(plain_lat, 50), 4), "p99_ms": round(percentile(plain_lat, 99), 4)},
+        "batched_vs_plain": round(value / plain_value, 2),
+        "ntt_stages": {"algorithmic_GBps": round(ntt_bytes / (ntt_ms * 1e-3) / 1e9, 1) if ntt_ms else None,
+                       "frac_of_hbm": round(ntt_bytes / (ntt_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ntt_ms else None},
+        "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kernels.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "bit_exact_sample": exact,
+    }
+    evk.close()
+    ring.close()
+    ctx.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -21,6 +21,7 @@ YH_ADD = 0
 YH_SUB = 1
 
 OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD = range(5)
+KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT = range(3)
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
@@ -79,6 +80,10 @@ _SIGS = {
     "yh_yashe_mul_keyswitch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_yashe_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_mem_footprint": (ctypes.c_size_t, [_vp, ctypes.c_int]),
+    "yh_ctx_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "yh_ctx_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_long)]),
+    "yh_ctx_reset_timing": (ctypes.c_int, [_vp]),
 }
 
 
@@ -190,6 +195,19 @@ class Context:
 
     def sync(self):
         check(lib().yh_ctx_sync(self.h), "yh_ctx_sync")
+
+    def set_timing(self, enable: bool):
+        check(lib().yh_ctx_set_timing(self.h, 1 if enable else 0), "yh_ctx_set_timing")
+
+    def reset_timing(self):
+        check(lib().yh_ctx_reset_timing(self.h), "yh_ctx_reset_timing")
+
+    def kernel_time(self, kernel_id: int):
+        """(total ms, launches) of one fused kernel since timing was enabled."""
+        ms = ctypes.c_double(0)
+        cnt = ctypes.c_long(0)
+        check(lib().yh_ctx_kernel_time(self.h, kernel_id, ctypes.byref(ms), ctypes.byref(cnt)), "yh_ctx_kernel_time")
+        return ms.value, cnt.value
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)

@@ -43,11 +43,21 @@ struct DevTables {
 
 }  // namespace
 
+struct TimedLaunch {
+  int kernel;
+  hipEvent_t start, stop;
+};
+
 struct yh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::map<int, DevTables> cyclic;  // N -> twiddles of the cyclic transform
   std::mutex mu;
+  bool timing = false;
+  std::vector<TimedLaunch> pending;
+  std::vector<hipEvent_t> event_pool;
+  double total_ms[3] = {0, 0, 0};
+  long launches[3] = {0, 0, 0};
 };
 
 struct yh_ring {
@@ -67,6 +77,37 @@ struct yh_evk {
 };
 
 namespace {
+
+hipEvent_t take_event(yh_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Records an event pair around one launch when timing is on.
+struct LaunchTimer {
+  yh_ctx* ctx;
+  int kernel;
+  hipEvent_t start = nullptr;
+  LaunchTimer(yh_ctx* c, int k) : ctx(c), kernel(k) {
+    if (ctx->timing) {
+      start = take_event(ctx);
+      if (start) (void)hipEventRecord(start, ctx->stream);
+    }
+  }
+  ~LaunchTimer() {
+    if (!start) return;
+    hipEvent_t stop = take_event(ctx);
+    if (!stop) return;
+    (void)hipEventRecord(stop, ctx->stream);
+    ctx->pending.push_back({kernel, start, stop});
+  }
+};
 
 int set_dev(yh_ctx* ctx) {
   YH_HIP(hipSetDevice(ctx->device));
@@ -110,6 +151,7 @@ size_t lds_bytes(int N) { return (size_t)(N + N / 16) * sizeof(uint64_t); }
 int launch_prod(yh_ring* r, const yhk::ProdArgs& pa, int N, int batch, bool negacyclic) {
   hipStream_t s = r->ctx->stream;
   int R = r->rd.set_R[pa.set];
+  LaunchTimer timer(r->ctx, YH_KERNEL_PROD);
   if (N >= 16) {
     dim3 grid(R, batch), block(N / 16);
     hipLaunchKernelGGL(yhk::k_prod, grid, block, lds_bytes(N), s, r->rd, pa);
@@ -128,6 +170,7 @@ void launch_icrt_t(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
 }
 
 int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
+  LaunchTimer timer(r->ctx, YH_KERNEL_ICRT);
   switch (r->rd.LQ) {
     case 1: launch_icrt_t<1>(r, ia, batch); break;
     case 2: launch_icrt_t<2>(r, ia, batch); break;
@@ -203,11 +246,13 @@ int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out
   ka.evk = k->d;
   ka.Y = r->d_y;
   dim3 grid(Rks, batch);
+  LaunchTimer* timer = new LaunchTimer(r->ctx, YH_KERNEL_KEYSWITCH);
   if (n >= 16) {
     hipLaunchKernelGGL(yhk::k_keyswitch, grid, dim3(n / 16), lds_bytes(n), r->ctx->stream, r->rd, ka);
   } else {
     hipLaunchKernelGGL(yhk::k_keyswitch_small, grid, dim3(64), 0, r->ctx->stream, r->rd, ka, k->d);
   }
+  delete timer;
   YH_HIP(hipGetLastError());
   return launch_icrt(r, icrt_args(r, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0), batch);
 }
@@ -216,6 +261,8 @@ int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out
 
 // ====================================================================== C ABI
 extern "C" {
+
+static int drain_timing(yh_ctx* ctx);
 
 const char* yh_last_error(void) { return g_err.c_str(); }
 int yh_version(void) { return 100; }
@@ -245,6 +292,8 @@ int yh_ctx_destroy(yh_ctx* ctx) {
   if (!ctx) return YH_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  drain_timing(ctx);
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   for (auto& kv : ctx->cyclic) { (void)hipFree(kv.second.fwd); (void)hipFree(kv.second.inv); }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -252,6 +301,45 @@ int yh_ctx_destroy(yh_ctx* ctx) {
 }
 
 void* yh_ctx_stream(yh_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+static int drain_timing(yh_ctx* ctx) {
+  for (auto& t : ctx->pending) {
+    YH_HIP(hipEventSynchronize(t.stop));
+    float ms = 0;
+    YH_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+    ctx->total_ms[t.kernel] += ms;
+    ctx->launches[t.kernel] += 1;
+    ctx->event_pool.push_back(t.start);
+    ctx->event_pool.push_back(t.stop);
+  }
+  ctx->pending.clear();
+  return YH_OK;
+}
+
+int yh_ctx_set_timing(yh_ctx* ctx, int enable) {
+  if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (int s = set_dev(ctx)) return s;
+  if (int s = drain_timing(ctx)) return s;
+  ctx->timing = enable != 0;
+  return YH_OK;
+}
+
+int yh_ctx_reset_timing(yh_ctx* ctx) {
+  if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (int s = set_dev(ctx)) return s;
+  if (int s = drain_timing(ctx)) return s;
+  for (int k = 0; k < 3; ++k) { ctx->total_ms[k] = 0; ctx->launches[k] = 0; }
+  return YH_OK;
+}
+
+int yh_ctx_kernel_time(yh_ctx* ctx, int kernel_id, double* total_ms, long* launches) {
+  if (!ctx || kernel_id < 0 || kernel_id > 2) return fail(YH_ERR_ARG, "bad argument");
+  if (int s = set_dev(ctx)) return s;
+  if (int s = drain_timing(ctx)) return s;
+  if (total_ms) *total_ms = ctx->total_ms[kernel_id];
+  if (launches) *launches = ctx->launches[kernel_id];
+  return YH_OK;
+}
 
 int yh_ctx_sync(yh_ctx* ctx) {
   if (!ctx) return fail(YH_ERR_ARG, "null ctx");

@@ -185,6 +185,21 @@ int yh_yashe_mul_keyswitch(yh_ring* ring, const yh_evk* evk, const uint64_t* d_c
  * decodes coefficient 0 only. */
 int yh_yashe_decrypt(yh_ring* ring, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch);
 
+/* ------------------------------------------------------------ instrumentation
+ * Per-kernel timing with HIP events recorded on the context stream around
+ * every launch of the fused kernels (no counterpart in the reference, which
+ * times whole operators with clock_gettime, src/benchmark_yashe.cpp:152-291).
+ * kernel ids: 0 product (CRT + NTTs + pointwise + INTT), 1 keyswitch
+ * multiply-accumulate, 2 ICRT epilogue. */
+#define YH_KERNEL_PROD 0
+#define YH_KERNEL_KEYSWITCH 1
+#define YH_KERNEL_ICRT 2
+int yh_ctx_set_timing(yh_ctx* ctx, int enable);
+/* Synchronises, then returns the summed milliseconds and launch count of
+ * kernel `kernel_id` since timing was enabled (or last reset). */
+int yh_ctx_kernel_time(yh_ctx* ctx, int kernel_id, double* total_ms, long* launches);
+int yh_ctx_reset_timing(yh_ctx* ctx);
+
 /* ------------------------------------------------------------ batching helpers */
 /* Device bytes one operation of `op` (0 ring_mul, 1 yashe_mul, 2 keyswitch,
  * 3 mul+keyswitch, 4 add) needs per batch element, inputs and outputs
