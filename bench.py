@@ -36,7 +36,29 @@ def parse():
     ap.add_argument("--plain-ops", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="run only the warmup + timed batched steps (for rocprofv3 runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a residue add over three 256 MiB buffers (known HBM traffic) for counter calibration")
     return ap.parse_args()
+
+
+def calibrate_copy(ya):
+    """yh_residue_addsub reads two [R][S] u64 buffers and writes one: 3*R*S*8
+    bytes of HBM traffic per launch, nothing cached between launches."""
+    ctx = ya.Context(0)
+    ring = ya.Ring(ctx, 64, Q)
+    S = (256 << 20) // 8 // ring.R
+    nbytes = ring.R * S * 8
+    a, b, c = ctx.alloc(nbytes), ctx.alloc(nbytes), ctx.alloc(nbytes)
+    ctx.sync()
+    for _ in range(5):
+        ring.residue_addsub(c, a, b, S, False)
+    ctx.sync()
+    print(json.dumps({"calibration": "k_residue_addsub", "read_bytes_per_launch": 2 * nbytes,
+                      "write_bytes_per_launch": nbytes, "launches": 5}))
+    ring.close()
+    ctx.close()
 
 
 def uniform_poly(rng, n, q):
@@ -59,6 +81,10 @@ def main():
         sys.exit(2)
 
     import cuyashe_amd as ya
+
+    if args.calibrate_copy:
+        calibrate_copy(ya)
+        return
 
     rng = random.Random(20240613)
     ctx = ya.Context(0)
@@ -96,6 +122,11 @@ def main():
     ctx.set_timing(False)
     ms_per_step = 1000.0 * elapsed / args.steps
     value = B * args.steps / elapsed
+    if args.profile_batched:
+        print(json.dumps({"profile": "batched mul+keyswitch", "batch": B, "steps": args.steps,
+                          "value": round(value, 1), "k_prod_avg_ms": prod_ms / max(prod_n, 1),
+                          "k_keyswitch_avg_ms": ks_ms / max(ks_n, 1), "k_icrt_avg_ms": icrt_ms / max(icrt_n, 1)}))
+        return
 
     # per-step latency (each step synchronised)
     lat = []

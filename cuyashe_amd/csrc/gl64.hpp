@@ -60,8 +60,21 @@ GL_HD uint64_t reduce128(uint64_t hi, uint64_t lo) {
   return t2;
 }
 
+// Full 64x64 -> 128 product from four 32x32 -> 64 partial products
+// (v_mad_u64_u32 on CDNA); no partial sum overflows 64 bits.
+GL_HD void mul128(uint64_t a, uint64_t b, uint64_t& hi, uint64_t& lo) {
+  uint64_t a0 = a & EPS, a1 = a >> 32, b0 = b & EPS, b1 = b >> 32;
+  uint64_t p00 = a0 * b0;
+  uint64_t mid = a0 * b1 + (p00 >> 32);         // < 2^64
+  uint64_t mid2 = a1 * b0 + (mid & EPS);        // < 2^64
+  hi = a1 * b1 + (mid >> 32) + (mid2 >> 32);
+  lo = (mid2 << 32) | (p00 & EPS);
+}
+
 GL_HD uint64_t mul(uint64_t a, uint64_t b) {
-  return reduce128(umulhi(a, b), a * b);
+  uint64_t hi, lo;
+  mul128(a, b, hi, lo);
+  return reduce128(hi, lo);
 }
 
 GL_HD uint64_t pow(uint64_t b, uint64_t e) {

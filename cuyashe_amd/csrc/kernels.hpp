@@ -22,19 +22,25 @@
 #include <stdint.h>
 
 #include "gl64.hpp"
+#include "ntt16.hpp"
 
 namespace yhk {
 
 constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stages)
+
+// Minimum waves per SIMD requested from the compiler for the transform
+// kernels. 2 caps them at 256 VGPRs with no spills; 4 (128 VGPRs) spills
+// ~600 B/lane and measured 20% slower at n = 4096.
+#ifndef YH_MIN_WAVES
+#define YH_MIN_WAVES 2
+#endif
 
 // Device view of one ring's tables (all pointers are device memory).
 struct RingDev {
   int n, logn, LQ, LW;
   int R, w_bits, lw;
   uint64_t t;
-  uint64_t n_inv;
-  const uint64_t* psi_rev;
-  const uint64_t* psiinv_rev;
+  NttTab tab;               // length-n transform tables (negacyclic)
   const uint64_t* primes;
   const uint64_t* barrett;  // floor(2^64 / p_i)
   const uint64_t* pow32;    // [R][2*LW]
@@ -46,6 +52,23 @@ struct RingDev {
   const uint64_t* invMi[2]; // [R]
   const uint64_t* M[2];     // [LW]
 };
+
+// ---------------------------------------------------------------- workgroup mapping
+// MI355X deals consecutive workgroup ids round-robin over its 8 XCDs, each
+// with a private L2. The fused kernels run one workgroup per (operand, prime);
+// this mapping keeps all primes of one operand on one XCD so its coefficient
+// reads (and the shared key/twiddle tables) hit that XCD's L2.
+constexpr int XCDS = 8;
+
+__host__ __device__ inline int xcd_grid(int batch, int R) { return ((batch + XCDS - 1) / XCDS) * XCDS * R; }
+
+// Workgroup w -> (operand, prime); false for padding workgroups.
+__device__ __forceinline__ bool xcd_map(int w, int R, int batch, int& pair, int& prime) {
+  const int x = w % XCDS, k = w / XCDS;
+  pair = (k / R) * XCDS + x;
+  prime = k % R;
+  return pair < batch;
+}
 
 // ---------------------------------------------------------------- small mod p
 
@@ -60,6 +83,13 @@ __device__ __forceinline__ uint64_t mod_p(uint64_t x, uint64_t p, uint64_t b) {
 // (-p/2, p/2]; centring halves the magnitude entering the convolution.
 __device__ __forceinline__ uint64_t center(uint64_t r, uint64_t p) {
   return r > (p >> 1) ? gl::P - (p - r) : r;
+}
+
+// centred residue r times w (the negacyclic twist), as a Goldilocks element
+__device__ __forceinline__ uint64_t twist_centered(uint64_t r, uint64_t p, uint64_t w) {
+  const bool neg = r > (p >> 1);
+  const uint64_t y = mul_small(neg ? p - r : r, w);
+  return neg ? gl::neg(y) : y;
 }
 
 // CRT of one coefficient (LIN 64-bit limbs, limb-major with stride `ls`):
@@ -97,43 +127,6 @@ __device__ __forceinline__ int pass_base(int g, int b0) {
   return ((g >> b0) << (b0 + 4)) | (g & ((1 << b0) - 1));
 }
 
-__device__ __forceinline__ void fwd_stages(uint64_t* x, int base, int b0, int logN, int s0, int ns,
-                                           const uint64_t* __restrict__ T) {
-  for (int s = s0; s < s0 + ns; ++s) {
-    int cb = logN - 1 - s - b0;  // toggled bit within c
-    int m = 1 << cb;
-#pragma unroll
-    for (int c = 0; c < E; ++c) {
-      if (c & m) continue;
-      int j = base + (c << b0);
-      uint64_t w = T[(1 << s) + (j >> (logN - s))];
-      uint64_t U = x[c];
-      uint64_t V = gl::mul(x[c | m], w);
-      x[c] = gl::add(U, V);
-      x[c | m] = gl::sub(U, V);
-    }
-  }
-}
-
-__device__ __forceinline__ void inv_stages(uint64_t* x, int base, int b0, int logN, int u0, int ns,
-                                           const uint64_t* __restrict__ Ti) {
-  int N = 1 << logN;
-  for (int u = u0; u < u0 + ns; ++u) {
-    int cb = u - b0;
-    int m = 1 << cb;
-#pragma unroll
-    for (int c = 0; c < E; ++c) {
-      if (c & m) continue;
-      int j = base + (c << b0);
-      uint64_t w = Ti[(N >> (u + 1)) + (j >> (u + 1))];
-      uint64_t U = x[c];
-      uint64_t V = x[c | m];
-      x[c] = gl::add(U, V);
-      x[c | m] = gl::mul(gl::sub(U, V), w);
-    }
-  }
-}
-
 __device__ __forceinline__ void lds_store(uint64_t* lds, const uint64_t* x, int base, int b0) {
 #pragma unroll
   for (int c = 0; c < E; ++c) lds[lds_idx(base + (c << b0))] = x[c];
@@ -143,47 +136,86 @@ __device__ __forceinline__ void lds_load(const uint64_t* lds, uint64_t* x, int b
   for (int c = 0; c < E; ++c) x[c] = lds[lds_idx(base + (c << b0))];
 }
 
-// Forward transform. On entry x holds the elements of the FIRST pass layout
-// (b0 = logN - 4, i.e. j = g + c*N/16); on exit x holds the LAST pass layout
-// (b0 = 0, j = 16*g + c), ready for a thread-local pointwise step.
-__device__ void ntt_fwd(uint64_t* x, uint64_t* lds, int logN, const uint64_t* __restrict__ T) {
-  int g = threadIdx.x;
-  int first = logN & 3;
-  if (first == 0) first = 4;
-  int s0 = 0;
-  int ns = first;
-  while (true) {
-    int b0 = logN - 4 - s0;
-    int base = pass_base(g, b0);
-    fwd_stages(x, base, b0, logN, s0, ns, T);
-    s0 += ns;
-    if (s0 >= logN) break;
-    lds_store(lds, x, base, b0);
-    __syncthreads();
-    int nb0 = logN - 4 - s0;
-    lds_load(lds, x, pass_base(g, nb0), nb0);
-    __syncthreads();
-    ns = 4;
+// Forward transform, all pass/stage/bit positions fixed at compile time.
+// On entry x holds the first-pass layout (b0 = LOGN - 4: j = g + c*N/16); on
+// exit the last-pass layout (b0 = 0: j = 16*g + c), ready for a thread-local
+// pointwise step.
+template <int LOGN>
+__device__ __forceinline__ void ntt2_fwd(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ T) {
+  constexpr int FIRST = (LOGN & 3) ? (LOGN & 3) : 4;
+  constexpr int NPASS = 1 + (LOGN - FIRST) / 4;
+  const int g = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < NPASS; ++p) {
+    const int s0 = p == 0 ? 0 : FIRST + 4 * (p - 1);
+    const int ns = p == 0 ? FIRST : 4;
+    const int b0 = LOGN - 4 - s0;
+    const int base = pass_base(g, b0);
+    if (p > 0) {
+      lds_load(lds, x, base, b0);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= ns) continue;
+      const int s = s0 + k;
+      const int m = 1 << (3 - k);  // stage s toggles bit LOGN-1-s = b0 + 3 - k
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        if (c & m) continue;
+        const int j = base + (c << b0);
+        const uint64_t w = T[(1 << s) + (j >> (LOGN - s))];
+        const uint64_t U = x[c];
+        const uint64_t V = gl::mul(x[c | m], w);
+        x[c] = gl::add(U, V);
+        x[c | m] = gl::sub(U, V);
+      }
+    }
+    if (p < NPASS - 1) {
+      lds_store(lds, x, base, b0);
+      __syncthreads();
+    }
   }
 }
 
-// Inverse transform. On entry x holds the b0 = 0 layout (as left by ntt_fwd);
-// on exit the b0 = logN - 4 layout (j = g + c*N/16). Not scaled by N^-1.
-__device__ void ntt_inv(uint64_t* x, uint64_t* lds, int logN, const uint64_t* __restrict__ Ti) {
-  int g = threadIdx.x;
-  int u0 = 0;
-  while (true) {
-    int ns = logN - u0 < 4 ? logN - u0 : 4;
-    int b0 = u0 < logN - 4 ? u0 : logN - 4;
-    int base = pass_base(g, b0);
-    inv_stages(x, base, b0, logN, u0, ns, Ti);
-    u0 += ns;
-    if (u0 >= logN) break;
-    lds_store(lds, x, base, b0);
-    __syncthreads();
-    int nb0 = u0 < logN - 4 ? u0 : logN - 4;
-    lds_load(lds, x, pass_base(g, nb0), nb0);
-    __syncthreads();
+// Inverse transform (not scaled by N^-1). On entry the b0 = 0 layout (as
+// left by ntt_fwd); on exit the b0 = LOGN - 4 layout (j = g + c*N/16).
+template <int LOGN>
+__device__ __forceinline__ void ntt2_inv(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ Ti) {
+  constexpr int N = 1 << LOGN;
+  constexpr int NPASS = (LOGN + 3) / 4;
+  const int g = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < NPASS; ++p) {
+    const int u0 = 4 * p;
+    const int ns = LOGN - u0 < 4 ? LOGN - u0 : 4;
+    const int b0 = u0 < LOGN - 4 ? u0 : LOGN - 4;
+    const int off = u0 - b0;  // a short last pass works on the top bits of c
+    const int base = pass_base(g, b0);
+    if (p > 0) {
+      lds_load(lds, x, base, b0);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= ns) continue;
+      const int u = u0 + k;
+      const int m = 1 << (k + off);
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        if (c & m) continue;
+        const int j = base + (c << b0);
+        const uint64_t w = Ti[(N >> (u + 1)) + (j >> (u + 1))];
+        const uint64_t U = x[c];
+        const uint64_t V = x[c | m];
+        x[c] = gl::add(U, V);
+        x[c | m] = gl::mul(gl::sub(U, V), w);
+      }
+    }
+    if (p < NPASS - 1) {
+      lds_store(lds, x, base, b0);
+      __syncthreads();
+    }
   }
 }
 
@@ -215,46 +247,54 @@ struct ProdArgs {
   size_t ostride_a, ostride_b;
   uint32_t* Y;          // [pair][R][N]
   int logN;
-  const uint64_t* T; const uint64_t* Ti;
-  uint64_t n_inv;
+  NttTab tab;           // tables of the length-N transform
   int set;
+  int batch;
 };
 
+// CRT of 16 coefficients into the entry layout of ntt16_fwd (j = g + c*N/16),
+// centred, and for negacyclic transforms twisted by psi^j.
+template <int LOGN, bool NEGA>
 __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
-                                         int logN, const uint64_t* pw, uint64_t p, uint64_t b) {
-  int g = threadIdx.x;
-  int stride = 1 << (logN - 4);
+                                         const uint64_t* pw, uint64_t p, uint64_t b, const uint64_t* twist) {
+  const int g = threadIdx.x;
+  constexpr int stride = 1 << (LOGN - 4);
 #pragma unroll
   for (int c = 0; c < E; ++c) {
-    int j = g + c * stride;
-    uint64_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, p, b) : 0;
-    x[c] = center(r, p);
+    const int j = g + c * stride;
+    const uint64_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, p, b) : 0;
+    x[c] = NEGA ? twist_centered(r, p, twist[j]) : center(r, p);
   }
 }
 
-__global__ void __launch_bounds__(1024) k_prod(RingDev rd, ProdArgs pa) {
+template <int LOGN, bool NEGA>
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_prod(RingDev rd, ProdArgs pa) {
   extern __shared__ uint64_t lds[];
-  const int i = blockIdx.x;
-  const int pair = blockIdx.y;
+  constexpr int N = 1 << LOGN;
+  int pair, i;
+  if (!xcd_map(blockIdx.x, rd.set_R[pa.set], pa.batch, pair, i)) return;
   const uint64_t p = rd.primes[i];
   const uint64_t b = rd.barrett[i];
   const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
   uint64_t xa[E], xb[E];
-  load_crt(xa, pa.A + pair * pa.ostride_a, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.logN, pw, p, b);
-  ntt_fwd(xa, lds, pa.logN, pa.T);
-  __syncthreads();
-  load_crt(xb, pa.B + pair * pa.ostride_b, pa.lin_b, pa.ncoef_b, pa.lstride_b, pa.logN, pw, p, b);
-  ntt_fwd(xb, lds, pa.logN, pa.T);
+  load_crt<LOGN, NEGA>(xa, pa.A + pair * pa.ostride_a, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, p, b, pa.tab.twist);
+  ntt16_fwd<LOGN>(xa, lds, pa.tab.fwd);
+  load_crt<LOGN, NEGA>(xb, pa.B + pair * pa.ostride_b, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, p, b, pa.tab.twist);
+  __syncthreads();  // every thread has left the LDS of the first transform
+  ntt16_fwd<LOGN>(xb, lds, pa.tab.fwd);
 #pragma unroll
   for (int c = 0; c < E; ++c) xa[c] = gl::mul(xa[c], xb[c]);
   __syncthreads();
-  ntt_inv(xa, lds, pa.logN, pa.Ti);
+  ntt16_inv<LOGN>(xa, lds, pa.tab.inv);
   const uint64_t invM = rd.invMi[pa.set][i];
-  const int N = 1 << pa.logN;
   uint32_t* y = pa.Y + ((size_t)pair * rd.set_R[pa.set] + i) * N;
-  const int stride = N >> 4;
+  constexpr int stride = N >> 4;
 #pragma unroll
-  for (int c = 0; c < E; ++c) y[threadIdx.x + c * stride] = finish_residue(xa[c], pa.n_inv, p, b, invM);
+  for (int c = 0; c < E; ++c) {
+    const int j = threadIdx.x + c * stride;
+    const uint64_t v = NEGA ? gl::mul(xa[c], pa.tab.untwist[j]) : gl::mul(xa[c], pa.tab.n_inv);
+    y[j] = finish_residue(v, 1, p, b, invM);
+  }
 }
 
 // Small transforms (N < 64): one thread per (pair, prime) computes the same
@@ -301,6 +341,7 @@ struct KsArgs {
   size_t ostride;
   const uint64_t* evk;      // [lw][R_ks][n] transform domain
   uint32_t* Y;              // [ct][R_ks][n]
+  int batch;
 };
 
 __device__ __forceinline__ uint64_t digit_of(const uint64_t* a, int n, int lq, int k, int wb) {
@@ -311,36 +352,69 @@ __device__ __forceinline__ uint64_t digit_of(const uint64_t* a, int n, int lq, i
   return v & ((1ull << wb) - 1);
 }
 
-__global__ void __launch_bounds__(1024) k_keyswitch(RingDev rd, KsArgs ka) {
+// The multiply-accumulate sum lives in LDS (behind the transform buffer) for
+// n <= 8192, which keeps the kernel free of register spills; n = 16384 has no
+// LDS left for it and keeps it in registers.
+template <int LOGN>
+__host__ __device__ constexpr bool ks_acc_in_lds() { return LOGN <= 13; }
+
+template <int LOGN>
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_keyswitch(RingDev rd, KsArgs ka) {
   extern __shared__ uint64_t lds[];
-  const int i = blockIdx.x;
-  const int ct = blockIdx.y;
+  constexpr int N = 1 << LOGN;
+  constexpr int stride = N >> 4;
+  constexpr bool ACC_LDS = ks_acc_in_lds<LOGN>();
+  const int Rks = rd.set_R[1];
+  int ct, i;
+  if (!xcd_map(blockIdx.x, Rks, ka.batch, ct, i)) return;
   const uint64_t p = rd.primes[i];
   const uint64_t b = rd.barrett[i];
-  const int logN = rd.logn, N = rd.n;
-  const int Rks = rd.set_R[1];
   const uint64_t* c0 = ka.C + ct * ka.ostride;
-  const int stride = N >> 4;
-  uint64_t acc[E], x[E];
+  // accumulator slot c of thread g at acc_lds[c*stride + g]: consecutive
+  // threads touch consecutive words
+  uint64_t* acc_lds = lds + N + (N >> 4);
+  uint64_t acc[ACC_LDS ? 1 : E], x[E];
+  if (!ACC_LDS) {
 #pragma unroll
-  for (int c = 0; c < E; ++c) acc[c] = 0;
+    for (int c = 0; c < E; ++c) acc[c] = 0;
+  }
   for (int k = 0; k < rd.lw; ++k) {
 #pragma unroll
     for (int c = 0; c < E; ++c) {
-      int j = threadIdx.x + c * stride;
-      x[c] = center(mod_p(digit_of(c0 + j, N, rd.LQ, k, rd.w_bits), p, b), p);
+      const int j = threadIdx.x + c * stride;
+      const uint64_t r = mod_p(digit_of(c0 + j, N, rd.LQ, k, rd.w_bits), p, b);
+      x[c] = twist_centered(r, p, rd.tab.twist[j]);
     }
-    ntt_fwd(x, lds, logN, rd.psi_rev);
+    if (k > 0) __syncthreads();  // previous transform's last LDS reads are done
+    ntt16_fwd<LOGN>(x, lds, rd.tab.fwd);
     const uint64_t* ev = ka.evk + ((size_t)k * Rks + i) * N + threadIdx.x * E;
 #pragma unroll
-    for (int c = 0; c < E; ++c) acc[c] = gl::add(acc[c], gl::mul(x[c], ev[c]));
-    __syncthreads();
+    for (int c = 0; c < E; ++c) {
+      const uint64_t prod = gl::mul(x[c], ev[c]);
+      if (ACC_LDS) {
+        uint64_t& a = acc_lds[c * stride + threadIdx.x];
+        a = k == 0 ? prod : gl::add(a, prod);
+      } else {
+        acc[ACC_LDS ? 0 : c] = gl::add(acc[ACC_LDS ? 0 : c], prod);
+      }
+    }
   }
-  ntt_inv(acc, lds, logN, rd.psiinv_rev);
+  __syncthreads();
+  if (ACC_LDS) {
+#pragma unroll
+    for (int c = 0; c < E; ++c) x[c] = acc_lds[c * stride + threadIdx.x];
+  } else {
+#pragma unroll
+    for (int c = 0; c < E; ++c) x[c] = acc[ACC_LDS ? 0 : c];
+  }
+  ntt16_inv<LOGN>(x, lds, rd.tab.inv);
   const uint64_t invM = rd.invMi[1][i];
   uint32_t* y = ka.Y + ((size_t)ct * Rks + i) * N;
 #pragma unroll
-  for (int c = 0; c < E; ++c) y[threadIdx.x + c * stride] = finish_residue(acc[c], rd.n_inv, p, b, invM);
+  for (int c = 0; c < E; ++c) {
+    const int j = threadIdx.x + c * stride;
+    y[j] = finish_residue(gl::mul(x[c], rd.tab.untwist[j]), 1, p, b, invM);
+  }
 }
 
 // Keyswitch for small rings (n < 64): direct negacyclic convolution per thread.
@@ -374,8 +448,27 @@ __global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* gamma_r
 
 // Pre-transform of the keyswitch keys: gamma_k (coefficients, [lw][LQ][n]) ->
 // NTT(gamma_k mod p_i, centred) for every prime of the keyswitch set.
-__global__ void __launch_bounds__(1024) k_evk_prep(RingDev rd, const uint64_t* G, uint64_t* out, int transform) {
+template <int LOGN>
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_evk_prep(RingDev rd, const uint64_t* G, uint64_t* out) {
   extern __shared__ uint64_t lds[];
+  constexpr int N = 1 << LOGN;
+  const int i = blockIdx.x;
+  const int k = blockIdx.y;
+  const int Rks = rd.set_R[1];
+  const uint64_t p = rd.primes[i];
+  const uint64_t b = rd.barrett[i];
+  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  const uint64_t* g = G + (size_t)k * rd.LQ * N;
+  uint64_t* o = out + ((size_t)k * Rks + i) * N;
+  uint64_t x[E];
+  load_crt<LOGN, true>(x, g, rd.LQ, N, N, pw, p, b, rd.tab.twist);
+  ntt16_fwd<LOGN>(x, lds, rd.tab.fwd);
+#pragma unroll
+  for (int c = 0; c < E; ++c) o[threadIdx.x * E + c] = x[c];
+}
+
+// Keyswitch keys of small rings (n < 16): centred residues, no transform.
+__global__ void k_evk_residues(RingDev rd, const uint64_t* G, uint64_t* out) {
   const int i = blockIdx.x;
   const int k = blockIdx.y;
   const int Rks = rd.set_R[1];
@@ -385,15 +478,7 @@ __global__ void __launch_bounds__(1024) k_evk_prep(RingDev rd, const uint64_t* G
   const int N = rd.n;
   const uint64_t* g = G + (size_t)k * rd.LQ * N;
   uint64_t* o = out + ((size_t)k * Rks + i) * N;
-  if (!transform) {
-    for (int j = threadIdx.x; j < N; j += blockDim.x) o[j] = center(crt_residue(g + j, rd.LQ, N, pw, p, b), p);
-    return;
-  }
-  uint64_t x[E];
-  load_crt(x, g, rd.LQ, N, N, rd.logn, pw, p, b);
-  ntt_fwd(x, lds, rd.logn, rd.psi_rev);
-#pragma unroll
-  for (int c = 0; c < E; ++c) o[threadIdx.x * E + c] = x[c];
+  for (int j = threadIdx.x; j < N; j += blockDim.x) o[j] = center(crt_residue(g + j, rd.LQ, N, pw, p, b), p);
 }
 
 // ---------------------------------------------------------------- ICRT + epilogues
@@ -772,18 +857,19 @@ __global__ void k_pointwise(uint64_t* A, const uint64_t* B, size_t count) {
 // (callNTT, src/cuda_functions.cu:447-458, with N^-1 scaling on the inverse).
 __device__ __forceinline__ int bitrev(int v, int bits) { return (int)(__brev((unsigned)v) >> (32 - bits)); }
 
-__global__ void __launch_bounds__(1024) k_ntt_cyclic(uint64_t* data, int logN, const uint64_t* T, const uint64_t* Ti,
-                                                     uint64_t Ninv, int inverse) {
+template <int LOGN>
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ntt_cyclic(uint64_t* data, const uint64_t* T, const uint64_t* Ti,
+                                                                 uint64_t Ninv, int inverse) {
   extern __shared__ uint64_t lds[];
-  const int N = 1 << logN;
+  constexpr int N = 1 << LOGN;
+  constexpr int stride = N >> 4;
   uint64_t* d = data + (size_t)blockIdx.x * N;
   const int g = threadIdx.x;
-  const int stride = N >> 4;
   uint64_t x[E];
   if (!inverse) {
 #pragma unroll
     for (int c = 0; c < E; ++c) x[c] = d[g + c * stride];
-    ntt_fwd(x, lds, logN, T);
+    ntt2_fwd<LOGN>(x, lds, T);
     // bit-reversed -> natural
     __syncthreads();
 #pragma unroll
@@ -791,21 +877,21 @@ __global__ void __launch_bounds__(1024) k_ntt_cyclic(uint64_t* data, int logN, c
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < E; ++c) {
-      int k = g + c * stride;
-      d[k] = lds[lds_idx(bitrev(k, logN))];
+      const int k = g + c * stride;
+      d[k] = lds[lds_idx(bitrev(k, LOGN))];
     }
   } else {
     // natural -> bit-reversed into the b0 = 0 layout
 #pragma unroll
     for (int c = 0; c < E; ++c) {
-      int k = g + c * stride;
-      lds[lds_idx(bitrev(k, logN))] = d[k];
+      const int k = g + c * stride;
+      lds[lds_idx(bitrev(k, LOGN))] = d[k];
     }
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < E; ++c) x[c] = lds[lds_idx(g * E + c)];
     __syncthreads();
-    ntt_inv(x, lds, logN, Ti);
+    ntt2_inv<LOGN>(x, lds, Ti);
 #pragma unroll
     for (int c = 0; c < E; ++c) d[g + c * stride] = gl::mul(x[c], Ninv);
   }

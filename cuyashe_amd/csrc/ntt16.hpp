@@ -1,0 +1,209 @@
+// Radix-16 Goldilocks transforms for the fused product kernels.
+//
+// A length-N transform (N = 2^LOGN, 16 <= N <= 16384) is held by N/16 threads,
+// 16 elements each. Forward = decimation in frequency in passes of four
+// radix-2 stages:
+//   pass p (b0 = LOGN - 4 - 4p, s = 2^b0): thread g takes the 16 elements
+//   base(g) + c*s, runs a 16-point DFT in registers and multiplies output slot
+//   c by w_{16s}^(o * brev4(c)), o = g mod s (the inter-pass twiddle).
+// A short final pass (radix 2^(LOGN mod 4)) runs on 16 consecutive elements.
+// Every root of unity of order <= 64 mod P = 2^64 - 2^32 + 1 is a power of two
+// (7^((P-1)/64) = 2^39), so the twiddles inside a 16-, 8-, 4- or 2-point DFT
+// are shifts with compile-time amounts; only the inter-pass twiddles are full
+// 64x64 products. The inverse runs the passes backwards as decimation in time
+// with inverse roots. The output order of the forward (and input order of the
+// inverse) is a fixed digit-reversed permutation shared by every operand, so
+// pointwise products and the resident keyswitch keys need no reordering.
+//
+// Negacyclic products (mod x^N + 1) twist the input by psi^j on load and
+// untwist by psi^-j N^-1 on the way out (psi^2 = w_N); cyclic products
+// (mod x^N - 1) skip the twist. The reference's transform
+// (src/cuda_functions.cu:404-458) is a radix-2 Stockham stage per launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gl64.hpp"
+
+namespace yhk {
+
+// Device pointers of the tables of one transform length.
+struct NttTab {
+  const uint64_t* fwd;      // inter-pass twiddles, forward
+  const uint64_t* inv;      // inter-pass twiddles, inverse
+  const uint64_t* twist;    // psi^j
+  const uint64_t* untwist;  // psi^-j * N^-1
+  uint64_t n_inv;
+};
+
+__host__ __device__ constexpr int brev4(int c) {
+  return ((c & 1) << 3) | ((c & 2) << 1) | ((c & 4) >> 1) | ((c & 8) >> 3);
+}
+
+// Exponent of the primitive r-th root 7^((P-1)/r) as a power of two.
+__host__ __device__ constexpr int root_exp(int r) { return (39 * (64 / r)) % 192; }
+
+// Offset of pass p's table: passes with b0 > 0 store 15 * 2^b0 entries.
+__host__ __device__ constexpr int pass_offset(int logn, int p) {
+  int off = 0;
+  for (int q = 0; q < p; ++q) {
+    int b0 = logn - 4 - 4 * q;
+    if (b0 > 0) off += 15 << b0;
+  }
+  return off;
+}
+
+// x * 2^e mod P; e is a compile-time constant after unrolling, so the
+// branches fold away.
+__device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int e) {
+  e %= 192;
+  if (e == 0) return x;
+  const bool neg = e >= 96;  // 2^96 = -1
+  if (neg) e -= 96;
+  uint64_t r;
+  if (e == 0) {
+    r = x;
+  } else if (e < 64) {
+    r = gl::reduce128(x >> (64 - e), x << e);
+  } else {
+    // 2^e = 2^(e-64) * 2^64 and 2^64 = 2^32 - 1
+    const int f = e - 64;
+    const uint64_t t = f == 0 ? x : gl::reduce128(x >> (64 - f), x << f);
+    r = gl::sub(gl::reduce128(t >> 32, t << 32), t);
+  }
+  return neg ? gl::neg(r) : r;
+}
+
+// a * b mod P for a < 2^32.
+__device__ __forceinline__ uint64_t mul_small(uint64_t a, uint64_t b) {
+  const uint64_t p0 = a * (b & gl::EPS);
+  const uint64_t p1 = a * (b >> 32) + (p0 >> 32);
+  return gl::reduce128(p1 >> 32, (p1 << 32) | (p0 & gl::EPS));
+}
+
+// In-register DIF of R slots: natural order in, bit-reversed out.
+template <int R>
+__device__ __forceinline__ void dif_r(uint64_t* x) {
+  constexpr int EW = root_exp(R);
+#pragma unroll
+  for (int ln = R; ln >= 2; ln >>= 1) {
+    const int h = ln / 2;
+#pragma unroll
+    for (int i = 0; i < R; i += ln) {
+#pragma unroll
+      for (int j = 0; j < h; ++j) {
+        const uint64_t u = x[i + j], v = x[i + j + h];
+        x[i + j] = gl::add(u, v);
+        x[i + j + h] = mul_pow2(gl::sub(u, v), EW * (R / ln) * j);
+      }
+    }
+  }
+}
+
+// In-register DIT of R slots with inverse roots: bit-reversed in, natural
+// out, unscaled.
+template <int R>
+__device__ __forceinline__ void dit_r(uint64_t* x) {
+  constexpr int EWI = (192 - root_exp(R)) % 192;
+#pragma unroll
+  for (int ln = 2; ln <= R; ln <<= 1) {
+    const int h = ln / 2;
+#pragma unroll
+    for (int i = 0; i < R; i += ln) {
+#pragma unroll
+      for (int j = 0; j < h; ++j) {
+        const uint64_t u = x[i + j];
+        const uint64_t v = mul_pow2(x[i + j + h], EWI * (R / ln) * j);
+        x[i + j] = gl::add(u, v);
+        x[i + j + h] = gl::sub(u, v);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int lds_pad(int j) { return j + (j >> 4); }
+__device__ __forceinline__ int pbase(int g, int b0) { return ((g >> b0) << (b0 + 4)) | (g & ((1 << b0) - 1)); }
+
+__device__ __forceinline__ void st16(uint64_t* lds, const uint64_t* x, int base, int b0) {
+#pragma unroll
+  for (int c = 0; c < 16; ++c) lds[lds_pad(base + (c << b0))] = x[c];
+}
+__device__ __forceinline__ void ld16(const uint64_t* lds, uint64_t* x, int base, int b0) {
+#pragma unroll
+  for (int c = 0; c < 16; ++c) x[c] = lds[lds_pad(base + (c << b0))];
+}
+
+// Forward. Entry: x holds elements g + c*N/16 (natural positions). Exit:
+// x holds the transform at positions 16g + c (permuted order). Leaves the LDS
+// in use: callers synchronise before the next transform writes it.
+template <int LOGN>
+__device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ tw) {
+  constexpr int NFULL = LOGN / 4;
+  constexpr int SHORT = LOGN % 4;
+  const int g = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < NFULL; ++p) {
+    const int b0 = LOGN - 4 - 4 * p;
+    const int base = pbase(g, b0);
+    if (p > 0) {
+      ld16(lds, x, base, b0);
+      __syncthreads();
+    }
+    dif_r<16>(x);
+    if (b0 > 0) {
+      const int s = 1 << b0;
+      const uint64_t* t = tw + pass_offset(LOGN, p) + (g & (s - 1));
+#pragma unroll
+      for (int c = 1; c < 16; ++c) x[c] = gl::mul(x[c], t[(brev4(c) - 1) * s]);
+    }
+    if (p < NFULL - 1 || SHORT) {
+      st16(lds, x, base, b0);
+      __syncthreads();
+    }
+  }
+  if (SHORT) {
+    ld16(lds, x, 16 * g, 0);
+    constexpr int R = 1 << SHORT;
+#pragma unroll
+    for (int blk = 0; blk < 16; blk += R) dif_r<R>(x + blk);
+  }
+}
+
+// Inverse (unscaled, no untwist). Entry: positions 16g + c as left by
+// ntt16_fwd. Exit: x holds elements g + c*N/16. Callers synchronise before
+// calling if another transform used the LDS just before.
+template <int LOGN>
+__device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ twi) {
+  constexpr int NFULL = LOGN / 4;
+  constexpr int SHORT = LOGN % 4;
+  const int g = threadIdx.x;
+  if (SHORT) {
+    constexpr int R = 1 << SHORT;
+#pragma unroll
+    for (int blk = 0; blk < 16; blk += R) dit_r<R>(x + blk);
+    st16(lds, x, 16 * g, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = NFULL - 1; p >= 0; --p) {
+    const int b0 = LOGN - 4 - 4 * p;
+    const int base = pbase(g, b0);
+    if (p < NFULL - 1 || SHORT) {
+      ld16(lds, x, base, b0);
+      __syncthreads();
+    }
+    if (b0 > 0) {
+      const int s = 1 << b0;
+      const uint64_t* t = twi + pass_offset(LOGN, p) + (g & (s - 1));
+#pragma unroll
+      for (int c = 1; c < 16; ++c) x[c] = gl::mul(x[c], t[(brev4(c) - 1) * s]);
+    }
+    dit_r<16>(x);
+    if (p > 0) {
+      st16(lds, x, base, b0);
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace yhk

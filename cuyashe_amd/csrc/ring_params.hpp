@@ -127,8 +127,6 @@ struct RingParams {
   IcrtSet full, ks;
   std::vector<uint64_t> q_limbs;    // [LQ]
   std::vector<uint64_t> mu_limbs;   // [LW+2-LQ]: floor(2^(64*(LW+1)) / q)
-  std::vector<uint64_t> psi_rev, psiinv_rev;  // [n] negacyclic twiddles
-  uint64_t n_inv = 0;
 
   static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
@@ -202,21 +200,55 @@ struct RingParams {
     BigInt mu = BigInt::pow2(64 * (LW + 1)) / q;
     mu_limbs.assign(LMU, 0);
     mu.to_limbs(mu_limbs.data(), LMU);
-    // Negacyclic twiddles: psi a primitive 2n-th root of unity mod P built
-    // from the reference generator 7; psi_rev[k] = psi^bitrev(k).
-    uint64_t psi = gl::pow(7, (gl::P - 1) / (2 * (uint64_t)n));
-    uint64_t psiinv = gl::inv(psi);
-    psi_rev.resize(n);
-    psiinv_rev.resize(n);
-    for (int k = 0; k < n; ++k) {
-      int r = 0;
-      for (int b = 0; b < logn; ++b) if (k >> b & 1) r |= 1 << (logn - 1 - b);
-      psi_rev[k] = gl::pow(psi, (uint64_t)r);
-      psiinv_rev[k] = gl::pow(psiinv, (uint64_t)r);
-    }
-    n_inv = gl::inv((uint64_t)n);
   }
 };
+
+// Tables of the radix-16 transforms of length N (layout documented in
+// ntt16.hpp): inter-pass twiddles w_L^(o*k) (k = 1..15, o < s, per pass with
+// s = 2^b0 > 1), their inverses, the negacyclic twist psi^j and the untwist
+// psi^-j * N^-1, with w_N = 7^((P-1)/N) and psi = 7^((P-1)/(2N)).
+struct Ntt16Tables {
+  std::vector<uint64_t> fwd, inv, twist, untwist;
+  uint64_t n_inv = 0;
+};
+
+inline Ntt16Tables ntt16_tables(int N) {
+  Ntt16Tables t;
+  int logN = 0;
+  while ((1 << logN) < N) ++logN;
+  const uint64_t w = gl::pow(7, (gl::P - 1) / (uint64_t)N);
+  for (int p = 0; p < logN / 4; ++p) {
+    const int b0 = logN - 4 - 4 * p;
+    if (b0 <= 0) continue;
+    const int s = 1 << b0;
+    const uint64_t wL = gl::pow(w, (uint64_t)(N / (16 * s)));
+    const uint64_t wLi = gl::inv(wL);
+    for (int k = 1; k < 16; ++k) {
+      const uint64_t step = gl::pow(wL, (uint64_t)k), stepi = gl::pow(wLi, (uint64_t)k);
+      uint64_t a = 1, ai = 1;
+      for (int o = 0; o < s; ++o) {
+        t.fwd.push_back(a);
+        t.inv.push_back(ai);
+        a = gl::mul(a, step);
+        ai = gl::mul(ai, stepi);
+      }
+    }
+  }
+  if (t.fwd.empty()) { t.fwd.push_back(1); t.inv.push_back(1); }
+  const uint64_t psi = gl::pow(7, (gl::P - 1) / (2 * (uint64_t)N));
+  const uint64_t psii = gl::inv(psi);
+  t.n_inv = gl::inv((uint64_t)N);
+  t.twist.resize(N);
+  t.untwist.resize(N);
+  uint64_t a = 1, ai = t.n_inv;
+  for (int j = 0; j < N; ++j) {
+    t.twist[j] = a;
+    t.untwist[j] = ai;
+    a = gl::mul(a, psi);
+    ai = gl::mul(ai, psii);
+  }
+  return t;
+}
 
 // Twiddle tables of the cyclic length-N transform in the bit-reversed layout
 // consumed by the Cooley-Tukey / Gentleman-Sande passes:

@@ -37,9 +37,14 @@ int fail(int code, const std::string& msg) {
   catch (const std::exception& e) { return fail(YH_ERR_INTERNAL, e.what()); }
 
 struct DevTables {
-  uint64_t* fwd = nullptr;
+  uint64_t* fwd = nullptr;  // radix-2 tables of the standalone cyclic transform
   uint64_t* inv = nullptr;
+  uint64_t* t16 = nullptr;  // radix-16 tables (one allocation)
+  yhk::NttTab tab{};
 };
+
+// Uploads the radix-16 tables of length N into one device allocation.
+int upload_ntt16(int N, uint64_t** buf, yhk::NttTab* tab);
 
 }  // namespace
 
@@ -64,6 +69,7 @@ struct yh_ring {
   yh_ctx* ctx = nullptr;
   std::unique_ptr<yh::RingParams> P;
   uint64_t* d_tables = nullptr;
+  uint64_t* d_ntt = nullptr;   // radix-16 tables of length n
   yhk::RingDev rd{};
   uint32_t* d_y = nullptr;     // ICRT input scratch
   size_t y_bytes = 0;
@@ -127,6 +133,23 @@ int ensure(yh_ctx* ctx, void** buf, size_t* have, size_t need) {
   return YH_OK;
 }
 
+int upload_ntt16(int N, uint64_t** buf, yhk::NttTab* tab) {
+  yh::Ntt16Tables t = yh::ntt16_tables(N);
+  std::vector<uint64_t> h;
+  size_t o_f = h.size(); h.insert(h.end(), t.fwd.begin(), t.fwd.end());
+  size_t o_i = h.size(); h.insert(h.end(), t.inv.begin(), t.inv.end());
+  size_t o_t = h.size(); h.insert(h.end(), t.twist.begin(), t.twist.end());
+  size_t o_u = h.size(); h.insert(h.end(), t.untwist.begin(), t.untwist.end());
+  YH_HIP(hipMalloc(buf, h.size() * sizeof(uint64_t)));
+  YH_HIP(hipMemcpy(*buf, h.data(), h.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  tab->fwd = *buf + o_f;
+  tab->inv = *buf + o_i;
+  tab->twist = *buf + o_t;
+  tab->untwist = *buf + o_u;
+  tab->n_inv = t.n_inv;
+  return YH_OK;
+}
+
 int get_cyclic(yh_ctx* ctx, int N, DevTables* out) {
   std::lock_guard<std::mutex> lk(ctx->mu);
   auto it = ctx->cyclic.find(N);
@@ -134,6 +157,7 @@ int get_cyclic(yh_ctx* ctx, int N, DevTables* out) {
   std::vector<uint64_t> f, iv;
   yh::cyclic_tables(N, f, iv);
   DevTables t;
+  if (int st = upload_ntt16(N, &t.t16, &t.tab)) return st;
   YH_HIP(hipMalloc(&t.fwd, N * sizeof(uint64_t)));
   YH_HIP(hipMalloc(&t.inv, N * sizeof(uint64_t)));
   YH_HIP(hipMemcpy(t.fwd, f.data(), N * sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -147,14 +171,48 @@ int ilog2i(long v) { int l = 0; while ((1L << l) < v) ++l; return l; }
 
 size_t lds_bytes(int N) { return (size_t)(N + N / 16) * sizeof(uint64_t); }
 
+template <int LOGN>
+size_t ks_lds_bytes() {
+  return lds_bytes(1 << LOGN) + (yhk::ks_acc_in_lds<LOGN>() ? ((size_t)1 << LOGN) * sizeof(uint64_t) : 0);
+}
+
+// Every transform kernel is instantiated per log2(N) in [4, 14].
+#define YH_LOGN_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
+
+template <int LOGN>
+void set_lds_attrs() {
+  const int bytes = (int)lds_bytes(1 << LOGN);
+  (void)hipFuncSetAttribute((const void*)yhk::k_prod<LOGN, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)yhk::k_prod<LOGN, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)yhk::k_keyswitch<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)ks_lds_bytes<LOGN>());
+  (void)hipFuncSetAttribute((const void*)yhk::k_evk_prep<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)yhk::k_ntt_cyclic<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
 // Launch the fused product kernel for `batch` pairs on N-point transforms.
-int launch_prod(yh_ring* r, const yhk::ProdArgs& pa, int N, int batch, bool negacyclic) {
+int launch_prod(yh_ring* r, yhk::ProdArgs pa, int N, int batch, bool negacyclic) {
+  pa.batch = batch;
   hipStream_t s = r->ctx->stream;
   int R = r->rd.set_R[pa.set];
   LaunchTimer timer(r->ctx, YH_KERNEL_PROD);
   if (N >= 16) {
-    dim3 grid(R, batch), block(N / 16);
-    hipLaunchKernelGGL(yhk::k_prod, grid, block, lds_bytes(N), s, r->rd, pa);
+    dim3 grid(yhk::xcd_grid(batch, R)), block(N / 16);
+    if (negacyclic) {
+      switch (pa.logN) {
+#define X(L) case L: hipLaunchKernelGGL((yhk::k_prod<L, true>), grid, block, lds_bytes(N), s, r->rd, pa); break;
+        YH_LOGN_CASES(X)
+#undef X
+        default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
+      }
+    } else {
+      switch (pa.logN) {
+#define X(L) case L: hipLaunchKernelGGL((yhk::k_prod<L, false>), grid, block, lds_bytes(N), s, r->rd, pa); break;
+        YH_LOGN_CASES(X)
+#undef X
+        default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
+      }
+    }
   } else {
     dim3 grid(R, batch), block(64);
     hipLaunchKernelGGL(yhk::k_prod_small, grid, block, 0, s, r->rd, pa, negacyclic ? 1 : 0);
@@ -221,9 +279,7 @@ int ring_product(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* out
   pa.ostride_a = pa.ostride_b = (size_t)LQ * n;
   pa.Y = r->d_y;
   pa.logN = r->rd.logn;
-  pa.T = r->rd.psi_rev;
-  pa.Ti = r->rd.psiinv_rev;
-  pa.n_inv = r->rd.n_inv;
+  pa.tab = r->rd.tab;
   pa.set = 0;
   st = launch_prod(r, pa, n, batch, true);
   if (st) return st;
@@ -245,10 +301,17 @@ int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out
   ka.ostride = (size_t)LQ * n;
   ka.evk = k->d;
   ka.Y = r->d_y;
+  ka.batch = batch;
   dim3 grid(Rks, batch);
+  dim3 grid_x(yhk::xcd_grid(batch, Rks));
   LaunchTimer* timer = new LaunchTimer(r->ctx, YH_KERNEL_KEYSWITCH);
   if (n >= 16) {
-    hipLaunchKernelGGL(yhk::k_keyswitch, grid, dim3(n / 16), lds_bytes(n), r->ctx->stream, r->rd, ka);
+    switch (r->rd.logn) {
+#define X(L) case L: hipLaunchKernelGGL(yhk::k_keyswitch<L>, grid_x, dim3(n / 16), ks_lds_bytes<L>(), r->ctx->stream, r->rd, ka); break;
+      YH_LOGN_CASES(X)
+#undef X
+      default: delete timer; return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
+    }
   } else {
     hipLaunchKernelGGL(yhk::k_keyswitch_small, grid, dim3(64), 0, r->ctx->stream, r->rd, ka, k->d);
   }
@@ -275,11 +338,9 @@ int yh_ctx_create(int device, yh_ctx** out) {
   YH_HIP(hipSetDevice(device));
   // Transforms keep a whole length-16384 vector (+1/16 padding) in LDS:
   // 139264 bytes, within gfx950's 160 KiB per workgroup.
-  const int max_lds = (int)lds_bytes(16384);
-  YH_HIP(hipFuncSetAttribute((const void*)yhk::k_prod, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-  YH_HIP(hipFuncSetAttribute((const void*)yhk::k_keyswitch, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-  YH_HIP(hipFuncSetAttribute((const void*)yhk::k_evk_prep, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-  YH_HIP(hipFuncSetAttribute((const void*)yhk::k_ntt_cyclic, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+#define X(L) set_lds_attrs<L>();
+  YH_LOGN_CASES(X)
+#undef X
   auto* c = new yh_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -294,7 +355,11 @@ int yh_ctx_destroy(yh_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   drain_timing(ctx);
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-  for (auto& kv : ctx->cyclic) { (void)hipFree(kv.second.fwd); (void)hipFree(kv.second.inv); }
+  for (auto& kv : ctx->cyclic) {
+    (void)hipFree(kv.second.fwd);
+    (void)hipFree(kv.second.inv);
+    (void)hipFree(kv.second.t16);
+  }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return YH_OK;
@@ -402,7 +467,7 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   // one table buffer: pack every array, remember offsets
   std::vector<uint64_t> h;
   auto put = [&h](const std::vector<uint64_t>& v) { size_t o = h.size(); h.insert(h.end(), v.begin(), v.end()); return o; };
-  size_t o_psi = put(P.psi_rev), o_psii = put(P.psiinv_rev), o_pr = put(P.primes), o_b = put(P.barrett);
+  size_t o_pr = put(P.primes), o_b = put(P.barrett);
   size_t o_pw = put(P.pow32), o_q = put(P.q_limbs), o_mu = put(P.mu_limbs);
   size_t o_mi0 = put(P.full.Mi_limbs), o_inv0 = put(P.full.invMi), o_m0 = put(P.full.M_limbs);
   size_t o_mi1 = put(P.ks.Mi_limbs), o_inv1 = put(P.ks.invMi), o_m1 = put(P.ks.M_limbs);
@@ -412,8 +477,11 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   yhk::RingDev& rd = r->rd;
   rd.n = P.n; rd.logn = P.logn; rd.LQ = P.LQ; rd.LW = P.LW;
   rd.R = (int)P.primes.size(); rd.w_bits = P.w_bits; rd.lw = P.lw;
-  rd.t = P.t; rd.n_inv = P.n_inv;
-  rd.psi_rev = d + o_psi; rd.psiinv_rev = d + o_psii; rd.primes = d + o_pr; rd.barrett = d + o_b;
+  rd.t = P.t;
+  if (P.n >= 16) {
+    if (int st = upload_ntt16(P.n, &r->d_ntt, &rd.tab)) return st;
+  }
+  rd.primes = d + o_pr; rd.barrett = d + o_b;
   rd.pow32 = d + o_pw; rd.q = d + o_q; rd.mu = d + o_mu;
   rd.set_R[0] = P.full.R; rd.Mi[0] = d + o_mi0; rd.invMi[0] = d + o_inv0; rd.M[0] = d + o_m0;
   rd.set_R[1] = P.ks.R; rd.Mi[1] = d + o_mi1; rd.invMi[1] = d + o_inv1; rd.M[1] = d + o_m1;
@@ -427,6 +495,7 @@ int yh_ring_destroy(yh_ring* r) {
   (void)hipSetDevice(r->ctx->device);
   (void)hipStreamSynchronize(r->ctx->stream);
   (void)hipFree(r->d_tables);
+  if (r->d_ntt) (void)hipFree(r->d_ntt);
   if (r->d_y) (void)hipFree(r->d_y);
   if (r->d_tmp) (void)hipFree(r->d_tmp);
   delete r;
@@ -516,8 +585,13 @@ int yh_ntt(yh_ctx* ctx, uint64_t* d_data, int N, int count, int direction) {
   DevTables t;
   if (int s = get_cyclic(ctx, N, &t)) return s;
   uint64_t Ninv = gl::inv((uint64_t)N);
-  hipLaunchKernelGGL(yhk::k_ntt_cyclic, dim3(count), dim3(N / 16), lds_bytes(N), ctx->stream, d_data, ilog2i(N),
-                     t.fwd, t.inv, Ninv, direction == YH_INVERSE ? 1 : 0);
+  const int inv = direction == YH_INVERSE ? 1 : 0;
+  switch (ilog2i(N)) {
+#define X(L) case L: hipLaunchKernelGGL(yhk::k_ntt_cyclic<L>, dim3(count), dim3(N / 16), lds_bytes(N), ctx->stream, d_data, t.fwd, t.inv, Ninv, inv); break;
+    YH_LOGN_CASES(X)
+#undef X
+    default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
+  }
   YH_HIP(hipGetLastError());
   return YH_OK;
 }
@@ -574,8 +648,7 @@ int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t
   pa.ostride_a = pa.ostride_b = 0;
   pa.Y = r->d_y;
   pa.logN = ilog2i(N);
-  pa.T = tw.fwd; pa.Ti = tw.inv;
-  pa.n_inv = gl::inv((uint64_t)N);
+  pa.tab = tw.tab;
   pa.set = 0;
   st = launch_prod(r, pa, (int)N, 1, false);
   if (st) return st;
@@ -644,10 +717,16 @@ int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
   hipError_t e = hipMalloc(&k->d, bytes);
   if (e != hipSuccess) { delete k; return fail(YH_ERR_NOMEM, hipGetErrorString(e)); }
   dim3 grid(r->rd.set_R[1], r->rd.lw);
-  int transform = r->rd.n >= 16 ? 1 : 0;
-  dim3 block(transform ? r->rd.n / 16 : 64);
-  hipLaunchKernelGGL(yhk::k_evk_prep, grid, block, transform ? lds_bytes(r->rd.n) : 0, r->ctx->stream, r->rd,
-                     d_gamma, k->d, transform);
+  if (r->rd.n >= 16) {
+    switch (r->rd.logn) {
+#define X(L) case L: hipLaunchKernelGGL(yhk::k_evk_prep<L>, grid, dim3(r->rd.n / 16), lds_bytes(r->rd.n), r->ctx->stream, r->rd, d_gamma, k->d); break;
+      YH_LOGN_CASES(X)
+#undef X
+      default: break;
+    }
+  } else {
+    hipLaunchKernelGGL(yhk::k_evk_residues, grid, dim3(64), 0, r->ctx->stream, r->rd, d_gamma, k->d);
+  }
   e = hipGetLastError();
   if (e != hipSuccess) { (void)hipFree(k->d); delete k; return fail(YH_ERR_HIP, hipGetErrorString(e)); }
   *out = k;
@@ -701,9 +780,7 @@ int yh_yashe_decrypt(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint6
   pa.ostride_b = (size_t)LQ * n;
   pa.Y = r->d_y;
   pa.logN = r->rd.logn;
-  pa.T = r->rd.psi_rev;
-  pa.Ti = r->rd.psiinv_rev;
-  pa.n_inv = r->rd.n_inv;
+  pa.tab = r->rd.tab;
   pa.set = 0;
   st = launch_prod(r, pa, n, batch, true);
   if (st) return st;
