@@ -70,6 +70,7 @@ _SIGS = {
     "yh_residue_addsub": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
     "yh_poly_mul_plain": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int)]),
+    "yh_poly_mul_plain_length": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "yh_ring_mul": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_ring_addsub": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
     "yh_ring_reduce": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int]),

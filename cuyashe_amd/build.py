@@ -41,8 +41,29 @@ def build_hip(verbose: bool = True) -> str:
     return LIB_HIP
 
 
+HOST_SRCS = ["polynomial.cpp", "yashe.cpp"]
+LIB_HOST = os.path.join(LIBDIR, "libcuyashe_host.so")
+TEST_API = os.path.join(ROOT, "tests", "cpp", "build", "test_api")
+
+
+def build_host() -> str:
+    """The drop-in C++ API (Polynomial / Ciphertext / Yashe) over the C ABI:
+    plain C++ (g++), linked against libyashe_hip.so."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    srcs = [os.path.join(HERE, "host", s) for s in HOST_SRCS]
+    tmp = LIB_HOST + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"), *srcs,
+          "-L", LIBDIR, "-lyashe_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, LIB_HOST)
+    os.makedirs(os.path.dirname(TEST_API), exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(ROOT, "tests", "cpp", "test_api.cpp"),
+          "-L", LIBDIR, "-lcuyashe_host", "-lyashe_hip", "-Wl,-rpath," + LIBDIR, "-Wl,-rpath,$ORIGIN/../../../cuyashe_amd/lib",
+          "-o", TEST_API])
+    return LIB_HOST
+
+
 def build_all():
-    return [build_hip()]
+    return [build_hip(), build_host()]
 
 
 if __name__ == "__main__":

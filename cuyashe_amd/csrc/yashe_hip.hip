@@ -27,7 +27,10 @@ int fail(int code, const std::string& msg) {
 #define YH_HIP(expr)                                                                        \
   do {                                                                                      \
     hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess) return fail(YH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    if (e_ != hipSuccess) {                                                                 \
+      (void)hipGetLastError(); /* consume, so the next call does not report it */           \
+      return fail(YH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    }                                                                                       \
   } while (0)
 
 #define YH_TRY_BEGIN try {
@@ -622,16 +625,9 @@ int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t
   const int n = r->rd.n;
   if (deg_a < 0 || deg_b < 0 || deg_a >= n || deg_b >= n) return fail(YH_ERR_ARG, "operand degrees must be in [0, n)");
   if (out_limbs < 1) return fail(YH_ERR_ARG, "out_limbs must be positive");
-  // N = 2^ceil(log2(deg a + deg b)) as in src/common.h:139, at least min_N
-  int sum = deg_a + deg_b;
-  long N = 1;
-  while (N < sum) N <<= 1;
-  if (N < min_N) N = min_N;
-  if (N < 2) N = 2;
-  // every operand coefficient needs a slot (deg_b = 0 with deg_a a power of two)
-  while (N < deg_a + 1 || N < deg_b + 1) N <<= 1;
+  if (min_N < 1 || (min_N & (min_N - 1))) return fail(YH_ERR_ARG, "min_N must be a power of two");
+  const long N = yh_poly_mul_plain_length(deg_a, deg_b, min_N);
   if (N > 16384) return fail(YH_ERR_UNSUPPORTED, "plain product longer than 16384");
-  if (N & (N - 1)) return fail(YH_ERR_ARG, "min_N must be a power of two");
   if (int s = set_dev(r->ctx)) return s;
   const int R = r->rd.R, LQ = r->rd.LQ;
   int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)R * N * sizeof(uint32_t));
@@ -654,6 +650,18 @@ int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t
   if (st) return st;
   if (N_out) *N_out = (int)N;
   return launch_icrt(r, icrt_args(r, (int)N, (int)N, 0, yhk::EPI_RAW, d_out, out_limbs, N, 0, 1), 1);
+}
+
+int yh_poly_mul_plain_length(int deg_a, int deg_b, int min_N) {
+  // 2^ceil(log2(deg a + deg b)) as in src/common.h:139, at least min_N and 2,
+  // and long enough to hold every operand coefficient (deg_b = 0 with deg_a a
+  // power of two)
+  long N = 1;
+  while (N < (long)deg_a + deg_b) N <<= 1;
+  if (N < min_N) N = min_N;
+  if (N < 2) N = 2;
+  while (N < (long)deg_a + 1 || N < (long)deg_b + 1) N <<= 1;
+  return (int)N;
 }
 
 int yh_ring_mul(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch) {

@@ -148,6 +148,8 @@ int yh_residue_addsub(yh_ring* ring, uint64_t* d_c, const uint64_t* d_a, const u
  * (src/cuda_functions.cu:617-738) -> icrt. */
 int yh_poly_mul_plain(yh_ring* ring, const uint64_t* d_a, int deg_a, const uint64_t* d_b, int deg_b,
                       int min_N, uint64_t* d_out, int out_limbs, int* N_out);
+/* The N that yh_poly_mul_plain uses for these degrees (size d_out from it). */
+int yh_poly_mul_plain_length(int deg_a, int deg_b, int min_N);
 /* c = a * b mod (x^n + 1, q), operands [batch][q_limbs][n] in [0, q).
  * Replaces the sequence c = a*b; c.reduce(); c %= q
  * (src/polynomial.cpp:52-60, src/cuda_functions.cu:1040-1078,

@@ -158,7 +158,7 @@ def ntt_length(deg_a: int, deg_b: int, min_N: int = 1) -> int:
     N = 1
     while N < deg_a + deg_b:
         N <<= 1
-    N = max(N, min_N, 2)
+    N = max(N, min_N, 2)  # at least 2 (a constant times a constant)
     while N < deg_a + 1 or N < deg_b + 1:
         N <<= 1
     return N

@@ -90,3 +90,12 @@ def test_ring_plan_rejects_bad_parameters():
         ya.ring_plan(64, 1)
     with pytest.raises(ya.YashError):
         ya.ring_plan(64, (1 << 600) + 1)
+
+
+def test_plain_length_rule_matches_oracle():
+    from oracle import cpuref
+    L = _lib.lib()
+    for da in range(0, 40):
+        for db in range(0, 40):
+            for mn in (1, 16):
+                assert L.yh_poly_mul_plain_length(da, db, mn) == cpuref.ntt_length(da, db, mn), (da, db, mn)

@@ -368,3 +368,18 @@ def test_decrypt_matches_oracle(gpu_ctx):
     for i in range(2):
         assert [int(v) for v in got[i]] == cpuref.decrypt(f, C[i], n, q, t)
     ring.close()
+
+
+@pytest.mark.parametrize("da,db", [(0, 0), (8, 0), (0, 7), (16, 0), (7, 1), (31, 1), (1, 1)])
+def test_plain_edge_degrees(gpu_ctx, da, db):
+    # constants and power-of-two degrees: the output length rule of the library
+    # (yh_poly_mul_plain_length) sizes every buffer
+    rng = random.Random(da * 100 + db)
+    q = Q127
+    ring = ya.Ring(gpu_ctx, 64, q)
+    a = rand_poly(rng, da + 1, q)
+    b = rand_poly(rng, db + 1, q)
+    got = _plain(gpu_ctx, ring, a, b)
+    N = cpuref.ntt_length(da, db)
+    assert got == cpuref.poly_mul_plain(a, b, N=N, qbits=q.bit_length())
+    ring.close()
