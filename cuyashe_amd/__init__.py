@@ -5,7 +5,7 @@ the C ABI of include/yashe_hip.h); this package only binds it for Python
 callers (tests, benchmarks, the serving loop).
 """
 from ._lib import (  # noqa: F401
-    Context, Ring, Evk, DeviceBuffer, YashError, lib, LIB_PATH,
+    Context, Ring, Evk, Graph, DeviceBuffer, YashError, lib, LIB_PATH,
     ints_to_array, array_to_ints, int_to_limbs, ring_plan,
     OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD,
     KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT,

@@ -85,6 +85,12 @@ _SIGS = {
     "yh_ctx_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_long)]),
     "yh_ctx_reset_timing": (ctypes.c_int, [_vp]),
+    "yh_ring_mul_bcast": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_ring_inverse": (ctypes.c_int, [_vp, _vp, _vp]),
+    "yh_graph_begin": (ctypes.c_int, [_vp]),
+    "yh_graph_end": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "yh_graph_launch": (ctypes.c_int, [_vp]),
+    "yh_graph_destroy": (ctypes.c_int, [_vp]),
 }
 
 
@@ -172,6 +178,13 @@ class DeviceBuffer:
         except Exception:
             pass
 
+    def view(self, offset: int, nbytes: int = None) -> "BufferView":
+        """A non-owning window [offset, offset + nbytes) of this buffer."""
+        nbytes = self.nbytes - offset if nbytes is None else int(nbytes)
+        if offset < 0 or nbytes < 0 or offset + nbytes > self.nbytes:
+            raise YashError(f"view [{offset}, {offset + nbytes}) outside a buffer of {self.nbytes} bytes")
+        return BufferView(self, offset, nbytes)
+
     def upload(self, arr: np.ndarray):
         a = np.ascontiguousarray(arr)
         assert a.nbytes <= self.nbytes
@@ -182,6 +195,22 @@ class DeviceBuffer:
         assert out.nbytes <= self.nbytes
         check(lib().yh_memcpy_dtoh(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "yh_memcpy_dtoh")
         return out
+
+
+class BufferView(DeviceBuffer):
+    """Window of a DeviceBuffer; keeps the owner alive, never frees."""
+
+    def __init__(self, owner: DeviceBuffer, offset: int, nbytes: int):
+        self.ctx = owner.ctx
+        self.owner = owner
+        self.nbytes = nbytes
+        self.ptr = owner.ptr + offset
+
+    def free(self):
+        self.ptr = None
+
+    def __del__(self):
+        pass
 
 
 class Context:
@@ -225,9 +254,33 @@ class Context:
     def pointwise(self, a: DeviceBuffer, b: DeviceBuffer, count: int):
         check(lib().yh_pointwise_mulmod(self.h, a.ptr, b.ptr, count), "yh_pointwise_mulmod")
 
+    def graph_begin(self):
+        """Start capturing the context stream's launches into a HIP graph."""
+        check(lib().yh_graph_begin(self.h), "yh_graph_begin")
+
+    def graph_end(self) -> "Graph":
+        h = _vp()
+        check(lib().yh_graph_end(self.h, ctypes.byref(h)), "yh_graph_end")
+        return Graph(h.value)
+
     def close(self):
         if self.h:
             lib().yh_ctx_destroy(self.h)
+            self.h = None
+
+
+class Graph:
+    """An instantiated capture of ring operations, replayed on the context stream."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self):
+        check(lib().yh_graph_launch(self.h), "yh_graph_launch")
+
+    def close(self):
+        if self.h:
+            lib().yh_graph_destroy(self.h)
             self.h = None
 
 
@@ -294,6 +347,14 @@ class Ring:
 
     def ring_mul(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
         check(lib().yh_ring_mul(self.h, a.ptr, b.ptr, out.ptr, batch), "yh_ring_mul")
+
+    def ring_mul_bcast(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
+        """out_k = a * b_k for one a and a batch of b."""
+        check(lib().yh_ring_mul_bcast(self.h, a.ptr, b.ptr, out.ptr, batch), "yh_ring_mul_bcast")
+
+    def inverse(self, f: DeviceBuffer, out: DeviceBuffer):
+        """out = f^-1 mod (x^n + 1, q); raises if f is not invertible."""
+        check(lib().yh_ring_inverse(self.h, f.ptr, out.ptr), "yh_ring_inverse")
 
     def ring_addsub(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1, sub: bool = False):
         check(lib().yh_ring_addsub(self.h, a.ptr, b.ptr, out.ptr, batch, YH_SUB if sub else YH_ADD), "yh_ring_addsub")

@@ -963,4 +963,42 @@ __global__ void k_reduce(RingDev rd, const uint64_t* C, int lin, uint64_t* out) 
   for (int l = 0; l < LQ; ++l) o[(size_t)l * n] = rem[l];
 }
 
+// Helpers of the norm-recursion inverse on [LQ][m] ring elements in [0, q).
+// g_j = (-1)^j f_j mod q (f(-x))
+template <int LQ>
+__global__ void k_negate_odd(RingDev rd, const uint64_t* F, uint64_t* G, int m) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  uint64_t v[LQ];
+  bool nz = false;
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) { v[l] = F[(size_t)l * m + j]; nz |= v[l] != 0; }
+  if ((j & 1) && nz) {
+    uint64_t qv[LQ];
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) qv[l] = rd.q[l];
+    sub_n<LQ>(qv, v);
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) v[l] = qv[l];
+  }
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) G[(size_t)l * m + j] = v[l];
+}
+
+// out_j = in_{2j}, j < m/2 (an even polynomial of degree < m as one in x^2)
+__global__ void k_take_even(const uint64_t* in, uint64_t* out, int m, int LQ) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = m / 2;
+  if (j >= h) return;
+  for (int l = 0; l < LQ; ++l) out[(size_t)l * h + j] = in[(size_t)l * m + 2 * j];
+}
+
+// out_{2j} = in_j, out_{2j+1} = 0, j < m/2 (p(y) -> p(x^2))
+__global__ void k_spread_even(const uint64_t* in, uint64_t* out, int m, int LQ) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int h = m / 2;
+  for (int l = 0; l < LQ; ++l) out[(size_t)l * m + j] = (j & 1) ? 0 : in[(size_t)l * h + j / 2];
+}
+
 }  // namespace yhk

@@ -66,6 +66,13 @@ struct yh_ctx {
   std::vector<hipEvent_t> event_pool;
   double total_ms[3] = {0, 0, 0};
   long launches[3] = {0, 0, 0};
+  bool capturing = false;
+};
+
+struct yh_graph {
+  yh_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct yh_ring {
@@ -78,6 +85,7 @@ struct yh_ring {
   size_t y_bytes = 0;
   uint64_t* d_tmp = nullptr;   // ring-element scratch (mul+keyswitch intermediate)
   size_t tmp_bytes = 0;
+  std::map<int, yh_ring*> sub;  // rings x^m + 1 over the same q, m < n (inverse recursion)
 };
 
 struct yh_evk {
@@ -104,7 +112,7 @@ struct LaunchTimer {
   int kernel;
   hipEvent_t start = nullptr;
   LaunchTimer(yh_ctx* c, int k) : ctx(c), kernel(k) {
-    if (ctx->timing) {
+    if (ctx->timing && !ctx->capturing) {
       start = take_event(ctx);
       if (start) (void)hipEventRecord(start, ctx->stream);
     }
@@ -125,6 +133,7 @@ int set_dev(yh_ctx* ctx) {
 
 int ensure(yh_ctx* ctx, void** buf, size_t* have, size_t need) {
   if (*have >= need) return YH_OK;
+  if (ctx->capturing) return fail(YH_ERR_ARG, "scratch would grow during graph capture; run the sequence once first");
   if (*buf) {
     YH_HIP(hipStreamSynchronize(ctx->stream));
     YH_HIP(hipFree(*buf));
@@ -157,6 +166,7 @@ int get_cyclic(yh_ctx* ctx, int N, DevTables* out) {
   std::lock_guard<std::mutex> lk(ctx->mu);
   auto it = ctx->cyclic.find(N);
   if (it != ctx->cyclic.end()) { *out = it->second; return YH_OK; }
+  if (ctx->capturing) return fail(YH_ERR_ARG, "transform tables would be built during graph capture");
   std::vector<uint64_t> f, iv;
   yh::cyclic_tables(N, f, iv);
   DevTables t;
@@ -411,6 +421,7 @@ int yh_ctx_kernel_time(yh_ctx* ctx, int kernel_id, double* total_ms, long* launc
 
 int yh_ctx_sync(yh_ctx* ctx) {
   if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (ctx->capturing) return fail(YH_ERR_ARG, "synchronize during graph capture");
   if (int s = set_dev(ctx)) return s;
   YH_HIP(hipStreamSynchronize(ctx->stream));
   return YH_OK;
@@ -428,6 +439,7 @@ int yh_malloc(yh_ctx* ctx, size_t bytes, void** dptr) {
 int yh_free(yh_ctx* ctx, void* dptr) {
   if (!ctx) return fail(YH_ERR_ARG, "null ctx");
   if (!dptr) return YH_OK;
+  if (ctx->capturing) return fail(YH_ERR_ARG, "free during graph capture");
   if (int s = set_dev(ctx)) return s;
   YH_HIP(hipStreamSynchronize(ctx->stream));
   YH_HIP(hipFree(dptr));
@@ -436,6 +448,7 @@ int yh_free(yh_ctx* ctx, void* dptr) {
 
 static int memcpy_kind(yh_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
   if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (ctx->capturing) return fail(YH_ERR_ARG, "synchronous copy during graph capture");
   if (bytes == 0) return YH_OK;
   if (!dst || !src) return fail(YH_ERR_ARG, "null pointer");
   if (int s = set_dev(ctx)) return s;
@@ -454,6 +467,7 @@ int yh_memcpy_dtod(yh_ctx* ctx, void* dst, const void* src, size_t bytes) {
 }
 int yh_memset(yh_ctx* ctx, void* dst, int value, size_t bytes) {
   if (!ctx || !dst) return fail(YH_ERR_ARG, "null argument");
+  if (ctx->capturing) return fail(YH_ERR_ARG, "memset nodes are not captured");
   if (int s = set_dev(ctx)) return s;
   YH_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
   return YH_OK;
@@ -497,6 +511,7 @@ int yh_ring_destroy(yh_ring* r) {
   if (!r) return YH_OK;
   (void)hipSetDevice(r->ctx->device);
   (void)hipStreamSynchronize(r->ctx->stream);
+  for (auto& kv : r->sub) yh_ring_destroy(kv.second);
   (void)hipFree(r->d_tables);
   if (r->d_ntt) (void)hipFree(r->d_ntt);
   if (r->d_y) (void)hipFree(r->d_y);
@@ -793,6 +808,157 @@ int yh_yashe_decrypt(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint6
   st = launch_prod(r, pa, n, batch, true);
   if (st) return st;
   return launch_icrt(r, icrt_args(r, n, n, 0, yhk::EPI_DECRYPT, d_m, 1, n, n, 0), batch);
+}
+
+int yh_ring_mul_bcast(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch) {
+  if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (int s = set_dev(r->ctx)) return s;
+  const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
+  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  if (st) return st;
+  yhk::ProdArgs pa{};
+  pa.A = d_a; pa.B = d_b;
+  pa.lin_a = pa.lin_b = LQ;
+  pa.ncoef_a = pa.ncoef_b = n;
+  pa.lstride_a = pa.lstride_b = n;
+  pa.ostride_a = 0;
+  pa.ostride_b = (size_t)LQ * n;
+  pa.Y = r->d_y;
+  pa.logN = r->rd.logn;
+  pa.tab = r->rd.tab;
+  pa.set = 0;
+  st = launch_prod(r, pa, n, batch, true);
+  if (st) return st;
+  return launch_icrt(r, icrt_args(r, n, n, 0, yhk::EPI_MODQ, d_out, LQ, n, (size_t)LQ * n, 0), batch);
+}
+
+static int sub_ring(yh_ring* r, int m, yh_ring** out) {
+  if (m == r->rd.n) { *out = r; return YH_OK; }
+  auto it = r->sub.find(m);
+  if (it != r->sub.end()) { *out = it->second; return YH_OK; }
+  std::vector<uint64_t> ql(r->P->q_limbs);
+  yh_ring* s = nullptr;
+  int st = yh_ring_create(r->ctx, m, ql.data(), (int)ql.size(), 0, 0, &s);
+  if (st) return st;
+  r->sub[m] = s;
+  *out = s;
+  return YH_OK;
+}
+
+extern "C++" {
+template <int LQ>
+static void launch_negate_odd(yh_ring* r, const uint64_t* F, uint64_t* G, int m) {
+  hipLaunchKernelGGL(yhk::k_negate_odd<LQ>, dim3((m + 255) / 256), dim3(256), 0, r->ctx->stream, r->rd, F, G, m);
+}
+}
+
+int yh_ring_inverse(yh_ring* r, const uint64_t* d_f, uint64_t* d_out) {
+  if (!r || !d_f || !d_out) return fail(YH_ERR_ARG, "null argument");
+  if (r->ctx->capturing) return fail(YH_ERR_ARG, "yh_ring_inverse cannot be captured");
+  if (int s = set_dev(r->ctx)) return s;
+  YH_TRY_BEGIN
+  const int n = r->rd.n, LQ = r->rd.LQ;
+  const size_t elt = (size_t)LQ * n * sizeof(uint64_t);
+  // level buffers: F_l (size n_l), G_l = F_l(-x); one allocation
+  std::vector<int> sizes;
+  for (int m = n; m >= 2; m /= 2) sizes.push_back(m);
+  uint64_t* pool = nullptr;
+  YH_HIP(hipMalloc(&pool, 6 * elt));
+  std::unique_ptr<uint64_t, void (*)(uint64_t*)> guard(pool, [](uint64_t* p) { (void)hipFree(p); });
+  uint64_t* F = pool;                // current F_l
+  uint64_t* Gs = pool + 1 * (elt / 8);  // G_l for every level, packed (total < 2 elt)
+  uint64_t* T = pool + 3 * (elt / 8);   // product scratch
+  uint64_t* U = pool + 4 * (elt / 8);   // inverse of the level below, spread
+  YH_HIP(hipMemcpyAsync(F, d_f, elt, hipMemcpyDeviceToDevice, r->ctx->stream));
+  std::vector<size_t> goff;
+  size_t off = 0;
+  for (int m : sizes) {
+    yh_ring* s = nullptr;
+    if (int st = sub_ring(r, m, &s)) return st;
+    uint64_t* G = Gs + off;
+    goff.push_back(off);
+    off += (size_t)LQ * m;
+    switch (LQ) {
+      case 1: launch_negate_odd<1>(s, F, G, m); break;
+      case 2: launch_negate_odd<2>(s, F, G, m); break;
+      case 3: launch_negate_odd<3>(s, F, G, m); break;
+      case 4: launch_negate_odd<4>(s, F, G, m); break;
+      case 5: launch_negate_odd<5>(s, F, G, m); break;
+      case 6: launch_negate_odd<6>(s, F, G, m); break;
+      case 7: launch_negate_odd<7>(s, F, G, m); break;
+      default: launch_negate_odd<8>(s, F, G, m); break;
+    }
+    YH_HIP(hipGetLastError());
+    if (int st = yh_ring_mul(s, F, G, T, 1)) return st;  // N(F)(x^2)
+    hipLaunchKernelGGL(yhk::k_take_even, dim3((m / 2 + 255) / 256), dim3(256), 0, r->ctx->stream, T, F, m, LQ);
+    YH_HIP(hipGetLastError());
+  }
+  // F now holds the scalar N(...)(y) in Z_q[y]/(y + 1): invert it on the host
+  std::vector<uint64_t> h(LQ);
+  YH_HIP(hipMemcpyAsync(h.data(), F, LQ * sizeof(uint64_t), hipMemcpyDeviceToHost, r->ctx->stream));
+  YH_HIP(hipStreamSynchronize(r->ctx->stream));
+  yh::BigInt c0 = yh::BigInt::from_limbs(h.data(), LQ);
+  if (c0.is_zero()) return fail(YH_ERR_ARG, "not invertible");
+  yh::BigInt ci;
+  try {
+    ci = yh::BigInt::inv_mod(c0, r->P->q);
+  } catch (const std::domain_error&) {
+    return fail(YH_ERR_ARG, "not invertible");
+  }
+  ci.to_limbs(h.data(), LQ);
+  YH_HIP(hipMemcpyAsync(F, h.data(), LQ * sizeof(uint64_t), hipMemcpyHostToDevice, r->ctx->stream));
+  // back up: Inv_l = G_l * Inv_{l+1}(x^2)
+  for (int k = (int)sizes.size() - 1; k >= 0; --k) {
+    const int m = sizes[k];
+    yh_ring* s = nullptr;
+    if (int st = sub_ring(r, m, &s)) return st;
+    hipLaunchKernelGGL(yhk::k_spread_even, dim3((m + 255) / 256), dim3(256), 0, r->ctx->stream, F, U, m, LQ);
+    YH_HIP(hipGetLastError());
+    uint64_t* dst = k == 0 ? d_out : F;
+    if (int st = yh_ring_mul(s, Gs + goff[k], U, dst, 1)) return st;
+  }
+  YH_HIP(hipStreamSynchronize(r->ctx->stream));
+  return YH_OK;
+  YH_TRY_END
+}
+
+int yh_graph_begin(yh_ctx* ctx) {
+  if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (ctx->capturing) return fail(YH_ERR_ARG, "already capturing");
+  if (int s = set_dev(ctx)) return s;
+  if (int s = drain_timing(ctx)) return s;
+  YH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  return YH_OK;
+}
+
+int yh_graph_end(yh_ctx* ctx, yh_graph** out) {
+  if (!ctx || !out) return fail(YH_ERR_ARG, "null argument");
+  if (!ctx->capturing) return fail(YH_ERR_ARG, "not capturing");
+  ctx->capturing = false;
+  auto g = std::make_unique<yh_graph>();
+  g->ctx = ctx;
+  YH_HIP(hipStreamEndCapture(ctx->stream, &g->graph));
+  YH_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return YH_OK;
+}
+
+int yh_graph_launch(yh_graph* g) {
+  if (!g) return fail(YH_ERR_ARG, "null graph");
+  if (int s = set_dev(g->ctx)) return s;
+  YH_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
+  return YH_OK;
+}
+
+int yh_graph_destroy(yh_graph* g) {
+  if (!g) return YH_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return YH_OK;
 }
 
 size_t yh_mem_footprint(const yh_ring* r, int op) {

@@ -1,0 +1,143 @@
+"""YASHE key generation, encryption and decryption on device-resident data.
+
+Mirrors the reference's Yashe class (src/yashe.cpp, src/yashe.h) for Python
+callers such as the serving loop: every ring operation runs through the C ABI
+of libyashe_hip.so, and only the small random samples (ternary and discrete
+Gaussian polynomials, messages) are drawn on the host and uploaded once.
+
+    keys:    f = t f' + 1 (f' ternary, f invertible mod q),
+             h = t g f^-1,                          src/yashe.cpp:37-69
+             gamma_k = f w^k + e_k + h s_k, k < lw   src/yashe.cpp:70-93
+    encrypt: c = h s + e + floor(q/t) m             src/yashe.cpp:104-142
+    decrypt: m = [round(t/q [f c]_q)]_t (f f after a mul without
+             keyswitch)                             src/yashe.cpp:167-197
+
+The samplers are not bit-compatible with the reference's (it draws from
+rand()/NTL's RNG); what matters for parity is the arithmetic on the samples.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+from ._lib import Context, DeviceBuffer, Ring, YashError, ints_to_array
+
+GAUSS_SIGMA = 8.0   # src/distribution.h: discrete Gaussian with sigma 8
+GAUSS_BOUND = 48    # truncated at 6 sigma
+
+
+class Yashe:
+    def __init__(self, ctx: Context, n: int, q: int, t: int, w_bits: int = 32, seed: int = 0,
+                 ring: Optional[Ring] = None):
+        self.ctx = ctx
+        self.ring = ring if ring is not None else Ring(ctx, n, q, t=t, w_bits=w_bits)
+        self.n, self.q, self.t = int(n), int(q), int(t)
+        self.L = self.ring.L
+        self.lw = self.ring.lw
+        self.delta = self.q // self.t
+        self.rng = np.random.default_rng(seed)
+        self._ternary = ints_to_array([self.q - 1, 0, 1], self.L)            # index v + 1
+        self._gauss = ints_to_array([v % self.q for v in range(-GAUSS_BOUND, GAUSS_BOUND + 1)], self.L)
+        self._dm = ints_to_array([self.delta * m for m in range(self.t)], self.L) if self.t <= (1 << 20) else None
+        self.elt_bytes = self.L * self.n * 8
+        self.f = self.ff = self.h = self.evk = None
+
+    # ------------------------------------------------------------ sampling
+    def ternary(self, batch: int) -> np.ndarray:
+        return self.rng.integers(-1, 2, size=(batch, self.n), dtype=np.int64)
+
+    def gaussian(self, batch: int) -> np.ndarray:
+        e = np.rint(self.rng.normal(0.0, GAUSS_SIGMA, size=(batch, self.n))).astype(np.int64)
+        return np.clip(e, -GAUSS_BOUND, GAUSS_BOUND)
+
+    def _lookup(self, table: np.ndarray, idx: np.ndarray) -> np.ndarray:
+        """[batch][L][n] limbs of table[idx]."""
+        return np.ascontiguousarray(np.transpose(table[:, idx], (1, 0, 2)))
+
+    def pack_small(self, v: np.ndarray, kind: str) -> np.ndarray:
+        if kind == "ternary":
+            return self._lookup(self._ternary, v + 1)
+        if kind == "gauss":
+            return self._lookup(self._gauss, v + GAUSS_BOUND)
+        raise ValueError(kind)
+
+    def pack_scaled_message(self, m: np.ndarray) -> np.ndarray:
+        """floor(q/t) m mod q as limbs; m in [0, t)."""
+        m = np.asarray(m, dtype=np.int64)
+        if self._dm is not None:
+            return self._lookup(self._dm, m)
+        return np.stack([ints_to_array([self.delta * int(x) for x in row], self.L) for row in m])
+
+    # ------------------------------------------------------------ keys
+    def generate_keys(self):
+        n, q, t = self.n, self.q, self.t
+        ring = self.ring
+        while True:
+            fp = self.ternary(1)[0]
+            f_vals = [(t * int(x) + (1 if j == 0 else 0)) % q for j, x in enumerate(fp)]
+            d_f = self.ctx.to_device(ints_to_array(f_vals, self.L)[None])
+            finv = self.ctx.alloc(self.elt_bytes)
+            try:
+                ring.inverse(d_f, finv)
+                break
+            except YashError:  # f not invertible: draw again (src/yashe.cpp:45-55)
+                continue
+        g = self.ternary(1)[0]
+        d_tg = self.ctx.to_device(ints_to_array([(t * int(x)) % q for x in g], self.L)[None])
+        self.h = self.ctx.alloc(self.elt_bytes)
+        ring.ring_mul(d_tg, finv, self.h, 1)
+        self.f = d_f
+        self.ff = self.ctx.alloc(self.elt_bytes)
+        ring.ring_mul(d_f, d_f, self.ff, 1)
+        if self.lw:
+            lw = self.lw
+            S = self.ctx.to_device(self.pack_small(self.ternary(lw), "ternary"))
+            gam = self.ctx.alloc(lw * self.elt_bytes)
+            ring.ring_mul_bcast(self.h, S, gam, lw)
+            E = self.ctx.to_device(self.pack_small(self.gaussian(lw), "gauss"))
+            ring.ring_addsub(gam, E, gam, lw)
+            fw = np.stack([ints_to_array([(v << (ring.w_bits * k)) % q for v in f_vals], self.L) for k in range(lw)])
+            FW = self.ctx.to_device(fw)
+            ring.ring_addsub(gam, FW, gam, lw)
+            self.evk = ring.evk(gam)
+            self.gamma = gam
+        return self
+
+    # ------------------------------------------------------------ ciphertexts
+    def encrypt(self, m: np.ndarray, out: Optional[DeviceBuffer] = None) -> DeviceBuffer:
+        """Encrypt a [batch][n] array of messages in [0, t) into device
+        ciphertexts [batch][L][n]."""
+        m = np.atleast_2d(np.asarray(m, dtype=np.int64))
+        batch = m.shape[0]
+        ring = self.ring
+        out = out if out is not None else self.ctx.alloc(batch * self.elt_bytes)
+        S = self.ctx.to_device(self.pack_small(self.ternary(batch), "ternary"))
+        ring.ring_mul_bcast(self.h, S, out, batch)
+        E = self.ctx.to_device(self.pack_small(self.gaussian(batch), "gauss"))
+        ring.ring_addsub(out, E, out, batch)
+        DM = self.ctx.to_device(self.pack_scaled_message(m))
+        ring.ring_addsub(out, DM, out, batch)
+        return out
+
+    def decrypt(self, c: DeviceBuffer, batch: int = 1, aftermul: bool = False) -> np.ndarray:
+        dm = self.ctx.alloc(batch * self.n * 8)
+        self.ring.decrypt(self.ff if aftermul else self.f, c, dm, batch)
+        return dm.download((batch, self.n))
+
+    def add(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
+        self.ring.ring_addsub(a, b, out, batch)
+
+    def mul_keyswitch(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
+        self.ring.mul_keyswitch(self.evk, a, b, out, batch)
+
+
+def negacyclic_mul_mod(a: np.ndarray, b: np.ndarray, t: int) -> np.ndarray:
+    """a b mod (x^n + 1, t) for message polynomials (exact in int64 while
+    n (t - 1)^2 < 2^63)."""
+    n = a.shape[-1]
+    assert n * (t - 1) ** 2 < (1 << 63)
+    full = np.convolve(np.mod(a, t).astype(np.int64), np.mod(b, t).astype(np.int64))
+    out = full[:n].copy()
+    out[: n - 1] -= full[n:]
+    return np.mod(out, t)

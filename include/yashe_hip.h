@@ -187,6 +187,28 @@ int yh_yashe_mul_keyswitch(yh_ring* ring, const yh_evk* evk, const uint64_t* d_c
  * decodes coefficient 0 only. */
 int yh_yashe_decrypt(yh_ring* ring, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch);
 
+/* c_b = a * b_b mod (x^n + 1, q) for one a and a batch of b (encryption's
+ * h*s over a batch). */
+int yh_ring_mul_bcast(yh_ring* ring, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch);
+/* f^-1 mod (x^n + 1, q) on the device, by the norm recursion
+ * f^-1(x) = f(-x) * N(f)^-1(x^2), N(f)(x^2) = f(x) f(-x): log2(n) ring
+ * products and one scalar inverse. Returns YH_ERR_ARG if f is not
+ * invertible. Replaces Polynomial::InvMod (src/polynomial.h:1338-1367, NTL
+ * on the host) for key generation (src/yashe.cpp:37-55). */
+int yh_ring_inverse(yh_ring* ring, const uint64_t* d_f, uint64_t* d_out);
+
+/* ------------------------------------------------------------ graphs
+ * Captures the launches issued on the context stream into a HIP graph
+ * (hipStreamBeginCapture), for replay of a fixed homomorphic circuit. Run
+ * the same sequence once before capturing so scratch buffers are sized:
+ * a call that would allocate during capture fails with YH_ERR_ARG. No
+ * counterpart in the reference (it launches everything eagerly). */
+typedef struct yh_graph yh_graph;
+int yh_graph_begin(yh_ctx* ctx);
+int yh_graph_end(yh_ctx* ctx, yh_graph** out);
+int yh_graph_launch(yh_graph* graph);
+int yh_graph_destroy(yh_graph* graph);
+
 /* ------------------------------------------------------------ instrumentation
  * Per-kernel timing with HIP events recorded on the context stream around
  * every launch of the fused kernels (no counterpart in the reference, which

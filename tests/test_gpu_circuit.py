@@ -1,0 +1,201 @@
+"""Key generation on the device, the scheme layer, and a homomorphic circuit
+captured into a HIP graph and replayed.
+
+Bit-exact against the oracle where it is affordable; everything else through
+decryption (Dec(Enc(m1) Enc(m2)) = m1 m2 mod t) and replay == eager.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import cuyashe_amd as ya
+from cuyashe_amd.scheme import Yashe, negacyclic_mul_mod
+from oracle import cpuref
+
+pytestmark = pytest.mark.gpu
+
+Q127 = (1 << 127) - 1
+Q255 = (1 << 255) - 19
+Q_POLY = 77287149995192912462927307869  # src/test.cu:45
+
+
+def rand_poly(rng, n, q):
+    return [rng.randrange(q) for _ in range(n)]
+
+
+def put(ctx, ring, polys):
+    return ctx.to_device(ring.pack(polys))
+
+
+def get(buf, ring, batch):
+    return ring.unpack(buf.download((batch, ring.L, ring.n)))
+
+
+@pytest.mark.parametrize("n,q", [(2, Q_POLY), (8, Q_POLY), (64, Q127), (256, Q255)])
+def test_ring_inverse_matches_oracle(gpu_ctx, n, q):
+    rng = random.Random(n)
+    ring = ya.Ring(gpu_ctx, n, q)
+    f = rand_poly(rng, n, q)
+    out = gpu_ctx.alloc(ring.L * n * 8)
+    ring.inverse(put(gpu_ctx, ring, [f]), out)
+    assert get(out, ring, 1)[0] == cpuref.poly_inv(f, n, q)
+    ring.close()
+
+
+def test_ring_inverse_large_is_an_inverse(gpu_ctx):
+    # f = t f' + 1 as in key generation, at the largest ring of the circuits
+    rng = np.random.default_rng(3)
+    n, q, t = 16384, (1 << 511) - 187, 1 << 16
+    ring = ya.Ring(gpu_ctx, n, q)
+    fp = rng.integers(-1, 2, size=n)
+    f = [(t * int(x) + (j == 0)) % q for j, x in enumerate(fp)]
+    df = put(gpu_ctx, ring, [f])
+    inv = gpu_ctx.alloc(ring.L * n * 8)
+    ring.inverse(df, inv)
+    one = gpu_ctx.alloc(ring.L * n * 8)
+    ring.ring_mul(df, inv, one, 1)
+    assert get(one, ring, 1)[0] == [1] + [0] * (n - 1)
+    ring.close()
+
+
+def test_ring_inverse_rejects_zero(gpu_ctx):
+    ring = ya.Ring(gpu_ctx, 16, Q127)
+    out = gpu_ctx.alloc(ring.L * 16 * 8)
+    with pytest.raises(ya.YashError):
+        ring.inverse(put(gpu_ctx, ring, [[0] * 16]), out)
+    ring.close()
+
+
+def test_ring_mul_bcast_matches_oracle(gpu_ctx):
+    rng = random.Random(5)
+    n, q, batch = 256, Q255, 3
+    ring = ya.Ring(gpu_ctx, n, q)
+    a = rand_poly(rng, n, q)
+    B = [rand_poly(rng, n, q) for _ in range(batch)]
+    out = gpu_ctx.alloc(batch * ring.L * n * 8)
+    ring.ring_mul_bcast(put(gpu_ctx, ring, [a]), put(gpu_ctx, ring, B), out, batch)
+    got = get(out, ring, batch)
+    for i in range(batch):
+        assert got[i] == cpuref.ring_mul(a, B[i], n, q)
+    ring.close()
+
+
+@pytest.mark.parametrize("n,q,t", [(1024, Q127, 1024), (4096, Q127, 1024), (2048, Q255, 1 << 16)])
+def test_scheme_roundtrip_add_mul(gpu_ctx, n, q, t):
+    Y = Yashe(gpu_ctx, n, q, t, 32, seed=n).generate_keys()
+    rng = np.random.default_rng(n + 1)
+    m1 = rng.integers(0, t, size=(4, n))
+    m2 = rng.integers(0, t, size=(4, n))
+    c1, c2 = Y.encrypt(m1), Y.encrypt(m2)
+    assert np.array_equal(Y.decrypt(c1, 4), m1)
+    s = gpu_ctx.alloc(4 * Y.elt_bytes)
+    Y.add(c1, c2, s, 4)
+    assert np.array_equal(Y.decrypt(s, 4), (m1 + m2) % t)
+    # mul without keyswitch decrypts under f^2 (src/yashe.cpp:167-197)
+    p = gpu_ctx.alloc(4 * Y.elt_bytes)
+    Y.ring.yashe_mul(c1, c2, p, 4)
+    k = gpu_ctx.alloc(4 * Y.elt_bytes)
+    Y.mul_keyswitch(c1, c2, k, 4)
+    want = np.stack([negacyclic_mul_mod(m1[i], m2[i], t) for i in range(4)])
+    assert np.array_equal(Y.decrypt(p, 4, aftermul=True), want)
+    assert np.array_equal(Y.decrypt(k, 4), want)
+    Y.ring.close()
+
+
+def _circuit(Y, X, A, Pm, B, Rm, S, U, V, Z):
+    """128 inputs -> depth-3 tree of 64 + 32 + 16 mul+keyswitch, 256 adds.
+
+    a_i = x_i + x_{i+1 mod 128}        128 adds
+    p_i = a_i a_{i+64}, i < 64          64 mul+ks
+    b_i = p_i + x_i                     64 adds
+    r_i = b_i b_{i+32}, i < 32          32 mul+ks
+    s_i = r_i + r_{i+1 mod 32}          32 adds
+    u_i = s_i s_{i+16}, i < 16          16 mul+ks
+    v_i = u_i + x_i                     16 adds
+    z_i = v_i + v_{i+1 mod 16}          16 adds
+    """
+    e = Y.elt_bytes
+    ring = Y.ring
+    ring.ring_addsub(X.view(0, 127 * e), X.view(e, 127 * e), A.view(0, 127 * e), 127)
+    ring.ring_addsub(X.view(127 * e, e), X.view(0, e), A.view(127 * e, e), 1)
+    ring.mul_keyswitch(Y.evk, A.view(0, 64 * e), A.view(64 * e, 64 * e), Pm, 64)
+    ring.ring_addsub(Pm, X.view(0, 64 * e), B, 64)
+    ring.mul_keyswitch(Y.evk, B.view(0, 32 * e), B.view(32 * e, 32 * e), Rm, 32)
+    ring.ring_addsub(Rm.view(0, 31 * e), Rm.view(e, 31 * e), S.view(0, 31 * e), 31)
+    ring.ring_addsub(Rm.view(31 * e, e), Rm.view(0, e), S.view(31 * e, e), 1)
+    ring.mul_keyswitch(Y.evk, S.view(0, 16 * e), S.view(16 * e, 16 * e), U, 16)
+    ring.ring_addsub(U, X.view(0, 16 * e), V, 16)
+    ring.ring_addsub(V.view(0, 15 * e), V.view(e, 15 * e), Z.view(0, 15 * e), 15)
+    ring.ring_addsub(V.view(15 * e, e), V.view(0, e), Z.view(15 * e, e), 1)
+
+
+def _plain_circuit(x, t):
+    mul = lambda a, b: negacyclic_mul_mod(a, b, t)  # noqa: E731
+    a = (x + np.roll(x, -1, axis=0)) % t
+    p = np.stack([mul(a[i], a[i + 64]) for i in range(64)])
+    b = (p + x[:64]) % t
+    r = np.stack([mul(b[i], b[i + 32]) for i in range(32)])
+    s = (r + np.roll(r, -1, axis=0)) % t
+    u = np.stack([mul(s[i], s[i + 16]) for i in range(16)])
+    v = (u + x[:16]) % t
+    return (v + np.roll(v, -1, axis=0)) % t
+
+
+def test_graph_circuit_depth3(gpu_ctx):
+    """128 ciphertexts, n = 8192, q = 2^255 - 19, t = 2^16, w = 2^32: the
+    depth-3 multiplication tree with 256 additions, captured once and
+    replayed; every replay equals the eager run and decrypts to the plain
+    circuit over Z_t[x]/(x^n + 1)."""
+    n, q, t = 8192, Q255, 1 << 16
+    Y = Yashe(gpu_ctx, n, q, t, 32, seed=41).generate_keys()
+    rng = np.random.default_rng(42)
+    x = rng.integers(0, t, size=(128, n))
+    X = Y.encrypt(x)
+    e = Y.elt_bytes
+    bufs = dict(A=128, Pm=64, B=64, Rm=32, S=32, U=16, V=16, Z=16)
+    B = {k: gpu_ctx.alloc(v * e) for k, v in bufs.items()}
+    _circuit(Y, X, **B)
+    eager = B["Z"].download((16, Y.L, n)).copy()
+    p0 = get(B["Pm"], Y.ring, 1)[0]
+    assert np.array_equal(Y.decrypt(B["Z"], 16), _plain_circuit(x, t))
+    # one level-1 node bit-exact against the oracle
+    a = get(B["A"], Y.ring, 128)
+    gam = get(Y.gamma, Y.ring, Y.lw)
+    assert p0 == cpuref.keyswitch(cpuref.yashe_mul(a[0], a[64], n, q, t), gam, n, q, 32)
+
+    gpu_ctx.graph_begin()
+    _circuit(Y, X, **B)
+    g = gpu_ctx.graph_end()
+    other = gpu_ctx.alloc(e)
+    for rep in range(3):
+        for buf in B.values():
+            buf_bytes = buf.nbytes
+            ya.lib().yh_memset(gpu_ctx.h, buf.ptr, 0xA5, buf_bytes)
+        gpu_ctx.sync()
+        g.launch()
+        gpu_ctx.sync()
+        assert np.array_equal(B["Z"].download((16, Y.L, n)), eager), rep
+        # eager work between replays
+        Y.ring.ring_mul(X.view(rep * e, e), X.view((rep + 1) * e, e), other, 1)
+        gpu_ctx.sync()
+    g.close()
+    Y.ring.close()
+
+
+def test_graph_capture_refuses_allocation(gpu_ctx):
+    # a capture that would have to grow scratch fails instead of allocating
+    ring = ya.Ring(gpu_ctx, 64, Q127)
+    a = put(gpu_ctx, ring, [[1] * 64] * 4)
+    out = gpu_ctx.alloc(4 * ring.L * 64 * 8)
+    gpu_ctx.graph_begin()
+    with pytest.raises(ya.YashError, match="capture"):
+        ring.ring_mul(a, a, out, 4)
+    with pytest.raises(ya.YashError, match="capture"):
+        gpu_ctx.sync()
+    gpu_ctx.graph_end().close()  # nothing was recorded
+    # the context works eagerly afterwards
+    ring.ring_mul(a, a, out, 4)
+    want = cpuref.ring_mul([1] * 64, [1] * 64, 64, Q127)
+    assert get(out, ring, 4) == [want] * 4
+    ring.close()
