@@ -12,7 +12,9 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libyashe_hip.so")
+# CUYASHE_AMD_LIB points at another build of the same library (e.g. one
+# compiled with different tuning macros).
+LIB_PATH = os.environ.get("CUYASHE_AMD_LIB") or os.path.join(HERE, "lib", "libyashe_hip.so")
 
 YH_OK = 0
 YH_FORWARD = 1
@@ -21,7 +23,7 @@ YH_ADD = 0
 YH_SUB = 1
 
 OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD = range(5)
-KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT = range(3)
+KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH = range(4)
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
@@ -37,6 +39,19 @@ class RingInfo(ctypes.Structure):
         ("R_ks", ctypes.c_int), ("w_bits", ctypes.c_int), ("lw", ctypes.c_int), ("work_limbs", ctypes.c_int),
         ("m_bits", ctypes.c_int), ("m_ks_bits", ctypes.c_int), ("t", ctypes.c_uint64), ("prime_max", ctypes.c_uint64),
     ]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("ring", ctypes.c_void_p), ("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("out", ctypes.c_void_p)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("wg_threads", ctypes.c_int), ("wgs_per_cu", ctypes.c_int), ("queues", ctypes.c_int),
+                ("waves", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"wg_threads": self.wg_threads, "wgs_per_cu": self.wgs_per_cu, "queues": self.queues,
+                "waves": self.waves}
 
 
 _lib = None
@@ -91,6 +106,13 @@ _SIGS = {
     "yh_graph_end": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "yh_graph_launch": (ctypes.c_int, [_vp]),
     "yh_graph_destroy": (ctypes.c_int, [_vp]),
+    "yh_batch_default_config": (ctypes.c_int, [ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.POINTER(BatchConfig)]),
+    "yh_batch_create": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.POINTER(BatchConfig),
+                                       ctypes.POINTER(_vp)]),
+    "yh_batch_run": (ctypes.c_int, [_vp]),
+    "yh_batch_destroy": (ctypes.c_int, [_vp]),
+    "yh_batch_autotune": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.POINTER(BatchConfig)]),
 }
 
 
@@ -381,6 +403,66 @@ class Ring:
         if self.h:
             lib().yh_ring_destroy(self.h)
             self.h = None
+
+
+def _batch_items(items) -> "ctypes.Array":
+    arr = (BatchItem * len(items))()
+    for k, (ring, a, b, out) in enumerate(items):
+        arr[k] = BatchItem(ring.h, a.ptr, b.ptr, out.ptr)
+    return arr
+
+
+def _config(cfg) -> Optional[BatchConfig]:
+    if cfg is None:
+        return None
+    if isinstance(cfg, BatchConfig):
+        return cfg
+    return BatchConfig(int(cfg["wg_threads"]), int(cfg["wgs_per_cu"]), int(cfg["queues"]), int(cfg.get("waves", 3)))
+
+
+class Batch:
+    """Uneven products c_k = a_k b_k mod (x^n_k + 1, q_k) in one persistent
+    launch. items: sequence of (Ring, a, b, out) device buffers."""
+
+    def __init__(self, ctx: Context, items, config=None):
+        self.ctx = ctx
+        self._keep = list(items)
+        arr = _batch_items(items)
+        c = _config(config)
+        h = _vp()
+        check(lib().yh_batch_create(ctx.h, arr, len(items), ctypes.byref(c) if c is not None else None,
+                                    ctypes.byref(h)), "yh_batch_create")
+        self.h = h.value
+
+    def run(self):
+        check(lib().yh_batch_run(self.h), "yh_batch_run")
+
+    def close(self):
+        if self.h:
+            lib().yh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_default_config(items) -> dict:
+    arr = _batch_items(items)
+    c = BatchConfig()
+    check(lib().yh_batch_default_config(arr, len(items), ctypes.byref(c)), "yh_batch_default_config")
+    return c.as_dict()
+
+
+def batch_autotune(ctx: Context, items, cache_path: Optional[str] = None, reps: int = 3) -> dict:
+    """Fastest launch shape for this batch composition (cached in cache_path)."""
+    arr = _batch_items(items)
+    c = BatchConfig()
+    path = cache_path.encode() if cache_path else None
+    check(lib().yh_batch_autotune(ctx.h, arr, len(items), path, reps, ctypes.byref(c)), "yh_batch_autotune")
+    return c.as_dict()
 
 
 class Evk:

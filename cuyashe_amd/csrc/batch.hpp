@@ -1,0 +1,207 @@
+// Persistent batched ring products over uneven rings in one launch.
+//
+// A batch is a list of independent products c = a*b mod (x^n + 1, q) whose
+// rings differ (n from 16 to 8192, q of any supported width, so the prime
+// count R differs too). The host splits every product into R tasks (one per
+// CRT prime) and packs tasks of equal n into bundles of W / (n/16) tasks, so
+// one workgroup of W threads runs a bundle as W / (n/16) side-by-side
+// transforms (CRT -> forward NTT of both operands -> pointwise -> inverse NTT
+// -> residue y_i). Bundles sit in per-XCD queues (all tasks of one product in
+// the same queue, so its operands stay in one L2); a workgroup drains its own
+// queue first and then takes bundles from the others.
+//
+// The workgroup whose task completes the last prime of a product (a
+// per-product counter, agent-scope release/acquire) runs that product's
+// ICRT + mod-q epilogue with all its threads, in the same launch. Nothing in the kernel waits on another workgroup: every
+// loop either consumes a bundle index or ends, so the grid always drains.
+//
+// Replaces the reference's per-product call chain callCRT -> callNTT ->
+// polynomialNTTMul -> callNTT -> callICRT (src/cuda_bn.cu:970-1045,
+// src/cuda_functions.cu:447-474), which it runs one product at a time.
+#pragma once
+#include "kernels.hpp"
+
+namespace yhk {
+
+struct BatchInst {
+  const uint64_t* A;  // [LQ][n], coefficients in [0, q) (any value below 2^(64 LQ) is reduced)
+  const uint64_t* B;
+  uint64_t* out;      // [LQ][n]
+  uint32_t* Y;        // [R][n] residue scratch
+  int ring;           // index into BatchArgs::rings
+  int pad;
+};
+
+struct BatchTask {
+  int inst;
+  int prime;
+};
+
+// Every task of a bundle belongs to the same ring, so the ring's tables are
+// workgroup-uniform (scalar loads); only the instance and prime vary.
+struct BatchBundle {
+  int logn;
+  int count;  // tasks in this bundle (<= W / (n/16))
+  int task0;  // first task index
+  int ring;   // index into BatchArgs::rings
+};
+
+struct BatchArgs {
+  const RingDev* rings;
+  const BatchInst* inst;
+  const BatchTask* tasks;
+  const BatchBundle* bundles;
+  const int* qstart;  // [nqueues + 1]: queue k holds bundles [qstart[k], qstart[k+1])
+  int* head;          // [nqueues] next unclaimed bundle of each queue
+  int* done;          // [ninst] primes finished per product
+  int nqueues;
+};
+
+constexpr int BATCH_MAX_THREADS = 512;
+constexpr int BATCH_MIN_LOGN = 4;
+constexpr int BATCH_MAX_LOGN = 13;
+
+// Dynamic LDS of a W-thread batch workgroup: W/(n/16) transforms of n
+// elements plus padding = 17 W words whatever n is.
+__host__ __device__ constexpr size_t batch_lds_bytes(int W) { return (size_t)W * 17 * sizeof(uint64_t); }
+
+__global__ void k_batch_reset(int* head, int nq, int* done, int ninst) {
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) head[i] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x) done[i] = 0;
+}
+
+// s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their
+// maxima): this wave's global loads and stores have completed.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// One bundle of tasks of length-2^logn rings (runtime length: one copy of the
+// transform code serves every n). Sets s_last[sub] for the sub-groups whose
+// task finished the last prime of its product.
+// Transforms of n <= 1024 lie inside one wavefront each, so their LDS
+// exchanges synchronise the wavefront only.
+__device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBundle& bd, const RingDev& rd,
+                                             uint64_t* lds, int* s_last) {
+  const int logn = bd.logn;
+  const int N = 1 << logn;
+  const int T = N >> 4;
+  const int sub = threadIdx.x >> (logn - 4);
+  const int g = threadIdx.x & (T - 1);
+  // sub-groups without a task repeat task 0's work (every thread must take
+  // part in the transform's barriers) and store nothing
+  const bool active = sub < bd.count;
+  const BatchTask tk = ba.tasks[bd.task0 + (active ? sub : 0)];
+  const bool wave = logn <= 10;
+  const BatchInst in = ba.inst[tk.inst];
+  const int i = tk.prime;
+  const uint64_t p = rd.primes[i];
+  const uint64_t b = rd.barrett[i];
+  const int LQ = rd.LQ;
+  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  const NttTab tab = rd.tab;
+  uint64_t* my = lds + (size_t)sub * (N + T);
+  uint64_t xa[E], xb[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) {
+    const int j = g + c * T;
+    xa[c] = twist_centered(crt_residue(in.A + j, LQ, N, pw, p, b), p, tab.twist[j]);
+  }
+  ntt16_fwd_rt(xa, my, tab.fwd, g, logn, wave);
+#pragma unroll
+  for (int c = 0; c < E; ++c) {
+    const int j = g + c * T;
+    xb[c] = twist_centered(crt_residue(in.B + j, LQ, N, pw, p, b), p, tab.twist[j]);
+  }
+  xsync(wave);
+  ntt16_fwd_rt(xb, my, tab.fwd, g, logn, wave);
+#pragma unroll
+  for (int c = 0; c < E; ++c) xa[c] = gl::mul(xa[c], xb[c]);
+  xsync(wave);
+  ntt16_inv_rt(xa, my, tab.inv, g, logn, wave);
+  if (active) {
+    const uint64_t invM = rd.invMi[0][i];
+    uint32_t* y = in.Y + (size_t)i * N;
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      const int j = g + c * T;
+      y[j] = finish_residue(gl::mul(xa[c], tab.untwist[j]), 1, p, b, invM);
+    }
+  }
+  // Publish the residues: every wave waits until its stores have reached L2
+  // (vmcnt = 0), then after the barrier one thread writes the L2 back at
+  // agent scope (release) and moves the counters.
+  wait_vmem();
+  __syncthreads();
+  if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (active && g == 0) {
+    const int prev = atomicAdd(&ba.done[tk.inst], 1);
+    s_last[sub] = prev == rd.set_R[0] - 1;
+  }
+  __syncthreads();
+}
+
+// ICRT + mod q of one product by the whole workgroup.
+template <int LQ>
+__device__ __forceinline__ void batch_icrt(const RingDev& rd, const BatchInst& in, int n) {
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    icrt_coef<LQ>(rd, in.Y + j, n, 0, EPI_MODQ, 0, in.out + j, LQ, n);
+}
+
+// MAXLQ bounds the limb count of q in the batch: with the in-kernel ICRT for
+// LQ <= 4 the kernel needs ~150 VGPRs, so WAVES = 3 waves per SIMD fit
+// without spills and WAVES = 4 (128 VGPRs) spills a little; LQ up to 8 needs
+// the full 256 (WAVES = 2). WAVES is a tuning knob of the launch shape.
+template <int MAXLQ, int WAVES>
+__global__ void __launch_bounds__(BATCH_MAX_THREADS, WAVES) k_batch_prod(BatchArgs ba) {
+  extern __shared__ uint64_t lds[];
+  __shared__ int s_b;
+  __shared__ int s_last[BATCH_MAX_THREADS];
+  const int q0 = blockIdx.x % ba.nqueues;  // consecutive workgroups go to consecutive XCDs
+  int qi = 0;
+  while (true) {
+    if (threadIdx.x == 0) {
+      int b = -1;
+      while (qi < ba.nqueues) {
+        const int q = (q0 + qi) % ba.nqueues;
+        const int k = atomicAdd(&ba.head[q], 1);
+        if (k < ba.qstart[q + 1] - ba.qstart[q]) {
+          b = ba.qstart[q] + k;
+          break;
+        }
+        ++qi;  // queue q is drained: move on to the next one
+      }
+      s_b = b;
+    }
+    __syncthreads();
+    const int bi = __builtin_amdgcn_readfirstlane(s_b);
+    if (bi < 0) break;
+    const BatchBundle bd = ba.bundles[bi];
+    const RingDev& rd = ba.rings[bd.ring];
+    batch_bundle(ba, bd, rd, lds, s_last);
+    // products completed by this bundle: ICRT by the whole workgroup (the
+    // bundle's ring is workgroup-uniform, so is its limb count)
+    const int n = 1 << bd.logn;
+    bool acquired = false;
+    for (int sub = 0; sub < bd.count; ++sub) {
+      if (!s_last[sub]) continue;
+      if (!acquired) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see every prime's residues
+        acquired = true;
+      }
+      const BatchInst in = ba.inst[ba.tasks[bd.task0 + sub].inst];
+      switch (rd.LQ) {
+        case 1: batch_icrt<1>(rd, in, n); break;
+        case 2: batch_icrt<2>(rd, in, n); break;
+        case 3: batch_icrt<3>(rd, in, n); break;
+        case 4: batch_icrt<4>(rd, in, n); break;
+        case 5: if (MAXLQ >= 5) batch_icrt<MAXLQ >= 5 ? 5 : 1>(rd, in, n); break;
+        case 6: if (MAXLQ >= 6) batch_icrt<MAXLQ >= 6 ? 6 : 1>(rd, in, n); break;
+        case 7: if (MAXLQ >= 7) batch_icrt<MAXLQ >= 7 ? 7 : 1>(rd, in, n); break;
+        default: if (MAXLQ >= 8) batch_icrt<MAXLQ >= 8 ? 8 : 1>(rd, in, n); break;
+      }
+    }
+    __syncthreads();  // s_b, s_last and the LDS are reused by the next bundle
+  }
+}
+
+}  // namespace yhk

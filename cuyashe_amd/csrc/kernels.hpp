@@ -559,6 +559,7 @@ __device__ void divmod_q(const uint64_t* a, const RingDev& rd, uint64_t* quo, ui
   uint64_t prod[LA + LMU];
 #pragma unroll
   for (int l = 0; l < LA + LMU; ++l) prod[l] = 0;
+#pragma unroll
   for (int i = 0; i < LA; ++i) {
     uint64_t carry = 0;
 #pragma unroll
@@ -580,9 +581,12 @@ __device__ void divmod_q(const uint64_t* a, const RingDev& rd, uint64_t* quo, ui
   uint64_t qq[LQ + 1];
 #pragma unroll
   for (int l = 0; l <= LQ; ++l) qq[l] = 0;
+#pragma unroll
   for (int k = 0; k < LMU; ++k) {
     uint64_t carry = 0;
-    for (int l = 0; k + l <= LQ; ++l) {
+#pragma unroll
+    for (int l = 0; l <= LQ; ++l) {
+      if (k + l > LQ) break;
       uint64_t ql = l < LQ ? rd.q[l] : 0;
       uint64_t lo = quo[k] * ql;
       uint64_t hi = gl::umulhi(quo[k], ql);
@@ -645,21 +649,20 @@ struct IcrtArgs {
   int positive;          // 1: result in [0, M) (plain product); 0: centred
 };
 
+// ICRT of one coefficient plus its epilogue. y points at the residue of
+// prime 0 (rows ystride apart), o at limb 0 of the output coefficient.
 template <int LQ>
-__global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
+__device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, size_t ystride, int set, int mode,
+                                          int positive, uint64_t* o, int out_limbs, size_t out_stride) {
   constexpr int LW = 2 * LQ + 1;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int pair = blockIdx.y;
-  if (j >= ia.ncoef) return;
-  const int RS = rd.set_R[ia.set];
-  const uint32_t* y = ia.Y + (size_t)pair * RS * ia.N + j;
-  const uint64_t* Mi = rd.Mi[ia.set];
+  const int RS = rd.set_R[set];
+  const uint64_t* Mi = rd.Mi[set];
   uint64_t acc[LW];
 #pragma unroll
   for (int l = 0; l < LW; ++l) acc[l] = 0;
   uint64_t flo = 0, fhi = 0;
   for (int i = 0; i < RS; ++i) {
-    uint64_t yi = y[(size_t)i * ia.N];
+    uint64_t yi = y[(size_t)i * ystride];
     // fixed-point y_i/p_i with 64 fractional bits (y_i < p_i keeps it < 2^64)
     uint64_t f = yi * rd.barrett[i];
     flo += f;
@@ -667,7 +670,7 @@ __global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
     mac32<LW>(acc, Mi + (size_t)i * LW, yi);
   }
   uint64_t k;
-  if (ia.positive) {
+  if (positive) {
     // bias covers the truncation error so that floor() never undershoots;
     // a one-too-large k leaves a negative value that the check below fixes.
     uint64_t bias = (uint64_t)RS << 41;
@@ -680,18 +683,20 @@ __global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
   uint64_t kM[LW];
 #pragma unroll
   for (int l = 0; l < LW; ++l) kM[l] = 0;
-  mac32<LW>(kM, rd.M[ia.set], k);
+  mac32<LW>(kM, rd.M[set], k);
   sub_n<LW>(acc, kM);
   bool neg = acc[LW - 1] >> 63;
-  if (ia.positive) {
-    if (neg) add_n<LW>(acc, rd.M[ia.set]);
+  if (positive) {
+    if (neg) add_n<LW>(acc, rd.M[set]);
     neg = false;
   } else if (neg) {
     neg_n<LW>(acc);
   }
-  uint64_t* o = ia.out + pair * ia.out_pstride + j;
-  if (ia.mode == EPI_RAW) {
-    for (int l = 0; l < ia.out_limbs; ++l) o[(size_t)l * ia.out_stride] = l < LW ? acc[l] : 0;
+  if (mode == EPI_RAW) {
+#pragma unroll
+    for (int l = 0; l < LW; ++l)
+      if (l < out_limbs) o[(size_t)l * out_stride] = acc[l];
+    for (int l = LW; l < out_limbs; ++l) o[(size_t)l * out_stride] = 0;
     return;
   }
   constexpr int LA = LW + 1;
@@ -700,7 +705,7 @@ __global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
 #pragma unroll
   for (int l = 0; l < LW; ++l) a[l] = acc[l];
   a[LW] = 0;
-  if (ia.mode == EPI_MODQ || ia.mode == EPI_DECRYPT) {
+  if (mode == EPI_MODQ || mode == EPI_DECRYPT) {
     divmod_q<LQ>(a, rd, quo, rem);
     bool nz = false;
 #pragma unroll
@@ -713,9 +718,9 @@ __global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
 #pragma unroll
       for (int l = 0; l < LQ; ++l) rem[l] = qv[l];
     }
-    if (ia.mode == EPI_MODQ) {
+    if (mode == EPI_MODQ) {
 #pragma unroll
-      for (int l = 0; l < LQ; ++l) o[(size_t)l * ia.out_stride] = rem[l];
+      for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = rem[l];
       return;
     }
     // decrypt: m = round(t * [g]_q / q) mod t, [g]_q in [0, q)
@@ -804,7 +809,17 @@ __global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
     for (int l = 0; l < LQ; ++l) r2[l] = qv[l];
   }
 #pragma unroll
-  for (int l = 0; l < LQ; ++l) o[(size_t)l * ia.out_stride] = r2[l];
+  for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = r2[l];
+}
+
+template <int LQ>
+__global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pair = blockIdx.y;
+  if (j >= ia.ncoef) return;
+  const int RS = rd.set_R[ia.set];
+  icrt_coef<LQ>(rd, ia.Y + (size_t)pair * RS * ia.N + j, ia.N, ia.set, ia.mode, ia.positive,
+                ia.out + pair * ia.out_pstride + j, ia.out_limbs, ia.out_stride);
 }
 
 // ---------------------------------------------------------------- primitives

@@ -136,11 +136,14 @@ __device__ __forceinline__ void ld16(const uint64_t* lds, uint64_t* x, int base,
 // Forward. Entry: x holds elements g + c*N/16 (natural positions). Exit:
 // x holds the transform at positions 16g + c (permuted order). Leaves the LDS
 // in use: callers synchronise before the next transform writes it.
+// g is the thread's index within the transform (threadIdx.x unless several
+// transforms share a workgroup); __syncthreads spans the whole workgroup, so
+// every thread of it calls the same transform.
 template <int LOGN>
-__device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ tw) {
+__device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ tw,
+                                          int g = threadIdx.x) {
   constexpr int NFULL = LOGN / 4;
   constexpr int SHORT = LOGN % 4;
-  const int g = threadIdx.x;
 #pragma unroll
   for (int p = 0; p < NFULL; ++p) {
     const int b0 = LOGN - 4 - 4 * p;
@@ -173,10 +176,10 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
 // ntt16_fwd. Exit: x holds elements g + c*N/16. Callers synchronise before
 // calling if another transform used the LDS just before.
 template <int LOGN>
-__device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ twi) {
+__device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ twi,
+                                          int g = threadIdx.x) {
   constexpr int NFULL = LOGN / 4;
   constexpr int SHORT = LOGN % 4;
-  const int g = threadIdx.x;
   if (SHORT) {
     constexpr int R = 1 << SHORT;
 #pragma unroll
@@ -202,6 +205,117 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
     if (p > 0) {
       st16(lds, x, base, b0);
       __syncthreads();
+    }
+  }
+}
+
+// Runtime-length variants (4 <= logn <= 14) for kernels that run transforms
+// of several lengths: the 16-point butterflies are the same compile-time
+// code, only pass count, strides and table offsets are runtime values, so one
+// copy of the code serves every length.
+__device__ __forceinline__ void short_dif(uint64_t* x, int r) {
+  if (r == 8) {
+#pragma unroll
+    for (int blk = 0; blk < 16; blk += 8) dif_r<8>(x + blk);
+  } else if (r == 4) {
+#pragma unroll
+    for (int blk = 0; blk < 16; blk += 4) dif_r<4>(x + blk);
+  } else if (r == 2) {
+#pragma unroll
+    for (int blk = 0; blk < 16; blk += 2) dif_r<2>(x + blk);
+  }
+}
+__device__ __forceinline__ void short_dit(uint64_t* x, int r) {
+  if (r == 8) {
+#pragma unroll
+    for (int blk = 0; blk < 16; blk += 8) dit_r<8>(x + blk);
+  } else if (r == 4) {
+#pragma unroll
+    for (int blk = 0; blk < 16; blk += 4) dit_r<4>(x + blk);
+  } else if (r == 2) {
+#pragma unroll
+    for (int blk = 0; blk < 16; blk += 2) dit_r<2>(x + blk);
+  }
+}
+
+// Synchronisation of one transform's threads around its LDS exchanges: the
+// whole workgroup, or (wave = true: transforms of at most 64 threads, each
+// inside one wavefront) only the wavefront, whose LDS operations execute in
+// order. `wave` must be the same for the whole workgroup.
+__device__ __forceinline__ void xsync(bool wave) {
+  if (wave) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void ntt16_fwd_rt(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ tw, int g,
+                                             int logn, bool wave = false) {
+  const int nfull = logn >> 2;
+  const int sh = logn & 3;
+  int off = 0;
+  for (int p = 0; p < nfull; ++p) {
+    const int b0 = logn - 4 - 4 * p;
+    const int base = pbase(g, b0);
+    if (p > 0) {
+      ld16(lds, x, base, b0);
+      xsync(wave);
+    }
+    dif_r<16>(x);
+    if (b0 > 0) {
+      const int s = 1 << b0;
+      const uint64_t* t = tw + off + (g & (s - 1));
+#pragma unroll
+      for (int c = 1; c < 16; ++c) x[c] = gl::mul(x[c], t[(brev4(c) - 1) * s]);
+      off += 15 * s;
+    }
+    if (p < nfull - 1 || sh) {
+      st16(lds, x, base, b0);
+      xsync(wave);
+    }
+  }
+  if (sh) {
+    ld16(lds, x, 16 * g, 0);
+    short_dif(x, 1 << sh);
+  }
+}
+
+__device__ __forceinline__ void ntt16_inv_rt(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ twi, int g,
+                                             int logn, bool wave = false) {
+  const int nfull = logn >> 2;
+  const int sh = logn & 3;
+  if (sh) {
+    short_dit(x, 1 << sh);
+    st16(lds, x, 16 * g, 0);
+    xsync(wave);
+  }
+  // table offset of pass p = sum of 15 * 2^b0 over the passes before it
+  int off = 0;
+  for (int p = 0; p < nfull; ++p) {
+    const int b0 = logn - 4 - 4 * p;
+    if (b0 > 0) off += 15 << b0;
+  }
+  for (int p = nfull - 1; p >= 0; --p) {
+    const int b0 = logn - 4 - 4 * p;
+    const int base = pbase(g, b0);
+    if (p < nfull - 1 || sh) {
+      ld16(lds, x, base, b0);
+      xsync(wave);
+    }
+    if (b0 > 0) {
+      const int s = 1 << b0;
+      off -= 15 * s;
+      const uint64_t* t = twi + off + (g & (s - 1));
+#pragma unroll
+      for (int c = 1; c < 16; ++c) x[c] = gl::mul(x[c], t[(brev4(c) - 1) * s]);
+    }
+    dit_r<16>(x);
+    if (p > 0) {
+      st16(lds, x, base, b0);
+      xsync(wave);
     }
   }
 }

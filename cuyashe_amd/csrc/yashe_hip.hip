@@ -2,8 +2,11 @@
 // Declarations and reference citations: include/yashe_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -12,6 +15,7 @@
 
 #include "../../include/yashe_hip.h"
 #include "bigint.hpp"
+#include "batch.hpp"
 #include "kernels.hpp"
 #include "ring_params.hpp"
 
@@ -64,8 +68,10 @@ struct yh_ctx {
   bool timing = false;
   std::vector<TimedLaunch> pending;
   std::vector<hipEvent_t> event_pool;
-  double total_ms[3] = {0, 0, 0};
-  long launches[3] = {0, 0, 0};
+  double total_ms[4] = {0, 0, 0, 0};
+  long launches[4] = {0, 0, 0, 0};
+  int cus = 0;           // compute units of the device
+  std::string arch;      // gcnArchName
   bool capturing = false;
 };
 
@@ -354,8 +360,20 @@ int yh_ctx_create(int device, yh_ctx** out) {
 #define X(L) set_lds_attrs<L>();
   YH_LOGN_CASES(X)
 #undef X
+  for (const void* f : {(const void*)yhk::k_batch_prod<4, 3>, (const void*)yhk::k_batch_prod<4, 4>,
+                        (const void*)yhk::k_batch_prod<8, 2>})
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)yhk::batch_lds_bytes(yhk::BATCH_MAX_THREADS));
   auto* c = new yh_ctx();
   c->device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+      c->cus = prop.multiProcessorCount;
+      c->arch = prop.gcnArchName;
+    }
+    if (c->cus <= 0) c->cus = 256;
+  }
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return fail(YH_ERR_HIP, hipGetErrorString(e)); }
   *out = c;
@@ -406,12 +424,12 @@ int yh_ctx_reset_timing(yh_ctx* ctx) {
   if (!ctx) return fail(YH_ERR_ARG, "null ctx");
   if (int s = set_dev(ctx)) return s;
   if (int s = drain_timing(ctx)) return s;
-  for (int k = 0; k < 3; ++k) { ctx->total_ms[k] = 0; ctx->launches[k] = 0; }
+  for (int k = 0; k < 4; ++k) { ctx->total_ms[k] = 0; ctx->launches[k] = 0; }
   return YH_OK;
 }
 
 int yh_ctx_kernel_time(yh_ctx* ctx, int kernel_id, double* total_ms, long* launches) {
-  if (!ctx || kernel_id < 0 || kernel_id > 2) return fail(YH_ERR_ARG, "bad argument");
+  if (!ctx || kernel_id < 0 || kernel_id > 3) return fail(YH_ERR_ARG, "bad argument");
   if (int s = set_dev(ctx)) return s;
   if (int s = drain_timing(ctx)) return s;
   if (total_ms) *total_ms = ctx->total_ms[kernel_id];
@@ -958,6 +976,333 @@ int yh_graph_destroy(yh_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
+  return YH_OK;
+}
+
+
+struct yh_batch {
+  yh_ctx* ctx = nullptr;
+  yh_batch_config cfg{};
+  int ninst = 0, nbundles = 0, grid = 0, W = 0, maxlq = 0;
+  void* d_mem = nullptr;
+  yhk::BatchArgs args{};
+  ~yh_batch() {
+    if (d_mem) (void)hipFree(d_mem);
+  }
+};
+
+extern "C++" {
+namespace {
+
+int validate_items(const yh_batch_item* items, int count, int* max_T) {
+  if (!items || count <= 0) return fail(YH_ERR_ARG, "empty batch");
+  int mt = 1;
+  for (int k = 0; k < count; ++k) {
+    const yh_batch_item& it = items[k];
+    if (!it.ring || !it.a || !it.b || !it.out) return fail(YH_ERR_ARG, "null pointer in batch item");
+    const int ln = it.ring->rd.logn;
+    if (ln < yhk::BATCH_MIN_LOGN || ln > yhk::BATCH_MAX_LOGN)
+      return fail(YH_ERR_ARG, "batched products take 16 <= n <= 8192");
+    if (it.ring->ctx != items[0].ring->ctx) return fail(YH_ERR_ARG, "batch items from different contexts");
+    mt = std::max(mt, (1 << ln) / 16);
+  }
+  *max_T = mt;
+  return YH_OK;
+}
+
+int check_config(const yh_batch_config& c, int max_T) {
+  const int W = c.wg_threads;
+  if (W < 64 || W > yhk::BATCH_MAX_THREADS || (W & (W - 1)) || W < max_T)
+    return fail(YH_ERR_ARG, "wg_threads must be a power of two in [max(64, n/16), 512]");
+  if (c.wgs_per_cu < 1 || c.wgs_per_cu > 16) return fail(YH_ERR_ARG, "wgs_per_cu out of range");
+  if (c.queues < 1 || c.queues > 64) return fail(YH_ERR_ARG, "queues out of range");
+  if (c.waves < 2 || c.waves > 4) return fail(YH_ERR_ARG, "waves must be 2, 3 or 4");
+  return YH_OK;
+}
+
+// Key of a batch composition: device plus the histogram of (log2 n, limbs).
+std::string batch_signature(const yh_ctx* ctx, const yh_batch_item* items, int count) {
+  std::map<std::pair<int, int>, int> h;
+  for (int k = 0; k < count; ++k) h[{items[k].ring->rd.logn, items[k].ring->rd.LQ}]++;
+  std::ostringstream o;
+  o << ctx->arch << ":cu" << ctx->cus << "|";
+  bool first = true;
+  for (auto& kv : h) {
+    o << (first ? "" : ",") << "n" << (1 << kv.first.first) << "L" << kv.first.second << "x" << kv.second;
+    first = false;
+  }
+  return o.str();
+}
+
+bool cache_lookup(const char* path, const std::string& key, yh_batch_config* out) {
+  if (!path) return false;
+  std::ifstream f(path);
+  if (!f) return false;
+  std::string line, want = "\"" + key + "\":";
+  while (std::getline(f, line)) {
+    size_t pos = line.find(want);
+    if (pos == std::string::npos) continue;
+    yh_batch_config c{};
+    double ms = 0;
+    if (sscanf(line.c_str() + pos + want.size(),
+               " {\"wg_threads\": %d, \"wgs_per_cu\": %d, \"queues\": %d, \"waves\": %d, \"ms\": %lf}",
+               &c.wg_threads, &c.wgs_per_cu, &c.queues, &c.waves, &ms) == 5) {
+      *out = c;
+      return true;
+    }
+  }
+  return false;
+}
+
+// The cache is a JSON object with one entry per line; new entries are added
+// before the closing brace (the file is rewritten whole).
+void cache_store(const char* path, const std::string& key, const yh_batch_config& c, double ms) {
+  if (!path) return;
+  std::vector<std::string> entries;
+  {
+    std::ifstream f(path);
+    std::string line;
+    while (f && std::getline(f, line)) {
+      if (line.size() > 2 && line[0] == ' ' && line.find("\"" + key + "\":") == std::string::npos) {
+        while (!line.empty() && (line.back() == ',' || line.back() == ' ')) line.pop_back();
+        entries.push_back(line);
+      }
+    }
+  }
+  char buf[256];
+  snprintf(buf, sizeof(buf), "{\"wg_threads\": %d, \"wgs_per_cu\": %d, \"queues\": %d, \"waves\": %d, \"ms\": %.4f}",
+           c.wg_threads, c.wgs_per_cu, c.queues, c.waves, ms);
+  entries.push_back("  \"" + key + "\": " + buf);
+  std::string tmp = std::string(path) + ".tmp";
+  {
+    std::ofstream f(tmp);
+    f << "{\n";
+    for (size_t i = 0; i < entries.size(); ++i) f << entries[i] << (i + 1 < entries.size() ? ",\n" : "\n");
+    f << "}\n";
+  }
+  std::rename(tmp.c_str(), path);
+}
+
+}  // namespace
+}  // extern "C++"
+
+int yh_batch_default_config(const yh_batch_item* items, int count, yh_batch_config* cfg) {
+  if (!cfg) return fail(YH_ERR_ARG, "null cfg");
+  int mt = 0;
+  if (int s = validate_items(items, count, &mt)) return s;
+  cfg->wg_threads = std::max(256, mt);
+  cfg->wgs_per_cu = 2;
+  cfg->queues = yhk::XCDS;
+  int maxlq = 0;
+  for (int k = 0; k < count; ++k) maxlq = std::max(maxlq, items[k].ring->rd.LQ);
+  cfg->waves = maxlq > 4 ? 2 : 4;
+  return YH_OK;
+}
+
+int yh_batch_create(yh_ctx* ctx, const yh_batch_item* items, int count, const yh_batch_config* cfg_in, yh_batch** out) {
+  if (!ctx || !out) return fail(YH_ERR_ARG, "null argument");
+  if (ctx->capturing) return fail(YH_ERR_ARG, "yh_batch_create allocates; plan before capturing");
+  int mt = 0;
+  if (int s = validate_items(items, count, &mt)) return s;
+  if (items[0].ring->ctx != ctx) return fail(YH_ERR_ARG, "batch items from another context");
+  yh_batch_config cfg;
+  if (cfg_in) cfg = *cfg_in;
+  else yh_batch_default_config(items, count, &cfg);
+  if (int s = check_config(cfg, mt)) return s;
+  if (int s = set_dev(ctx)) return s;
+  YH_TRY_BEGIN
+  const int W = cfg.wg_threads;
+  const int nq = cfg.queues;
+  // distinct rings
+  std::vector<yh_ring*> rings;
+  std::map<yh_ring*, int> ridx;
+  for (int k = 0; k < count; ++k)
+    if (!ridx.count(items[k].ring)) { ridx[items[k].ring] = (int)rings.size(); rings.push_back(items[k].ring); }
+  // products to queues: largest first, each to the least loaded queue
+  std::vector<int> order(count);
+  for (int k = 0; k < count; ++k) order[k] = k;
+  auto cost = [&](int k) {
+    const yh_ring* r = items[k].ring;
+    return (double)r->rd.R * (double)r->rd.n * (r->rd.logn + 2);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+  std::vector<double> load(nq, 0.0);
+  std::vector<std::vector<int>> qinst(nq);
+  for (int k : order) {
+    int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[best] += cost(k);
+    qinst[best].push_back(k);
+  }
+  // tasks and bundles, queue by queue; inside a queue larger rings first and
+  // the primes of one product adjacent
+  std::vector<yhk::BatchTask> tasks;
+  std::vector<yhk::BatchBundle> bundles;
+  std::vector<int> qstart(nq + 1, 0);
+  // rings from the largest n down; a bundle holds tasks of one ring
+  std::vector<int> ring_order(rings.size());
+  for (size_t i = 0; i < rings.size(); ++i) ring_order[i] = (int)i;
+  std::stable_sort(ring_order.begin(), ring_order.end(), [&](int x, int y) {
+    return rings[x]->rd.logn != rings[y]->rd.logn ? rings[x]->rd.logn > rings[y]->rd.logn
+                                                  : rings[x]->rd.R > rings[y]->rd.R;
+  });
+  for (int q = 0; q < nq; ++q) {
+    qstart[q] = (int)bundles.size();
+    for (int ri : ring_order) {
+      const int ln = rings[ri]->rd.logn;
+      const int cap = W / ((1 << ln) / 16);
+      int open = -1;
+      for (int k : qinst[q]) {
+        if (items[k].ring != rings[ri]) continue;
+        for (int i = 0; i < rings[ri]->rd.R; ++i) {
+          if (open < 0 || bundles[open].count == cap) {
+            bundles.push_back({ln, 0, (int)tasks.size(), ri});
+            open = (int)bundles.size() - 1;
+          }
+          tasks.push_back({k, i});
+          bundles[open].count++;
+        }
+      }
+    }
+  }
+  qstart[nq] = (int)bundles.size();
+  // one device allocation: rings | inst | tasks | bundles | qstart | head | done | Y
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  size_t off = 0;
+  const size_t o_rings = off; off = align(off + rings.size() * sizeof(yhk::RingDev));
+  const size_t o_inst = off; off = align(off + count * sizeof(yhk::BatchInst));
+  const size_t o_tasks = off; off = align(off + tasks.size() * sizeof(yhk::BatchTask));
+  const size_t o_bund = off; off = align(off + bundles.size() * sizeof(yhk::BatchBundle));
+  const size_t o_qs = off; off = align(off + qstart.size() * sizeof(int));
+  const size_t o_head = off; off = align(off + nq * sizeof(int));
+  const size_t o_done = off; off = align(off + count * sizeof(int));
+  const size_t host_bytes = o_head;
+  std::vector<size_t> o_y(count);
+  for (int k = 0; k < count; ++k) {
+    o_y[k] = off;
+    off = align(off + (size_t)items[k].ring->rd.R * items[k].ring->rd.n * sizeof(uint32_t));
+  }
+  auto b = std::make_unique<yh_batch>();
+  b->ctx = ctx;
+  b->cfg = cfg;
+  b->W = W;
+  b->ninst = count;
+  b->nbundles = (int)bundles.size();
+  for (int k = 0; k < count; ++k) b->maxlq = std::max(b->maxlq, items[k].ring->rd.LQ);
+  YH_HIP(hipMalloc(&b->d_mem, off));
+  char* base = (char*)b->d_mem;
+  std::vector<char> h(host_bytes, 0);
+  for (size_t i = 0; i < rings.size(); ++i)
+    memcpy(h.data() + o_rings + i * sizeof(yhk::RingDev), &rings[i]->rd, sizeof(yhk::RingDev));
+  for (int k = 0; k < count; ++k) {
+    yhk::BatchInst in{items[k].a, items[k].b, items[k].out, (uint32_t*)(base + o_y[k]), ridx[items[k].ring], 0};
+    (void)in.ring;
+    memcpy(h.data() + o_inst + k * sizeof(yhk::BatchInst), &in, sizeof(in));
+  }
+  if (!tasks.empty()) memcpy(h.data() + o_tasks, tasks.data(), tasks.size() * sizeof(yhk::BatchTask));
+  if (!bundles.empty()) memcpy(h.data() + o_bund, bundles.data(), bundles.size() * sizeof(yhk::BatchBundle));
+  memcpy(h.data() + o_qs, qstart.data(), qstart.size() * sizeof(int));
+  YH_HIP(hipMemcpy(base, h.data(), host_bytes, hipMemcpyHostToDevice));
+  yhk::BatchArgs& a = b->args;
+  a.rings = (const yhk::RingDev*)(base + o_rings);
+  a.inst = (const yhk::BatchInst*)(base + o_inst);
+  a.tasks = (const yhk::BatchTask*)(base + o_tasks);
+  a.bundles = (const yhk::BatchBundle*)(base + o_bund);
+  a.qstart = (const int*)(base + o_qs);
+  a.head = (int*)(base + o_head);
+  a.done = (int*)(base + o_done);
+  a.nqueues = nq;
+  b->grid = std::min(ctx->cus * cfg.wgs_per_cu, std::max(b->nbundles, 1));
+  *out = b.release();
+  return YH_OK;
+  YH_TRY_END
+}
+
+int yh_batch_run(yh_batch* b) {
+  if (!b) return fail(YH_ERR_ARG, "null batch");
+  if (int s = set_dev(b->ctx)) return s;
+  hipStream_t st = b->ctx->stream;
+  hipLaunchKernelGGL(yhk::k_batch_reset, dim3(std::max(1, std::min(64, (b->ninst + 255) / 256))), dim3(256), 0, st,
+                     b->args.head, b->args.nqueues, b->args.done, b->ninst);
+  YH_HIP(hipGetLastError());
+  LaunchTimer timer(b->ctx, YH_KERNEL_BATCH);
+  const dim3 grid(b->grid), block(b->W);
+  const size_t lds = yhk::batch_lds_bytes(b->W);
+  if (b->maxlq > 4)
+    hipLaunchKernelGGL((yhk::k_batch_prod<8, 2>), grid, block, lds, st, b->args);
+  else if (b->cfg.waves >= 4)
+    hipLaunchKernelGGL((yhk::k_batch_prod<4, 4>), grid, block, lds, st, b->args);
+  else
+    hipLaunchKernelGGL((yhk::k_batch_prod<4, 3>), grid, block, lds, st, b->args);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
+
+int yh_batch_destroy(yh_batch* b) {
+  if (!b) return YH_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  delete b;
+  return YH_OK;
+}
+
+int yh_batch_autotune(yh_ctx* ctx, const yh_batch_item* items, int count, const char* cache_path, int reps,
+                      yh_batch_config* out) {
+  if (!ctx || !out) return fail(YH_ERR_ARG, "null argument");
+  if (ctx->capturing) return fail(YH_ERR_ARG, "autotune during graph capture");
+  int mt = 0;
+  if (int s = validate_items(items, count, &mt)) return s;
+  if (int s = set_dev(ctx)) return s;
+  const std::string key = batch_signature(ctx, items, count);
+  if (cache_lookup(cache_path, key, out) && check_config(*out, mt) == YH_OK) return YH_OK;
+  reps = std::max(1, reps);
+  if (int s = drain_timing(ctx)) return s;
+  const bool was_timing = ctx->timing;
+  ctx->timing = false;
+  hipEvent_t e0, e1;
+  YH_HIP(hipEventCreate(&e0));
+  YH_HIP(hipEventCreate(&e1));
+  double best_ms = 1e30;
+  yh_batch_config best{};
+  int rc = YH_OK;
+  int maxlq = 0;
+  for (int k = 0; k < count; ++k) maxlq = std::max(maxlq, items[k].ring->rd.LQ);
+  std::vector<yh_batch_config> cand;
+  for (int W = std::max(64, mt); W <= yhk::BATCH_MAX_THREADS; W *= 2)
+    for (int waves : {2, 3, 4}) {
+      if ((maxlq > 4) != (waves == 2)) continue;  // the kernels that exist
+      for (int wgs : {1, 2, 3, 4}) {
+        if ((W / 64) * wgs > 4 * waves) continue;  // more workgroups than can be resident
+        for (int nq : {1, yhk::XCDS}) cand.push_back({W, wgs, nq, waves});
+      }
+    }
+  for (const yh_batch_config& c : cand) {
+    yh_batch* b = nullptr;
+    if ((rc = yh_batch_create(ctx, items, count, &c, &b))) break;
+    std::vector<float> t;
+    rc = yh_batch_run(b);  // warm-up
+    for (int r = 0; r < reps && rc == YH_OK; ++r) {
+      if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = fail(YH_ERR_HIP, "event record"); break; }
+      if ((rc = yh_batch_run(b))) break;
+      if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        rc = fail(YH_ERR_HIP, "event sync");
+        break;
+      }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      t.push_back(ms);
+    }
+    yh_batch_destroy(b);
+    if (rc) break;
+    std::sort(t.begin(), t.end());
+    const double med = t[t.size() / 2];
+    if (med < best_ms) { best_ms = med; best = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->timing = was_timing;
+  if (rc) return rc;
+  *out = best;
+  cache_store(cache_path, key, best, best_ms);
   return YH_OK;
 }
 

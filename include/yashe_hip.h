@@ -209,6 +209,53 @@ int yh_graph_end(yh_ctx* ctx, yh_graph** out);
 int yh_graph_launch(yh_graph* graph);
 int yh_graph_destroy(yh_graph* graph);
 
+/* ------------------------------------------------------------ uneven batches
+ * Many independent ring products c_k = a_k * b_k mod (x^n_k + 1, q_k) over
+ * rings of different n (16..8192) and q, in ONE persistent launch: every
+ * product becomes R_k per-prime tasks, packed into workgroup-sized bundles
+ * and drained from per-XCD work queues; the task that finishes a product's
+ * last prime runs its ICRT in the same launch. Operands and outputs are
+ * [L][n] device buffers of each item's ring (values in [0, q)). Results are
+ * identical to yh_ring_mul on each item alone. The reference multiplies one
+ * product at a time (Polynomial::operator*, src/common.h:136-226, followed by
+ * reduce() and %= q); there is no batched counterpart. */
+typedef struct yh_batch_item {
+  yh_ring* ring;
+  const uint64_t* a;
+  const uint64_t* b;
+  uint64_t* out;
+} yh_batch_item;
+
+/* Launch shape of a batch: threads per workgroup (64..512, a power of two,
+ * at least the largest n/16), workgroups per CU in the persistent grid, the
+ * number of work queues (1, or 8 = one per XCD) and the waves per SIMD the
+ * kernel is compiled for (3 or 4 when every q has at most 4 limbs, else 2). */
+typedef struct yh_batch_config {
+  int wg_threads;
+  int wgs_per_cu;
+  int queues;
+  int waves;
+} yh_batch_config;
+
+typedef struct yh_batch yh_batch;
+/* The default shape for these items (no measurement). */
+int yh_batch_default_config(const yh_batch_item* items, int count, yh_batch_config* cfg);
+/* Plans a batch: uploads the descriptors and sizes scratch once; cfg NULL =
+ * the default shape. The plan keeps the item pointers: run it again after
+ * writing new operands into the same buffers. */
+int yh_batch_create(yh_ctx* ctx, const yh_batch_item* items, int count, const yh_batch_config* cfg, yh_batch** out);
+/* Enqueues the batch on the context stream (capturable into a graph). */
+int yh_batch_run(yh_batch* batch);
+int yh_batch_destroy(yh_batch* batch);
+/* Picks the fastest launch shape for this batch composition by timing every
+ * candidate `reps` times (HIP events); with cache_path non-NULL the result is
+ * read from / appended to that JSON file under a key naming the device and
+ * the composition (histogram of n and limb counts), so a second call does
+ * not measure. Operands are only read and outputs written as by
+ * yh_batch_run. */
+int yh_batch_autotune(yh_ctx* ctx, const yh_batch_item* items, int count, const char* cache_path, int reps,
+                      yh_batch_config* out);
+
 /* ------------------------------------------------------------ instrumentation
  * Per-kernel timing with HIP events recorded on the context stream around
  * every launch of the fused kernels (no counterpart in the reference, which
@@ -218,6 +265,7 @@ int yh_graph_destroy(yh_graph* graph);
 #define YH_KERNEL_PROD 0
 #define YH_KERNEL_KEYSWITCH 1
 #define YH_KERNEL_ICRT 2
+#define YH_KERNEL_BATCH 3
 int yh_ctx_set_timing(yh_ctx* ctx, int enable);
 /* Synchronises, then returns the summed milliseconds and launch count of
  * kernel `kernel_id` since timing was enabled (or last reset). */
