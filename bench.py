@@ -40,7 +40,119 @@ def parse():
                     help="run only the warmup + timed batched steps (for rocprofv3 runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a residue add over three 256 MiB buffers (known HBM traffic) for counter calibration")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the uneven-batch and graph-circuit measurements")
+    ap.add_argument("--only-uneven", action="store_true",
+                    help="run only the uneven batched products (for rocprofv3 runs)")
+    ap.add_argument("--only-circuit", action="store_true",
+                    help="run only the graph-captured circuit (for rocprofv3 runs)")
     return ap.parse_args()
+
+
+TUNE_CACHE = os.path.join(ROOT, "cuyashe_amd", "tune_cache.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+
+
+def measured_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` at this batch from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if not line.startswith("traffic "):
+                    continue
+                kv = dict(tok.split("=", 1) for tok in line.split()[1:])
+                if kv.get("kernel") == kernel and int(kv.get("batch", -1)) == batch:
+                    return int(float(kv["bytes_per_launch"]))
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
+def bench_uneven(ya, ctx, reps=10):
+    """512 independent ring products, n in {1024..8192} weighted
+    .4/.3/.2/.1, q 127- or 255-bit, operands uniform mod q: one persistent
+    launch with the autotuned shape, against one launch pair per ring."""
+    from cuyashe_amd import workloads
+    spec = workloads.uneven_spec(512, 7)
+    rings, groups, items = workloads.make_batch(ctx, spec, 7)
+    cfg = ya.batch_autotune(ctx, items, TUNE_CACHE, reps=5)
+    b = ya.Batch(ctx, items, cfg)
+    b.run()
+    ctx.sync()
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / reps
+    k_ms, k_n = ctx.kernel_time(ya.KERNEL_BATCH)
+    ctx.set_timing(False)
+    refs = {key: ctx.alloc(g[3].nbytes) for key, g in groups.items()}
+
+    def per_ring():
+        for key, (A, B, OUT, ha, hb, idx) in groups.items():
+            rings[key].ring_mul(A, B, refs[key], len(idx))
+
+    per_ring()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        per_ring()
+    ctx.sync()
+    wall_ring = (time.perf_counter() - t0) / reps
+    exact = all(bool((g[2].download(g[3].shape) == refs[key].download(g[3].shape)).all())
+                for key, g in groups.items())
+    b.close()
+    for r in rings.values():
+        r.close()
+    hist = {}
+    for n, q in spec:
+        k = f"n{n}/q{q.bit_length()}"
+        hist[k] = hist.get(k, 0) + 1
+    return {"workload": "512 uneven ring products, n in {1024,2048,4096,8192} (.4/.3/.2/.1), q 2^127-1 or 2^255-19",
+            "value": round(512 / wall, 1), "unit": "products/s", "batch_ms": round(wall * 1e3, 4),
+            "kernel_avg_ms": round(k_ms / max(k_n, 1), 4), "launches_per_batch": 2,
+            "per_ring_launches_ms": round(wall_ring * 1e3, 4), "speedup_vs_per_ring": round(wall_ring / wall, 3),
+            "config": cfg, "rings": hist, "identical_to_per_ring": exact}
+
+
+def bench_circuit(ya, ctx, reps=5):
+    """128 ciphertexts at n=8192, q=2^255-19, t=2^16, w=2^32: depth-3 tree of
+    112 mul+keyswitch nodes and 256 additions, captured once into a HIP graph
+    and replayed."""
+    import numpy as np
+    from cuyashe_amd import workloads
+    from cuyashe_amd.scheme import Yashe
+    n, q, t = 8192, workloads.Q255, 1 << 16
+    Y = Yashe(ctx, n, q, t, 32, seed=41).generate_keys()
+    x = np.random.default_rng(42).integers(0, t, size=(128, n))
+    X = Y.encrypt(x)
+    bufs = {k: ctx.alloc(v * Y.elt_bytes) for k, v in workloads.CIRCUIT_BUFFERS.items()}
+    workloads.depth3_circuit(Y, X, **bufs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    workloads.depth3_circuit(Y, X, **bufs)
+    ctx.sync()
+    eager_s = time.perf_counter() - t0
+    eager = bufs["Z"].download((16, Y.L, n)).copy()
+    ctx.graph_begin()
+    workloads.depth3_circuit(Y, X, **bufs)
+    g = ctx.graph_end()
+    g.launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    ctx.sync()
+    graph_s = (time.perf_counter() - t0) / reps
+    same = bool((bufs["Z"].download((16, Y.L, n)) == eager).all())
+    g.close()
+    Y.ring.close()
+    return {"workload": "depth-3 circuit: 128 ciphertexts, n=8192, q=2^255-19, t=2^16, 112 mul+ks + 256 adds",
+            "graph_ms": round(graph_s * 1e3, 3), "eager_ms": round(eager_s * 1e3, 3),
+            "mul_ks_per_s": round(112 / graph_s, 1), "replay_equals_eager": same}
 
 
 def calibrate_copy(ya):
@@ -84,6 +196,12 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy(ya)
+        return
+    if args.only_uneven or args.only_circuit:
+        ctx = ya.Context(0)
+        out = bench_uneven(ya, ctx) if args.only_uneven else bench_circuit(ya, ctx)
+        print(json.dumps(out))
+        ctx.close()
         return
 
     rng = random.Random(20240613)
@@ -166,7 +284,8 @@ def main():
     dom = kernels[dom_name]
     achieved = dom["bytes_per_launch"] / (dom["avg_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom_name}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name, B),
+                "kernel": dom_name}
     ntt_bytes = kernels["k_prod"]["bytes_per_launch"] * prod_n + kernels["k_keyswitch"]["bytes_per_launch"] * ks_n
     ntt_ms = prod_ms + ks_ms
 
@@ -189,6 +308,10 @@ def main():
                "sample": f"{done} mul+keyswitch ops at n=4096, q=2^127-1 through the oracle (oracle/cpuref.c "
                          f"Stockham NTT + Python ICRT glue), single thread, {cpu_s:.1f} s"}
         exact = bool(match)
+
+    secondary = None
+    if not args.no_secondary:
+        secondary = {"uneven_products": bench_uneven(ya, ctx), "graph_circuit": bench_circuit(ya, ctx)}
 
     res = {
         "metric": METRIC,
@@ -217,6 +340,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "bit_exact_sample": exact,
+        "secondary": secondary,
     }
     evk.close()
     ring.close()

@@ -4,13 +4,12 @@ Every product of a mixed batch must be bit-identical to the same product run
 alone (yh_ring_mul) and to the oracle.
 """
 import json
-import os
-import random
 
 import numpy as np
 import pytest
 
 import cuyashe_amd as ya
+from cuyashe_amd import workloads
 from oracle import cpuref
 
 pytestmark = pytest.mark.gpu
@@ -20,45 +19,8 @@ Q255 = (1 << 255) - 19
 Q511 = (1 << 511) - 187
 
 
-def uniform_mod_q(rng, count, n, q):
-    """[count][L][n] limbs, uniform in [0, q) (top-limb rejection)."""
-    L = (q.bit_length() + 63) // 64
-    top_bits = q.bit_length() - 64 * (L - 1)
-    a = rng.integers(0, 1 << 63, size=(count, L, n), dtype=np.uint64) * np.uint64(2) + \
-        rng.integers(0, 2, size=(count, L, n), dtype=np.uint64)
-    mask = np.uint64((1 << top_bits) - 1) if top_bits < 64 else np.uint64((1 << 64) - 1)
-    a[:, L - 1, :] &= mask
-    qtop = np.uint64(q >> (64 * (L - 1)))
-    for c, j in zip(*np.nonzero(a[:, L - 1, :] >= qtop)):
-        v = sum(int(a[c, l, j]) << (64 * l) for l in range(L))
-        if v >= q:
-            a[c, :, j] = 0
-    return a
-
-
 def make_batch(ctx, spec, seed):
-    """spec: list of (n, q). Groups the products by ring into one buffer per
-    ring; returns rings, per-ring (A, B, OUT, host A, host B, indices) and the
-    item list in the original order."""
-    rng = np.random.default_rng(seed)
-    rings, groups = {}, {}
-    for k, (n, q) in enumerate(spec):
-        groups.setdefault((n, q), []).append(k)
-    items = [None] * len(spec)
-    bufs = {}
-    for key, idx in groups.items():
-        n, q = key
-        ring = ya.Ring(ctx, n, q)
-        rings[key] = ring
-        ha = uniform_mod_q(rng, len(idx), n, q)
-        hb = uniform_mod_q(rng, len(idx), n, q)
-        A, B = ctx.to_device(ha), ctx.to_device(hb)
-        OUT = ctx.alloc(ha.nbytes)
-        e = ring.L * n * 8
-        for m, k in enumerate(idx):
-            items[k] = (ring, A.view(m * e, e), B.view(m * e, e), OUT.view(m * e, e))
-        bufs[key] = (A, B, OUT, ha, hb, idx)
-    return rings, bufs, items
+    return workloads.make_batch(ctx, spec, seed)
 
 
 def alone(ctx, rings, bufs):
@@ -73,9 +35,7 @@ def alone(ctx, rings, bufs):
 
 
 def c3_spec(count, seed):
-    r = random.Random(seed)
-    ns = r.choices([1024, 2048, 4096, 8192], weights=[0.4, 0.3, 0.2, 0.1], k=count)
-    return [(n, Q127 if r.random() < 0.5 else Q255) for n in ns]
+    return workloads.uneven_spec(count, seed)
 
 
 def test_batch_small_rings_match_oracle(gpu_ctx):

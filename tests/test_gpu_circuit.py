@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import cuyashe_amd as ya
+from cuyashe_amd import workloads
 from cuyashe_amd.scheme import Yashe, negacyclic_mul_mod
 from oracle import cpuref
 
@@ -103,43 +104,8 @@ def test_scheme_roundtrip_add_mul(gpu_ctx, n, q, t):
     Y.ring.close()
 
 
-def _circuit(Y, X, A, Pm, B, Rm, S, U, V, Z):
-    """128 inputs -> depth-3 tree of 64 + 32 + 16 mul+keyswitch, 256 adds.
-
-    a_i = x_i + x_{i+1 mod 128}        128 adds
-    p_i = a_i a_{i+64}, i < 64          64 mul+ks
-    b_i = p_i + x_i                     64 adds
-    r_i = b_i b_{i+32}, i < 32          32 mul+ks
-    s_i = r_i + r_{i+1 mod 32}          32 adds
-    u_i = s_i s_{i+16}, i < 16          16 mul+ks
-    v_i = u_i + x_i                     16 adds
-    z_i = v_i + v_{i+1 mod 16}          16 adds
-    """
-    e = Y.elt_bytes
-    ring = Y.ring
-    ring.ring_addsub(X.view(0, 127 * e), X.view(e, 127 * e), A.view(0, 127 * e), 127)
-    ring.ring_addsub(X.view(127 * e, e), X.view(0, e), A.view(127 * e, e), 1)
-    ring.mul_keyswitch(Y.evk, A.view(0, 64 * e), A.view(64 * e, 64 * e), Pm, 64)
-    ring.ring_addsub(Pm, X.view(0, 64 * e), B, 64)
-    ring.mul_keyswitch(Y.evk, B.view(0, 32 * e), B.view(32 * e, 32 * e), Rm, 32)
-    ring.ring_addsub(Rm.view(0, 31 * e), Rm.view(e, 31 * e), S.view(0, 31 * e), 31)
-    ring.ring_addsub(Rm.view(31 * e, e), Rm.view(0, e), S.view(31 * e, e), 1)
-    ring.mul_keyswitch(Y.evk, S.view(0, 16 * e), S.view(16 * e, 16 * e), U, 16)
-    ring.ring_addsub(U, X.view(0, 16 * e), V, 16)
-    ring.ring_addsub(V.view(0, 15 * e), V.view(e, 15 * e), Z.view(0, 15 * e), 15)
-    ring.ring_addsub(V.view(15 * e, e), V.view(0, e), Z.view(15 * e, e), 1)
-
-
-def _plain_circuit(x, t):
-    mul = lambda a, b: negacyclic_mul_mod(a, b, t)  # noqa: E731
-    a = (x + np.roll(x, -1, axis=0)) % t
-    p = np.stack([mul(a[i], a[i + 64]) for i in range(64)])
-    b = (p + x[:64]) % t
-    r = np.stack([mul(b[i], b[i + 32]) for i in range(32)])
-    s = (r + np.roll(r, -1, axis=0)) % t
-    u = np.stack([mul(s[i], s[i + 16]) for i in range(16)])
-    v = (u + x[:16]) % t
-    return (v + np.roll(v, -1, axis=0)) % t
+_circuit = workloads.depth3_circuit
+_plain_circuit = workloads.depth3_plain
 
 
 def test_graph_circuit_depth3(gpu_ctx):
@@ -153,8 +119,7 @@ def test_graph_circuit_depth3(gpu_ctx):
     x = rng.integers(0, t, size=(128, n))
     X = Y.encrypt(x)
     e = Y.elt_bytes
-    bufs = dict(A=128, Pm=64, B=64, Rm=32, S=32, U=16, V=16, Z=16)
-    B = {k: gpu_ctx.alloc(v * e) for k, v in bufs.items()}
+    B = {k: gpu_ctx.alloc(v * e) for k, v in workloads.CIRCUIT_BUFFERS.items()}
     _circuit(Y, X, **B)
     eager = B["Z"].download((16, Y.L, n)).copy()
     p0 = get(B["Pm"], Y.ring, 1)[0]
