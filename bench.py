@@ -46,6 +46,10 @@ def parse():
                     help="run only the uneven batched products (for rocprofv3 runs)")
     ap.add_argument("--only-circuit", action="store_true",
                     help="run only the graph-captured circuit (for rocprofv3 runs)")
+    ap.add_argument("--serving", action="store_true",
+                    help="run only the serving loop over a sweep of arrival rates")
+    ap.add_argument("--serving-requests", type=int, default=2000)
+    ap.add_argument("--serving-qps", type=str, default="2000,5000,10000,20000")
     return ap.parse_args()
 
 
@@ -116,6 +120,27 @@ def bench_uneven(ya, ctx, reps=10):
             "kernel_avg_ms": round(k_ms / max(k_n, 1), 4), "launches_per_batch": 2,
             "per_ring_launches_ms": round(wall_ring * 1e3, 4), "speedup_vs_per_ring": round(wall_ring / wall, 3),
             "config": cfg, "rings": hist, "identical_to_per_ring": exact}
+
+
+def bench_serving(ya, ctx, qps_list, count):
+    """Continuous batching over 12 resident rings (n in {2048..16384}, q
+    127/255/511-bit, t=1024): Poisson arrivals, 60% add / 30% mul+keyswitch /
+    10% encrypt+decrypt, admission by HBM footprint, every result decrypted
+    and checked."""
+    from cuyashe_amd.serving import Server, c5_rings, poisson_requests
+    rings = c5_rings()
+    t_setup = time.perf_counter()
+    srv = Server(ctx, rings, t=1024, pool=32, slots=64, max_batch=64, seed=11)
+    setup_s = time.perf_counter() - t_setup
+    srv.run(poisson_requests(200, 20000.0, rings, 1024, 32, seed=99))  # warm-up: scratch and staging sized
+    points = []
+    for k, qps in enumerate(qps_list):
+        st = srv.run(poisson_requests(count, qps, rings, 1024, 32, seed=k))
+        st["offered_qps"] = qps
+        points.append(st)
+    return {"workload": "serving: 12 rings n in {2048,4096,8192,16384} x q in {2^127-1, 2^255-19, 2^511-187}, t=1024, "
+                        "Poisson arrivals, 60% add / 30% mul+ks / 10% enc+dec",
+            "setup_s": round(setup_s, 2), "points": points}
 
 
 def bench_circuit(ya, ctx, reps=5):
@@ -197,9 +222,12 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(ya)
         return
-    if args.only_uneven or args.only_circuit:
+    if args.only_uneven or args.only_circuit or args.serving:
         ctx = ya.Context(0)
-        out = bench_uneven(ya, ctx) if args.only_uneven else bench_circuit(ya, ctx)
+        if args.serving:
+            out = bench_serving(ya, ctx, [float(v) for v in args.serving_qps.split(",")], args.serving_requests)
+        else:
+            out = bench_uneven(ya, ctx) if args.only_uneven else bench_circuit(ya, ctx)
         print(json.dumps(out))
         ctx.close()
         return
@@ -311,7 +339,8 @@ def main():
 
     secondary = None
     if not args.no_secondary:
-        secondary = {"uneven_products": bench_uneven(ya, ctx), "graph_circuit": bench_circuit(ya, ctx)}
+        secondary = {"uneven_products": bench_uneven(ya, ctx), "graph_circuit": bench_circuit(ya, ctx),
+                     "serving": bench_serving(ya, ctx, [10000.0], 1500)}
 
     res = {
         "metric": METRIC,

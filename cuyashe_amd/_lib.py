@@ -110,6 +110,17 @@ _SIGS = {
     "yh_batch_create": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.POINTER(BatchConfig),
                                        ctypes.POINTER(_vp)]),
     "yh_batch_run": (ctypes.c_int, [_vp]),
+    "yh_ring_addsub_list": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "yh_yashe_mul_keyswitch_list": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_yashe_decrypt_list": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_ring_use_own_stream": (ctypes.c_int, [_vp]),
+    "yh_ring_sync": (ctypes.c_int, [_vp]),
+    "yh_event_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "yh_event_record": (ctypes.c_int, [_vp, _vp, _vp]),
+    "yh_event_query": (ctypes.c_int, [_vp]),
+    "yh_event_sync": (ctypes.c_int, [_vp]),
+    "yh_event_elapsed_ms": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "yh_event_destroy": (ctypes.c_int, [_vp]),
     "yh_batch_destroy": (ctypes.c_int, [_vp]),
     "yh_batch_autotune": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                          ctypes.POINTER(BatchConfig)]),
@@ -291,6 +302,44 @@ class Context:
             self.h = None
 
 
+class Event:
+    """A point in a ring's stream (or the context stream when ring is None)."""
+
+    def __init__(self, ctx: "Context"):
+        h = _vp()
+        check(lib().yh_event_create(ctx.h, ctypes.byref(h)), "yh_event_create")
+        self.h = h.value
+        self.ctx = ctx
+
+    def record(self, ring: Optional["Ring"] = None):
+        check(lib().yh_event_record(self.h, self.ctx.h, ring.h if ring is not None else None), "yh_event_record")
+
+    def done(self) -> bool:
+        rc = lib().yh_event_query(self.h)
+        if rc < 0:
+            check(rc, "yh_event_query")
+        return rc == 1
+
+    def sync(self):
+        check(lib().yh_event_sync(self.h), "yh_event_sync")
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = ctypes.c_float(0)
+        check(lib().yh_event_elapsed_ms(self.h, stop.h, ctypes.byref(ms)), "yh_event_elapsed_ms")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().yh_event_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Graph:
     """An instantiated capture of ring operations, replayed on the context stream."""
 
@@ -369,6 +418,27 @@ class Ring:
 
     def ring_mul(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
         check(lib().yh_ring_mul(self.h, a.ptr, b.ptr, out.ptr, batch), "yh_ring_mul")
+
+    # -- pointer-list variants: lists are DeviceBuffers holding `batch` device pointers
+    def addsub_list(self, a_list: DeviceBuffer, b_list: DeviceBuffer, out_list: DeviceBuffer, batch: int,
+                    sub: bool = False):
+        check(lib().yh_ring_addsub_list(self.h, a_list.ptr, b_list.ptr, out_list.ptr, batch, YH_SUB if sub else YH_ADD),
+              "yh_ring_addsub_list")
+
+    def mul_keyswitch_list(self, evk: "Evk", c1_list: DeviceBuffer, c2_list: DeviceBuffer, out_list: DeviceBuffer,
+                           batch: int):
+        check(lib().yh_yashe_mul_keyswitch_list(self.h, evk.h, c1_list.ptr, c2_list.ptr, out_list.ptr, batch),
+              "yh_yashe_mul_keyswitch_list")
+
+    def decrypt_list(self, f: DeviceBuffer, c_list: DeviceBuffer, m_list: DeviceBuffer, batch: int):
+        check(lib().yh_yashe_decrypt_list(self.h, f.ptr, c_list.ptr, m_list.ptr, batch), "yh_yashe_decrypt_list")
+
+    def use_own_stream(self):
+        """Run this ring's work on a stream of its own (overlaps other rings)."""
+        check(lib().yh_ring_use_own_stream(self.h), "yh_ring_use_own_stream")
+
+    def sync(self):
+        check(lib().yh_ring_sync(self.h), "yh_ring_sync")
 
     def ring_mul_bcast(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
         """out_k = a * b_k for one a and a batch of b."""

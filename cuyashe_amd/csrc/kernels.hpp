@@ -241,6 +241,10 @@ __device__ __forceinline__ uint32_t finish_residue(uint64_t x, uint64_t ninv, ui
 // offset pair*ostride; coefficients j >= ncoef read as zero.
 struct ProdArgs {
   const uint64_t* A; const uint64_t* B;
+  // optional per-pair operand addresses (device arrays of device pointers);
+  // when set they replace A + pair*ostride_a / B + pair*ostride_b
+  const uint64_t* const* a_list;
+  const uint64_t* const* b_list;
   int lin_a, lin_b;
   int ncoef_a, ncoef_b;
   size_t lstride_a, lstride_b;
@@ -277,9 +281,11 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_prod(RingDev
   const uint64_t b = rd.barrett[i];
   const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
   uint64_t xa[E], xb[E];
-  load_crt<LOGN, NEGA>(xa, pa.A + pair * pa.ostride_a, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, p, b, pa.tab.twist);
+  const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
+  const uint64_t* Bp = pa.b_list ? pa.b_list[pair] : pa.B + pair * pa.ostride_b;
+  load_crt<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, p, b, pa.tab.twist);
   ntt16_fwd<LOGN>(xa, lds, pa.tab.fwd);
-  load_crt<LOGN, NEGA>(xb, pa.B + pair * pa.ostride_b, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, p, b, pa.tab.twist);
+  load_crt<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, p, b, pa.tab.twist);
   __syncthreads();  // every thread has left the LDS of the first transform
   ntt16_fwd<LOGN>(xb, lds, pa.tab.fwd);
 #pragma unroll
@@ -308,9 +314,11 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
   const uint64_t b = rd.barrett[i];
   const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
   uint64_t ra[64], rb[64], rc[64];
+  const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
+  const uint64_t* Bp = pa.b_list ? pa.b_list[pair] : pa.B + pair * pa.ostride_b;
   for (int j = 0; j < N; ++j) {
-    const uint64_t* a = pa.A + pair * pa.ostride_a + j;
-    const uint64_t* bb = pa.B + pair * pa.ostride_b + j;
+    const uint64_t* a = A + j;
+    const uint64_t* bb = Bp + j;
     ra[j] = j < pa.ncoef_a ? center(crt_residue(a, pa.lin_a, pa.lstride_a, pw, p, b), p) : 0;
     rb[j] = j < pa.ncoef_b ? center(crt_residue(bb, pa.lin_b, pa.lstride_b, pw, p, b), p) : 0;
     rc[j] = 0;
@@ -647,6 +655,7 @@ struct IcrtArgs {
   size_t out_stride;     // elements between limbs
   size_t out_pstride;    // elements between pairs
   int positive;          // 1: result in [0, M) (plain product); 0: centred
+  uint64_t* const* out_list;  // optional per-pair output addresses (replace out + pair*out_pstride)
 };
 
 // ICRT of one coefficient plus its epilogue. y points at the residue of
@@ -818,8 +827,9 @@ __global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
   const int pair = blockIdx.y;
   if (j >= ia.ncoef) return;
   const int RS = rd.set_R[ia.set];
-  icrt_coef<LQ>(rd, ia.Y + (size_t)pair * RS * ia.N + j, ia.N, ia.set, ia.mode, ia.positive,
-                ia.out + pair * ia.out_pstride + j, ia.out_limbs, ia.out_stride);
+  uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
+  icrt_coef<LQ>(rd, ia.Y + (size_t)pair * RS * ia.N + j, ia.N, ia.set, ia.mode, ia.positive, o + j, ia.out_limbs,
+                ia.out_stride);
 }
 
 // ---------------------------------------------------------------- primitives
@@ -914,14 +924,23 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ntt_cyclic(u
 
 // Coefficient-domain add/sub mod q for ring elements in [0, q) (homomorphic
 // addition on resident ciphertexts).
+// Optional per-element address lists (device arrays of device pointers)
+// replace A/B/C + element*LQ*n.
+struct AddsubLists {
+  const uint64_t* const* a;
+  const uint64_t* const* b;
+  uint64_t* const* c;
+};
+
 template <int LQ>
-__global__ void k_addsub_q(RingDev rd, const uint64_t* A, const uint64_t* B, uint64_t* C, int n, size_t count, int sub) {
+__global__ void k_addsub_q(RingDev rd, const uint64_t* A, const uint64_t* B, uint64_t* C, int n, size_t count, int sub,
+                           AddsubLists lists) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
   size_t pr = t / n, j = t % n;
-  const uint64_t* a = A + pr * LQ * n + j;
-  const uint64_t* b = B + pr * LQ * n + j;
-  uint64_t* c = C + pr * LQ * n + j;
+  const uint64_t* a = (lists.a ? lists.a[pr] : A + pr * LQ * n) + j;
+  const uint64_t* b = (lists.b ? lists.b[pr] : B + pr * LQ * n) + j;
+  uint64_t* c = (lists.c ? lists.c[pr] : C + pr * LQ * n) + j;
   uint64_t x[LQ + 1], y[LQ + 1], qe[LQ + 1];
 #pragma unroll
   for (int l = 0; l < LQ; ++l) { x[l] = a[(size_t)l * n]; y[l] = b[(size_t)l * n]; qe[l] = rd.q[l]; }

@@ -92,6 +92,13 @@ struct yh_ring {
   uint64_t* d_tmp = nullptr;   // ring-element scratch (mul+keyswitch intermediate)
   size_t tmp_bytes = 0;
   std::map<int, yh_ring*> sub;  // rings x^m + 1 over the same q, m < n (inverse recursion)
+  hipStream_t stream = nullptr;  // the context stream, or the ring's own (yh_ring_use_own_stream)
+  bool own_stream = false;
+};
+
+struct yh_event {
+  int device = 0;
+  hipEvent_t ev = nullptr;
 };
 
 struct yh_evk {
@@ -116,18 +123,19 @@ hipEvent_t take_event(yh_ctx* ctx) {
 struct LaunchTimer {
   yh_ctx* ctx;
   int kernel;
+  hipStream_t stream;
   hipEvent_t start = nullptr;
-  LaunchTimer(yh_ctx* c, int k) : ctx(c), kernel(k) {
+  LaunchTimer(yh_ctx* c, int k, hipStream_t s) : ctx(c), kernel(k), stream(s) {
     if (ctx->timing && !ctx->capturing) {
       start = take_event(ctx);
-      if (start) (void)hipEventRecord(start, ctx->stream);
+      if (start) (void)hipEventRecord(start, stream);
     }
   }
   ~LaunchTimer() {
     if (!start) return;
     hipEvent_t stop = take_event(ctx);
     if (!stop) return;
-    (void)hipEventRecord(stop, ctx->stream);
+    (void)hipEventRecord(stop, stream);
     ctx->pending.push_back({kernel, start, stop});
   }
 };
@@ -137,11 +145,12 @@ int set_dev(yh_ctx* ctx) {
   return YH_OK;
 }
 
-int ensure(yh_ctx* ctx, void** buf, size_t* have, size_t need) {
+// Grows a scratch buffer used by work on `stream` (waits for that work first).
+int ensure(yh_ctx* ctx, hipStream_t stream, void** buf, size_t* have, size_t need) {
   if (*have >= need) return YH_OK;
   if (ctx->capturing) return fail(YH_ERR_ARG, "scratch would grow during graph capture; run the sequence once first");
   if (*buf) {
-    YH_HIP(hipStreamSynchronize(ctx->stream));
+    YH_HIP(hipStreamSynchronize(stream));
     YH_HIP(hipFree(*buf));
     *buf = nullptr;
     *have = 0;
@@ -212,9 +221,9 @@ void set_lds_attrs() {
 // Launch the fused product kernel for `batch` pairs on N-point transforms.
 int launch_prod(yh_ring* r, yhk::ProdArgs pa, int N, int batch, bool negacyclic) {
   pa.batch = batch;
-  hipStream_t s = r->ctx->stream;
+  hipStream_t s = r->stream;
   int R = r->rd.set_R[pa.set];
-  LaunchTimer timer(r->ctx, YH_KERNEL_PROD);
+  LaunchTimer timer(r->ctx, YH_KERNEL_PROD, r->stream);
   if (N >= 16) {
     dim3 grid(yhk::xcd_grid(batch, R)), block(N / 16);
     if (negacyclic) {
@@ -243,11 +252,11 @@ int launch_prod(yh_ring* r, yhk::ProdArgs pa, int N, int batch, bool negacyclic)
 template <int LQ>
 void launch_icrt_t(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
   dim3 block(256), grid((ia.ncoef + 255) / 256, batch);
-  hipLaunchKernelGGL(yhk::k_icrt<LQ>, grid, block, 0, r->ctx->stream, r->rd, ia);
+  hipLaunchKernelGGL(yhk::k_icrt<LQ>, grid, block, 0, r->stream, r->rd, ia);
 }
 
 int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
-  LaunchTimer timer(r->ctx, YH_KERNEL_ICRT);
+  LaunchTimer timer(r->ctx, YH_KERNEL_ICRT, r->stream);
   switch (r->rd.LQ) {
     case 1: launch_icrt_t<1>(r, ia, batch); break;
     case 2: launch_icrt_t<2>(r, ia, batch); break;
@@ -264,10 +273,28 @@ int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
 }
 
 template <int LQ>
-void launch_addsub_t(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* c, int batch, int sub) {
+void launch_addsub_t(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* c, int batch, int sub,
+                     yhk::AddsubLists lists) {
   size_t count = (size_t)batch * r->rd.n;
   dim3 block(256), grid((unsigned)((count + 255) / 256));
-  hipLaunchKernelGGL(yhk::k_addsub_q<LQ>, grid, block, 0, r->ctx->stream, r->rd, a, b, c, r->rd.n, count, sub);
+  hipLaunchKernelGGL(yhk::k_addsub_q<LQ>, grid, block, 0, r->stream, r->rd, a, b, c, r->rd.n, count, sub, lists);
+}
+
+int addsub_impl(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* c, int batch, int sub,
+                yhk::AddsubLists lists) {
+  switch (r->rd.LQ) {
+    case 1: launch_addsub_t<1>(r, a, b, c, batch, sub, lists); break;
+    case 2: launch_addsub_t<2>(r, a, b, c, batch, sub, lists); break;
+    case 3: launch_addsub_t<3>(r, a, b, c, batch, sub, lists); break;
+    case 4: launch_addsub_t<4>(r, a, b, c, batch, sub, lists); break;
+    case 5: launch_addsub_t<5>(r, a, b, c, batch, sub, lists); break;
+    case 6: launch_addsub_t<6>(r, a, b, c, batch, sub, lists); break;
+    case 7: launch_addsub_t<7>(r, a, b, c, batch, sub, lists); break;
+    case 8: launch_addsub_t<8>(r, a, b, c, batch, sub, lists); break;
+    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
 }
 
 yhk::IcrtArgs icrt_args(yh_ring* r, int N, int ncoef, int set, int mode, uint64_t* out, int out_limbs,
@@ -286,9 +313,11 @@ yhk::IcrtArgs icrt_args(yh_ring* r, int N, int ncoef, int set, int mode, uint64_
   return ia;
 }
 
-int ring_product(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* out, int batch, int mode) {
+int ring_product(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* out, int batch, int mode,
+                 const uint64_t* const* a_list = nullptr, const uint64_t* const* b_list = nullptr,
+                 uint64_t* const* out_list = nullptr) {
   const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
   if (st) return st;
   yhk::ProdArgs pa{};
   pa.A = a; pa.B = b;
@@ -300,20 +329,25 @@ int ring_product(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* out
   pa.logN = r->rd.logn;
   pa.tab = r->rd.tab;
   pa.set = 0;
+  pa.a_list = a_list;
+  pa.b_list = b_list;
   st = launch_prod(r, pa, n, batch, true);
   if (st) return st;
-  return launch_icrt(r, icrt_args(r, n, n, 0, mode, out, LQ, n, (size_t)LQ * n, 0), batch);
+  yhk::IcrtArgs ia = icrt_args(r, n, n, 0, mode, out, LQ, n, (size_t)LQ * n, 0);
+  ia.out_list = out_list;
+  return launch_icrt(r, ia, batch);
 }
 
 template <int LQ>
 void launch_reduce_t(yh_ring* r, const uint64_t* c, int lin, uint64_t* out, int batch) {
   dim3 block(256), grid((r->rd.n + 255) / 256, batch);
-  hipLaunchKernelGGL(yhk::k_reduce<LQ>, grid, block, 0, r->ctx->stream, r->rd, c, lin, out);
+  hipLaunchKernelGGL(yhk::k_reduce<LQ>, grid, block, 0, r->stream, r->rd, c, lin, out);
 }
 
-int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out, int batch) {
+int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out, int batch,
+                   uint64_t* const* out_list = nullptr) {
   const int n = r->rd.n, LQ = r->rd.LQ, Rks = r->rd.set_R[1];
-  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)batch * Rks * n * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * Rks * n * sizeof(uint32_t));
   if (st) return st;
   yhk::KsArgs ka{};
   ka.C = c;
@@ -323,20 +357,22 @@ int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out
   ka.batch = batch;
   dim3 grid(Rks, batch);
   dim3 grid_x(yhk::xcd_grid(batch, Rks));
-  LaunchTimer* timer = new LaunchTimer(r->ctx, YH_KERNEL_KEYSWITCH);
+  LaunchTimer* timer = new LaunchTimer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
   if (n >= 16) {
     switch (r->rd.logn) {
-#define X(L) case L: hipLaunchKernelGGL(yhk::k_keyswitch<L>, grid_x, dim3(n / 16), ks_lds_bytes<L>(), r->ctx->stream, r->rd, ka); break;
+#define X(L) case L: hipLaunchKernelGGL(yhk::k_keyswitch<L>, grid_x, dim3(n / 16), ks_lds_bytes<L>(), r->stream, r->rd, ka); break;
       YH_LOGN_CASES(X)
 #undef X
       default: delete timer; return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
     }
   } else {
-    hipLaunchKernelGGL(yhk::k_keyswitch_small, grid, dim3(64), 0, r->ctx->stream, r->rd, ka, k->d);
+    hipLaunchKernelGGL(yhk::k_keyswitch_small, grid, dim3(64), 0, r->stream, r->rd, ka, k->d);
   }
   delete timer;
   YH_HIP(hipGetLastError());
-  return launch_icrt(r, icrt_args(r, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0), batch);
+  yhk::IcrtArgs ia = icrt_args(r, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0);
+  ia.out_list = out_list;
+  return launch_icrt(r, ia, batch);
 }
 
 }  // namespace
@@ -497,6 +533,7 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   if (int s = set_dev(ctx)) return s;
   auto r = std::make_unique<yh_ring>();
   r->ctx = ctx;
+  r->stream = ctx->stream;
   r->P = std::make_unique<yh::RingParams>(n, yh::BigInt::from_limbs(q, q_limbs), t, w_bits);
   const yh::RingParams& P = *r->P;
   // one table buffer: pack every array, remember offsets
@@ -528,8 +565,9 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
 int yh_ring_destroy(yh_ring* r) {
   if (!r) return YH_OK;
   (void)hipSetDevice(r->ctx->device);
-  (void)hipStreamSynchronize(r->ctx->stream);
+  (void)hipStreamSynchronize(r->stream);
   for (auto& kv : r->sub) yh_ring_destroy(kv.second);
+  if (r->own_stream) (void)hipStreamDestroy(r->stream);
   (void)hipFree(r->d_tables);
   if (r->d_ntt) (void)hipFree(r->d_ntt);
   if (r->d_y) (void)hipFree(r->d_y);
@@ -595,7 +633,7 @@ int yh_crt(yh_ring* r, const uint64_t* d_coefs, int limbs, int ncoef, int spacin
   if (ncoef < 0 || spacing < ncoef || spacing <= 0) return fail(YH_ERR_ARG, "bad spacing");
   if (int s = set_dev(r->ctx)) return s;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  hipLaunchKernelGGL(yhk::k_crt, grid, block, 0, r->ctx->stream, r->rd, d_coefs, limbs, ncoef, (size_t)ncoef,
+  hipLaunchKernelGGL(yhk::k_crt, grid, block, 0, r->stream, r->rd, d_coefs, limbs, ncoef, (size_t)ncoef,
                      d_res, spacing);
   YH_HIP(hipGetLastError());
   return YH_OK;
@@ -605,10 +643,10 @@ int yh_icrt(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, uint64_t*
   if (!r || !d_res || !d_coefs) return fail(YH_ERR_ARG, "null argument");
   if (ncoef <= 0 || ncoef > spacing || out_limbs < 1) return fail(YH_ERR_ARG, "bad sizes");
   if (int s = set_dev(r->ctx)) return s;
-  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)r->rd.R * spacing * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)r->rd.R * spacing * sizeof(uint32_t));
   if (st) return st;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  hipLaunchKernelGGL(yhk::k_residue_to_y, grid, block, 0, r->ctx->stream, r->rd, d_res, spacing, r->d_y, 0);
+  hipLaunchKernelGGL(yhk::k_residue_to_y, grid, block, 0, r->stream, r->rd, d_res, spacing, r->d_y, 0);
   YH_HIP(hipGetLastError());
   return launch_icrt(r, icrt_args(r, spacing, ncoef, 0, yhk::EPI_RAW, d_coefs, out_limbs, ncoef, 0, 1), 1);
 }
@@ -646,7 +684,7 @@ int yh_residue_addsub(yh_ring* r, uint64_t* d_c, const uint64_t* d_a, const uint
   if (!r || !d_c || !d_a || !d_b || spacing <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (int s = set_dev(r->ctx)) return s;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  hipLaunchKernelGGL(yhk::k_residue_addsub, grid, block, 0, r->ctx->stream, r->rd, d_c, d_a, d_b, spacing,
+  hipLaunchKernelGGL(yhk::k_residue_addsub, grid, block, 0, r->stream, r->rd, d_c, d_a, d_b, spacing,
                      op == YH_SUB ? 1 : 0);
   YH_HIP(hipGetLastError());
   return YH_OK;
@@ -663,7 +701,7 @@ int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t
   if (N > 16384) return fail(YH_ERR_UNSUPPORTED, "plain product longer than 16384");
   if (int s = set_dev(r->ctx)) return s;
   const int R = r->rd.R, LQ = r->rd.LQ;
-  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)R * N * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)R * N * sizeof(uint32_t));
   if (st) return st;
   DevTables tw{};
   if (N >= 16) {
@@ -706,20 +744,7 @@ int yh_ring_mul(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* 
 int yh_ring_addsub(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch, int op) {
   if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (int s = set_dev(r->ctx)) return s;
-  int sub = op == YH_SUB ? 1 : 0;
-  switch (r->rd.LQ) {
-    case 1: launch_addsub_t<1>(r, d_a, d_b, d_out, batch, sub); break;
-    case 2: launch_addsub_t<2>(r, d_a, d_b, d_out, batch, sub); break;
-    case 3: launch_addsub_t<3>(r, d_a, d_b, d_out, batch, sub); break;
-    case 4: launch_addsub_t<4>(r, d_a, d_b, d_out, batch, sub); break;
-    case 5: launch_addsub_t<5>(r, d_a, d_b, d_out, batch, sub); break;
-    case 6: launch_addsub_t<6>(r, d_a, d_b, d_out, batch, sub); break;
-    case 7: launch_addsub_t<7>(r, d_a, d_b, d_out, batch, sub); break;
-    case 8: launch_addsub_t<8>(r, d_a, d_b, d_out, batch, sub); break;
-    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
-  }
-  YH_HIP(hipGetLastError());
-  return YH_OK;
+  return addsub_impl(r, d_a, d_b, d_out, batch, op == YH_SUB ? 1 : 0, yhk::AddsubLists{nullptr, nullptr, nullptr});
 }
 
 int yh_ring_reduce(yh_ring* r, const uint64_t* d_c, int in_limbs, uint64_t* d_out, int batch) {
@@ -760,13 +785,13 @@ int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
   dim3 grid(r->rd.set_R[1], r->rd.lw);
   if (r->rd.n >= 16) {
     switch (r->rd.logn) {
-#define X(L) case L: hipLaunchKernelGGL(yhk::k_evk_prep<L>, grid, dim3(r->rd.n / 16), lds_bytes(r->rd.n), r->ctx->stream, r->rd, d_gamma, k->d); break;
+#define X(L) case L: hipLaunchKernelGGL(yhk::k_evk_prep<L>, grid, dim3(r->rd.n / 16), lds_bytes(r->rd.n), r->stream, r->rd, d_gamma, k->d); break;
       YH_LOGN_CASES(X)
 #undef X
       default: break;
     }
   } else {
-    hipLaunchKernelGGL(yhk::k_evk_residues, grid, dim3(64), 0, r->ctx->stream, r->rd, d_gamma, k->d);
+    hipLaunchKernelGGL(yhk::k_evk_residues, grid, dim3(64), 0, r->stream, r->rd, d_gamma, k->d);
   }
   e = hipGetLastError();
   if (e != hipSuccess) { (void)hipFree(k->d); delete k; return fail(YH_ERR_HIP, hipGetErrorString(e)); }
@@ -777,7 +802,7 @@ int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
 int yh_evk_destroy(yh_evk* k) {
   if (!k) return YH_OK;
   (void)hipSetDevice(k->ring->ctx->device);
-  (void)hipStreamSynchronize(k->ring->ctx->stream);
+  (void)hipStreamSynchronize(k->ring->stream);
   (void)hipFree(k->d);
   delete k;
   return YH_OK;
@@ -798,19 +823,17 @@ int yh_yashe_mul_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, co
   if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
   if (int s = set_dev(r->ctx)) return s;
   size_t need = (size_t)batch * r->rd.LQ * r->rd.n * sizeof(uint64_t);
-  int st = ensure(r->ctx, (void**)&r->d_tmp, &r->tmp_bytes, need);
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_tmp, &r->tmp_bytes, need);
   if (st) return st;
   st = ring_product(r, d_c1, d_c2, r->d_tmp, batch, yhk::EPI_SCALE);
   if (st) return st;
   return keyswitch_impl(r, k, r->d_tmp, d_out, batch);
 }
 
-int yh_yashe_decrypt(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch) {
-  if (!r || !d_f || !d_c || !d_m || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
-  if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
-  if (int s = set_dev(r->ctx)) return s;
+static int decrypt_impl(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch,
+                        const uint64_t* const* c_list, uint64_t* const* m_list) {
   const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
   if (st) return st;
   yhk::ProdArgs pa{};
   pa.A = d_f; pa.B = d_c;
@@ -823,16 +846,124 @@ int yh_yashe_decrypt(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint6
   pa.logN = r->rd.logn;
   pa.tab = r->rd.tab;
   pa.set = 0;
+  pa.b_list = c_list;
   st = launch_prod(r, pa, n, batch, true);
   if (st) return st;
-  return launch_icrt(r, icrt_args(r, n, n, 0, yhk::EPI_DECRYPT, d_m, 1, n, n, 0), batch);
+  yhk::IcrtArgs ia = icrt_args(r, n, n, 0, yhk::EPI_DECRYPT, d_m, 1, n, n, 0);
+  ia.out_list = m_list;
+  return launch_icrt(r, ia, batch);
+}
+
+int yh_yashe_decrypt(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch) {
+  if (!r || !d_f || !d_c || !d_m || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
+  if (int s = set_dev(r->ctx)) return s;
+  return decrypt_impl(r, d_f, d_c, d_m, batch, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------- pointer-list variants
+int yh_ring_addsub_list(yh_ring* r, const uint64_t* const* d_a, const uint64_t* const* d_b, uint64_t* const* d_out,
+                        int batch, int op) {
+  if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (int s = set_dev(r->ctx)) return s;
+  return addsub_impl(r, nullptr, nullptr, nullptr, batch, op == YH_SUB ? 1 : 0, yhk::AddsubLists{d_a, d_b, d_out});
+}
+
+int yh_yashe_mul_keyswitch_list(yh_ring* r, const yh_evk* k, const uint64_t* const* d_c1,
+                                const uint64_t* const* d_c2, uint64_t* const* d_out, int batch) {
+  if (!r || !k || !d_c1 || !d_c2 || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (k->ring != r) return fail(YH_ERR_ARG, "key belongs to another ring");
+  if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
+  if (int s = set_dev(r->ctx)) return s;
+  size_t need = (size_t)batch * r->rd.LQ * r->rd.n * sizeof(uint64_t);
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_tmp, &r->tmp_bytes, need);
+  if (st) return st;
+  st = ring_product(r, nullptr, nullptr, r->d_tmp, batch, yhk::EPI_SCALE, d_c1, d_c2, nullptr);
+  if (st) return st;
+  return keyswitch_impl(r, k, r->d_tmp, nullptr, batch, d_out);
+}
+
+int yh_yashe_decrypt_list(yh_ring* r, const uint64_t* d_f, const uint64_t* const* d_c, uint64_t* const* d_m,
+                          int batch) {
+  if (!r || !d_f || !d_c || !d_m || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
+  if (int s = set_dev(r->ctx)) return s;
+  return decrypt_impl(r, d_f, nullptr, nullptr, batch, d_c, d_m);
+}
+
+// ---------------------------------------------------------------- streams and events
+int yh_ring_use_own_stream(yh_ring* r) {
+  if (!r) return fail(YH_ERR_ARG, "null ring");
+  if (r->own_stream) return YH_OK;
+  if (r->ctx->capturing) return fail(YH_ERR_ARG, "cannot change streams during graph capture");
+  if (int s = set_dev(r->ctx)) return s;
+  YH_HIP(hipStreamSynchronize(r->stream));
+  hipStream_t st;
+  YH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  r->stream = st;
+  r->own_stream = true;
+  return YH_OK;
+}
+
+int yh_ring_sync(yh_ring* r) {
+  if (!r) return fail(YH_ERR_ARG, "null ring");
+  if (r->ctx->capturing && !r->own_stream) return fail(YH_ERR_ARG, "synchronize during graph capture");
+  if (int s = set_dev(r->ctx)) return s;
+  YH_HIP(hipStreamSynchronize(r->stream));
+  return YH_OK;
+}
+
+int yh_event_create(yh_ctx* ctx, yh_event** out) {
+  if (!ctx || !out) return fail(YH_ERR_ARG, "null argument");
+  if (int s = set_dev(ctx)) return s;
+  auto e = std::make_unique<yh_event>();
+  e->device = ctx->device;
+  YH_HIP(hipEventCreate(&e->ev));
+  *out = e.release();
+  return YH_OK;
+}
+
+int yh_event_record(yh_event* e, yh_ctx* ctx, yh_ring* ring) {
+  if (!e || (!ctx && !ring)) return fail(YH_ERR_ARG, "null argument");
+  YH_HIP(hipSetDevice(e->device));
+  YH_HIP(hipEventRecord(e->ev, ring ? ring->stream : ctx->stream));
+  return YH_OK;
+}
+
+int yh_event_query(yh_event* e) {
+  if (!e) return fail(YH_ERR_ARG, "null event");
+  hipError_t r = hipEventQuery(e->ev);
+  if (r == hipSuccess) return 1;
+  if (r == hipErrorNotReady) return 0;
+  (void)hipGetLastError();
+  return fail(YH_ERR_HIP, hipGetErrorString(r));
+}
+
+int yh_event_sync(yh_event* e) {
+  if (!e) return fail(YH_ERR_ARG, "null event");
+  YH_HIP(hipEventSynchronize(e->ev));
+  return YH_OK;
+}
+
+int yh_event_elapsed_ms(yh_event* start, yh_event* stop, float* ms) {
+  if (!start || !stop || !ms) return fail(YH_ERR_ARG, "null argument");
+  YH_HIP(hipEventElapsedTime(ms, start->ev, stop->ev));
+  return YH_OK;
+}
+
+int yh_event_destroy(yh_event* e) {
+  if (!e) return YH_OK;
+  (void)hipSetDevice(e->device);
+  (void)hipEventDestroy(e->ev);
+  delete e;
+  return YH_OK;
 }
 
 int yh_ring_mul_bcast(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch) {
   if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (int s = set_dev(r->ctx)) return s;
   const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
   if (st) return st;
   yhk::ProdArgs pa{};
   pa.A = d_a; pa.B = d_b;
@@ -853,11 +984,16 @@ int yh_ring_mul_bcast(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint
 static int sub_ring(yh_ring* r, int m, yh_ring** out) {
   if (m == r->rd.n) { *out = r; return YH_OK; }
   auto it = r->sub.find(m);
-  if (it != r->sub.end()) { *out = it->second; return YH_OK; }
+  if (it != r->sub.end()) {
+    it->second->stream = r->stream;  // same stream as the parent's work
+    *out = it->second;
+    return YH_OK;
+  }
   std::vector<uint64_t> ql(r->P->q_limbs);
   yh_ring* s = nullptr;
   int st = yh_ring_create(r->ctx, m, ql.data(), (int)ql.size(), 0, 0, &s);
   if (st) return st;
+  s->stream = r->stream;
   r->sub[m] = s;
   *out = s;
   return YH_OK;
@@ -866,7 +1002,7 @@ static int sub_ring(yh_ring* r, int m, yh_ring** out) {
 extern "C++" {
 template <int LQ>
 static void launch_negate_odd(yh_ring* r, const uint64_t* F, uint64_t* G, int m) {
-  hipLaunchKernelGGL(yhk::k_negate_odd<LQ>, dim3((m + 255) / 256), dim3(256), 0, r->ctx->stream, r->rd, F, G, m);
+  hipLaunchKernelGGL(yhk::k_negate_odd<LQ>, dim3((m + 255) / 256), dim3(256), 0, r->stream, r->rd, F, G, m);
 }
 }
 
@@ -887,7 +1023,7 @@ int yh_ring_inverse(yh_ring* r, const uint64_t* d_f, uint64_t* d_out) {
   uint64_t* Gs = pool + 1 * (elt / 8);  // G_l for every level, packed (total < 2 elt)
   uint64_t* T = pool + 3 * (elt / 8);   // product scratch
   uint64_t* U = pool + 4 * (elt / 8);   // inverse of the level below, spread
-  YH_HIP(hipMemcpyAsync(F, d_f, elt, hipMemcpyDeviceToDevice, r->ctx->stream));
+  YH_HIP(hipMemcpyAsync(F, d_f, elt, hipMemcpyDeviceToDevice, r->stream));
   std::vector<size_t> goff;
   size_t off = 0;
   for (int m : sizes) {
@@ -908,13 +1044,13 @@ int yh_ring_inverse(yh_ring* r, const uint64_t* d_f, uint64_t* d_out) {
     }
     YH_HIP(hipGetLastError());
     if (int st = yh_ring_mul(s, F, G, T, 1)) return st;  // N(F)(x^2)
-    hipLaunchKernelGGL(yhk::k_take_even, dim3((m / 2 + 255) / 256), dim3(256), 0, r->ctx->stream, T, F, m, LQ);
+    hipLaunchKernelGGL(yhk::k_take_even, dim3((m / 2 + 255) / 256), dim3(256), 0, r->stream, T, F, m, LQ);
     YH_HIP(hipGetLastError());
   }
   // F now holds the scalar N(...)(y) in Z_q[y]/(y + 1): invert it on the host
   std::vector<uint64_t> h(LQ);
-  YH_HIP(hipMemcpyAsync(h.data(), F, LQ * sizeof(uint64_t), hipMemcpyDeviceToHost, r->ctx->stream));
-  YH_HIP(hipStreamSynchronize(r->ctx->stream));
+  YH_HIP(hipMemcpyAsync(h.data(), F, LQ * sizeof(uint64_t), hipMemcpyDeviceToHost, r->stream));
+  YH_HIP(hipStreamSynchronize(r->stream));
   yh::BigInt c0 = yh::BigInt::from_limbs(h.data(), LQ);
   if (c0.is_zero()) return fail(YH_ERR_ARG, "not invertible");
   yh::BigInt ci;
@@ -924,18 +1060,18 @@ int yh_ring_inverse(yh_ring* r, const uint64_t* d_f, uint64_t* d_out) {
     return fail(YH_ERR_ARG, "not invertible");
   }
   ci.to_limbs(h.data(), LQ);
-  YH_HIP(hipMemcpyAsync(F, h.data(), LQ * sizeof(uint64_t), hipMemcpyHostToDevice, r->ctx->stream));
+  YH_HIP(hipMemcpyAsync(F, h.data(), LQ * sizeof(uint64_t), hipMemcpyHostToDevice, r->stream));
   // back up: Inv_l = G_l * Inv_{l+1}(x^2)
   for (int k = (int)sizes.size() - 1; k >= 0; --k) {
     const int m = sizes[k];
     yh_ring* s = nullptr;
     if (int st = sub_ring(r, m, &s)) return st;
-    hipLaunchKernelGGL(yhk::k_spread_even, dim3((m + 255) / 256), dim3(256), 0, r->ctx->stream, F, U, m, LQ);
+    hipLaunchKernelGGL(yhk::k_spread_even, dim3((m + 255) / 256), dim3(256), 0, r->stream, F, U, m, LQ);
     YH_HIP(hipGetLastError());
     uint64_t* dst = k == 0 ? d_out : F;
     if (int st = yh_ring_mul(s, Gs + goff[k], U, dst, 1)) return st;
   }
-  YH_HIP(hipStreamSynchronize(r->ctx->stream));
+  YH_HIP(hipStreamSynchronize(r->stream));
   return YH_OK;
   YH_TRY_END
 }
@@ -1224,7 +1360,7 @@ int yh_batch_run(yh_batch* b) {
   hipLaunchKernelGGL(yhk::k_batch_reset, dim3(std::max(1, std::min(64, (b->ninst + 255) / 256))), dim3(256), 0, st,
                      b->args.head, b->args.nqueues, b->args.done, b->ninst);
   YH_HIP(hipGetLastError());
-  LaunchTimer timer(b->ctx, YH_KERNEL_BATCH);
+  LaunchTimer timer(b->ctx, YH_KERNEL_BATCH, b->ctx->stream);
   const dim3 grid(b->grid), block(b->W);
   const size_t lds = yhk::batch_lds_bytes(b->W);
   if (b->maxlq > 4)

@@ -197,6 +197,40 @@ int yh_ring_mul_bcast(yh_ring* ring, const uint64_t* d_a, const uint64_t* d_b, u
  * on the host) for key generation (src/yashe.cpp:37-55). */
 int yh_ring_inverse(yh_ring* ring, const uint64_t* d_f, uint64_t* d_out);
 
+/* ------------------------------------------------------------ pointer lists
+ * The same operations over ciphertexts scattered in device memory: d_a,
+ * d_b, d_c1, d_c2, d_c, d_out and d_m are DEVICE arrays of `batch` device
+ * pointers (each to one [L][n] ring element, or one [n] message for d_m),
+ * so a serving loop batches resident ciphertexts without gathering them.
+ * Results equal the contiguous variants element by element. The reference
+ * runs each operation on one Polynomial/Ciphertext object at a time
+ * (src/ciphertext.cpp:38-107, src/yashe.cpp:167-197). */
+int yh_ring_addsub_list(yh_ring* ring, const uint64_t* const* d_a, const uint64_t* const* d_b, uint64_t* const* d_out,
+                        int batch, int op);
+int yh_yashe_mul_keyswitch_list(yh_ring* ring, const yh_evk* evk, const uint64_t* const* d_c1,
+                                const uint64_t* const* d_c2, uint64_t* const* d_out, int batch);
+int yh_yashe_decrypt_list(yh_ring* ring, const uint64_t* d_f, const uint64_t* const* d_c, uint64_t* const* d_m,
+                          int batch);
+
+/* ------------------------------------------------------------ streams and events
+ * A ring's operations run on the context stream unless the ring is given a
+ * stream of its own (default priority), after which work on different rings
+ * overlaps on the device. Such a ring is outside yh_graph_begin/end captures
+ * and yh_ctx_sync; wait for it with yh_ring_sync or an event. Events mark a
+ * point in a ring's (or, with ring NULL, the context's) stream. No
+ * counterpart in the reference (every call there ends in
+ * cudaDeviceSynchronize). */
+int yh_ring_use_own_stream(yh_ring* ring);
+int yh_ring_sync(yh_ring* ring);
+typedef struct yh_event yh_event;
+int yh_event_create(yh_ctx* ctx, yh_event** out);
+int yh_event_record(yh_event* ev, yh_ctx* ctx, yh_ring* ring);
+/* 1 when the work before the event has finished, 0 while it runs, < 0 on error */
+int yh_event_query(yh_event* ev);
+int yh_event_sync(yh_event* ev);
+int yh_event_elapsed_ms(yh_event* start, yh_event* stop, float* ms);
+int yh_event_destroy(yh_event* ev);
+
 /* ------------------------------------------------------------ graphs
  * Captures the launches issued on the context stream into a HIP graph
  * (hipStreamBeginCapture), for replay of a fixed homomorphic circuit. Run

@@ -1,0 +1,56 @@
+"""The continuous-batching serving loop on the GPU: every decryption checked,
+admission by HBM footprint, results identical alone and in the mix."""
+import copy
+
+import pytest
+
+import cuyashe_amd as ya
+from cuyashe_amd import workloads
+from cuyashe_amd.serving import Request, Server, poisson_requests
+
+pytestmark = pytest.mark.gpu
+
+RINGS = [(2048, workloads.Q127), (4096, workloads.Q255), (2048, workloads.Q511)]
+
+
+@pytest.fixture(scope="module")
+def server(gpu_ctx):
+    return Server(gpu_ctx, RINGS, t=1024, pool=16, slots=48, max_batch=32, seed=5, keep_outputs=True)
+
+
+def test_serving_mix_all_decrypts(server):
+    reqs = poisson_requests(400, 20000.0, RINGS, 1024, 16, seed=1)
+    st = server.run(reqs)
+    assert st["all_decrypts_ok"], [r.rid for r in reqs if not r.ok][:10]
+    assert all(r.done is not None and r.latency > 0 for r in reqs)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    assert st["mean_batch"] > 1.0  # arrivals during earlier work were batched
+
+
+def test_serving_alone_equals_mix(server):
+    reqs = poisson_requests(150, 50000.0, RINGS, 1024, 16, seed=2)
+    st = server.run(reqs)
+    assert st["all_decrypts_ok"]
+    picked = [r for r in reqs if r.kind == "mulks"][:6] + [r for r in reqs if r.kind == "add"][:4] + \
+        [r for r in reqs if r.kind == "encdec"][:4]
+    for r in picked:
+        alone = copy.copy(r)
+        alone.arrival, alone.done, alone.ok, alone.output = 0.0, None, None, None
+        server.run([alone])
+        assert alone.ok
+        assert (alone.output == r.output).all(), (r.rid, r.kind)
+
+
+def test_serving_admission_budget(gpu_ctx):
+    one = Server(gpu_ctx, RINGS[:1], t=1024, pool=8, slots=16, max_batch=16, seed=9)
+    fp = max(one.footprint(Request(0, k, RINGS[0])) for k in ("add", "mulks", "encdec"))
+    one.budget = 3 * fp
+    reqs = poisson_requests(60, 1e6, RINGS[:1], 1024, 8, seed=4)  # a burst
+    st = one.run(reqs)
+    assert st["all_decrypts_ok"]
+    assert st["peak_reserved_bytes"] <= 3 * fp
+    assert st["groups"] >= 12  # only a few requests in flight at a time
+    big = max(("add", "mulks", "encdec"), key=lambda k: one.footprint(Request(0, k, RINGS[0])))
+    one.budget = fp - 1
+    with pytest.raises(ValueError):
+        one.run([Request(0, big, RINGS[0], a=0, b=1, msg=reqs[0].msg)])
