@@ -187,8 +187,8 @@ int yh_yashe_mul_keyswitch(yh_ring* ring, const yh_evk* evk, const uint64_t* d_c
  * decodes coefficient 0 only. */
 int yh_yashe_decrypt(yh_ring* ring, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch);
 
-/* c_b = a * b_b mod (x^n + 1, q) for one a and a batch of b (encryption's
- * h*s over a batch). */
+/* c_b = a * b_b mod (x^n + 1, q) for one a and a batch of b: the h*s of
+ * Yashe::encrypt (src/yashe.cpp:104-142) for many messages under one key. */
 int yh_ring_mul_bcast(yh_ring* ring, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch);
 /* f^-1 mod (x^n + 1, q) on the device, by the norm recursion
  * f^-1(x) = f(-x) * N(f)^-1(x^2), N(f)(x^2) = f(x) f(-x): log2(n) ring
