@@ -212,7 +212,9 @@ void set_lds_attrs() {
   const int bytes = (int)lds_bytes(1 << LOGN);
   (void)hipFuncSetAttribute((const void*)yhk::k_prod<LOGN, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_prod<LOGN, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_keyswitch<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void*)yhk::k_keyswitch<LOGN, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)ks_lds_bytes<LOGN>());
+  (void)hipFuncSetAttribute((const void*)yhk::k_keyswitch<LOGN, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)ks_lds_bytes<LOGN>());
   (void)hipFuncSetAttribute((const void*)yhk::k_evk_prep<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ntt_cyclic<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -360,7 +362,13 @@ int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out
   LaunchTimer* timer = new LaunchTimer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
   if (n >= 16) {
     switch (r->rd.logn) {
-#define X(L) case L: hipLaunchKernelGGL(yhk::k_keyswitch<L>, grid_x, dim3(n / 16), ks_lds_bytes<L>(), r->stream, r->rd, ka); break;
+#define X(L)                                                                                                  \
+  case L:                                                                                                       \
+    if (r->rd.w_bits == 32)                                                                                     \
+      hipLaunchKernelGGL((yhk::k_keyswitch<L, true>), grid_x, dim3(n / 16), ks_lds_bytes<L>(), r->stream, r->rd, ka); \
+    else                                                                                                        \
+      hipLaunchKernelGGL((yhk::k_keyswitch<L, false>), grid_x, dim3(n / 16), ks_lds_bytes<L>(), r->stream, r->rd, ka); \
+    break;
       YH_LOGN_CASES(X)
 #undef X
       default: delete timer; return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
