@@ -29,19 +29,24 @@ namespace yhk {
 constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stages)
 
 // Minimum waves per SIMD requested from the compiler for the transform
-// kernels. 2 caps them at 256 VGPRs with no spills; 4 (128 VGPRs) spills
-// ~600 B/lane and measured 20% slower at n = 4096.
+// kernels (2 caps them at 256 VGPRs).
+//
+// Register pressure: every address a transform computes from threadIdx.x
+// (LDS slots, table and operand offsets) is invariant across the transforms
+// of one thread, and the compiler keeps them all live (over 100 VGPRs). The
+// kernels therefore hand each transform a copy of the thread index that an
+// empty asm makes opaque (`opaque_tid`): the addresses are recomputed (a few
+// cheap instructions) and the registers are free for more waves.
 #ifndef YH_MIN_WAVES
 #define YH_MIN_WAVES 2
 #endif
-// The fused product kernel fits three waves per SIMD without spills (its
-// second operand re-reads the tables instead of keeping them live) and ran
-// 14% faster there than at two; the keyswitch stays at two (three spills).
+// The fused product kernel fits three waves per SIMD without spills (140
+// VGPRs at n = 4096, 116 at n >= 8192); the keyswitch fits four (126).
 #ifndef YH_PROD_WAVES
 #define YH_PROD_WAVES 3
 #endif
 #ifndef YH_KS_WAVES
-#define YH_KS_WAVES YH_MIN_WAVES
+#define YH_KS_WAVES 4
 #endif
 
 // Device view of one ring's tables (all pointers are device memory).
@@ -61,6 +66,14 @@ struct RingDev {
   const uint64_t* invMi[2]; // [R]
   const uint64_t* M[2];     // [LW]
 };
+
+// threadIdx.x, recomputed after `dep` is ready (see the register note above).
+template <typename T>
+__device__ __forceinline__ int opaque_tid(const T& dep) {
+  int g = threadIdx.x;
+  asm volatile("" : "+v"(g) : "v"(dep));
+  return g;
+}
 
 // ---------------------------------------------------------------- workgroup mapping
 // MI355X deals consecutive workgroup ids round-robin over its 8 XCDs, each
@@ -269,8 +282,8 @@ struct ProdArgs {
 // centred, and for negacyclic transforms twisted by psi^j.
 template <int LOGN, bool NEGA>
 __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
-                                         const uint64_t* pw, uint64_t p, uint64_t b, const uint64_t* twist) {
-  const int g = threadIdx.x;
+                                         const uint64_t* pw, uint64_t p, uint64_t b, const uint64_t* twist,
+                                         int g = threadIdx.x) {
   constexpr int stride = 1 << (LOGN - 4);
 #pragma unroll
   for (int c = 0; c < E; ++c) {
@@ -292,27 +305,30 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   uint64_t xa[E], xb[E];
   const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
   const uint64_t* Bp = pa.b_list ? pa.b_list[pair] : pa.B + pair * pa.ostride_b;
-  load_crt<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, p, b, pa.tab.twist);
-  ntt16_fwd<LOGN>(xa, lds, pa.tab.fwd);
+  const int g1 = opaque_tid(pair);
+  load_crt<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, p, b, pa.tab.twist, g1);
+  ntt16_fwd<LOGN>(xa, lds, pa.tab.fwd, g1);
   // The second operand re-reads the tables (L1 hits) rather than keeping the
   // first transform's twiddles live in registers (the empty asm hides that
   // the addresses repeat); fewer registers, more waves per SIMD.
   const uint64_t* twist2 = pa.tab.twist;
   const uint64_t* fwd2 = pa.tab.fwd;
   asm volatile("" : "+s"(twist2), "+s"(fwd2));
-  load_crt<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, p, b, twist2);
+  const int g2 = opaque_tid(xa[E - 1]);
+  load_crt<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, p, b, twist2, g2);
   __syncthreads();  // every thread has left the LDS of the first transform
-  ntt16_fwd<LOGN>(xb, lds, fwd2);
+  ntt16_fwd<LOGN>(xb, lds, fwd2, g2);
 #pragma unroll
   for (int c = 0; c < E; ++c) xa[c] = gl::mul(xa[c], xb[c]);
   __syncthreads();
-  ntt16_inv<LOGN>(xa, lds, pa.tab.inv);
+  const int g3 = opaque_tid(xa[0]);
+  ntt16_inv<LOGN>(xa, lds, pa.tab.inv, g3);
   const uint64_t invM = rd.invMi[pa.set][i];
   uint32_t* y = pa.Y + ((size_t)pair * rd.set_R[pa.set] + i) * N;
   constexpr int stride = N >> 4;
 #pragma unroll
   for (int c = 0; c < E; ++c) {
-    const int j = threadIdx.x + c * stride;
+    const int j = g3 + c * stride;
     const uint64_t v = NEGA ? gl::mul(xa[c], pa.tab.untwist[j]) : gl::mul(xa[c], pa.tab.n_inv);
     y[j] = finish_residue(v, 1, p, b, invM);
   }
@@ -375,77 +391,53 @@ __device__ __forceinline__ uint64_t digit_of(const uint64_t* a, int n, int lq, i
   return v & ((1ull << wb) - 1);
 }
 
-// The multiply-accumulate sum lives in LDS (behind the transform buffer) for
-// n <= 8192, which keeps the kernel free of register spills; n = 16384 has no
-// LDS left for it and keeps it in registers.
-template <int LOGN>
-__host__ __device__ constexpr bool ks_acc_in_lds() { return LOGN <= 13; }
-
 // W32: digits are the 32-bit words of the limbs (w = 2^32), read directly.
+// The multiply-accumulate sum stays in registers (126 VGPRs with the opaque
+// thread index, four waves per SIMD, no spills at any n).
 template <int LOGN, bool W32>
 __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_keyswitch(RingDev rd, KsArgs ka) {
   extern __shared__ uint64_t lds[];
   constexpr int N = 1 << LOGN;
   constexpr int stride = N >> 4;
-  constexpr bool ACC_LDS = ks_acc_in_lds<LOGN>();
   const int Rks = rd.set_R[1];
   int ct, i;
   if (!xcd_map(blockIdx.x, Rks, ka.batch, ct, i)) return;
   const uint64_t p = rd.primes[i];
   const uint64_t b = rd.barrett[i];
   const uint64_t* c0 = ka.C + ct * ka.ostride;
-  // accumulator slot c of thread g at acc_lds[c*stride + g]: consecutive
-  // threads touch consecutive words
-  uint64_t* acc_lds = lds + N + (N >> 4);
-  uint64_t acc[ACC_LDS ? 1 : E], x[E];
-  if (!ACC_LDS) {
+  uint64_t acc[E], x[E];
 #pragma unroll
-    for (int c = 0; c < E; ++c) acc[c] = 0;
-  }
+  for (int c = 0; c < E; ++c) acc[c] = 0;
   for (int k = 0; k < rd.lw; ++k) {
-    // Re-read the tables each digit instead of keeping ~90 loop-invariant
-    // table registers live across the loop (the empty asm hides the
-    // invariance): the loads hit L1/L2 and the kernel fits more waves.
+    // Re-read the tables for each digit rather than keeping loop-invariant
+    // table values live across the loop (the empty asm hides the invariance).
     const uint64_t* twist = rd.tab.twist;
     const uint64_t* fwd = rd.tab.fwd;
     asm volatile("" : "+s"(twist), "+s"(fwd));
+    const int g = opaque_tid(k);
     const uint32_t* w32 = reinterpret_cast<const uint32_t*>(c0 + (size_t)(k >> 1) * N) + (k & 1);
 #pragma unroll
     for (int c = 0; c < E; ++c) {
-      const int j = threadIdx.x + c * stride;
+      const int j = g + c * stride;
       const uint64_t d = W32 ? (uint64_t)w32[2 * j] : digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
       const uint64_t r = mod_p(d, p, b);
       x[c] = twist_centered(r, p, twist[j]);
     }
     if (k > 0) __syncthreads();  // previous transform's last LDS reads are done
-    ntt16_fwd<LOGN>(x, lds, fwd);
-    const uint64_t* ev = ka.evk + ((size_t)k * Rks + i) * N + threadIdx.x * E;
+    ntt16_fwd<LOGN>(x, lds, fwd, g);
+    const uint64_t* ev = ka.evk + ((size_t)k * Rks + i) * N + g * E;
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      const uint64_t prod = gl::mul(x[c], ev[c]);
-      if (ACC_LDS) {
-        uint64_t& a = acc_lds[c * stride + threadIdx.x];
-        a = k == 0 ? prod : gl::add(a, prod);
-      } else {
-        acc[ACC_LDS ? 0 : c] = gl::add(acc[ACC_LDS ? 0 : c], prod);
-      }
-    }
+    for (int c = 0; c < E; ++c) acc[c] = gl::add(acc[c], gl::mul(x[c], ev[c]));
   }
   __syncthreads();
-  if (ACC_LDS) {
-#pragma unroll
-    for (int c = 0; c < E; ++c) x[c] = acc_lds[c * stride + threadIdx.x];
-  } else {
-#pragma unroll
-    for (int c = 0; c < E; ++c) x[c] = acc[ACC_LDS ? 0 : c];
-  }
-  ntt16_inv<LOGN>(x, lds, rd.tab.inv);
+  const int g = opaque_tid(acc[0]);
+  ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g);
   const uint64_t invM = rd.invMi[1][i];
   uint32_t* y = ka.Y + ((size_t)ct * Rks + i) * N;
 #pragma unroll
   for (int c = 0; c < E; ++c) {
-    const int j = threadIdx.x + c * stride;
-    y[j] = finish_residue(gl::mul(x[c], rd.tab.untwist[j]), 1, p, b, invM);
+    const int j = g + c * stride;
+    y[j] = finish_residue(gl::mul(acc[c], rd.tab.untwist[j]), 1, p, b, invM);
   }
 }
 

@@ -201,7 +201,7 @@ size_t lds_bytes(int N) { return (size_t)(N + N / 16) * sizeof(uint64_t); }
 
 template <int LOGN>
 size_t ks_lds_bytes() {
-  return lds_bytes(1 << LOGN) + (yhk::ks_acc_in_lds<LOGN>() ? ((size_t)1 << LOGN) * sizeof(uint64_t) : 0);
+  return lds_bytes(1 << LOGN);
 }
 
 // Every transform kernel is instantiated per log2(N) in [4, 14].
