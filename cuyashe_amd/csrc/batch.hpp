@@ -27,7 +27,7 @@ struct BatchInst {
   const uint64_t* A;  // [LQ][n], coefficients in [0, q) (any value below 2^(64 LQ) is reduced)
   const uint64_t* B;
   uint64_t* out;      // [LQ][n]
-  uint32_t* Y;        // [R][n] residue scratch
+  uint32_t* Y;        // [R + 1][n] residue scratch (layout of store_y)
   int ring;           // index into BatchArgs::rings
   int pad;
 };
@@ -100,16 +100,19 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   const NttTab tab = rd.tab;
   uint64_t* my = lds + (size_t)sub * (N + T);
   uint64_t xa[E], xb[E];
+  const bool gold = i == 0;  // modulus 0 is P itself (exact residues, not centred)
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g + c * T;
-    xa[c] = twist_centered(crt_residue(in.A + j, LQ, N, pw, p, b), p, tab.twist[j]);
+    xa[c] = gold ? gl::mul(crt_residue_P(in.A + j, LQ, N), tab.twist[j])
+                 : twist_centered(crt_residue(in.A + j, LQ, N, pw, p, b), p, tab.twist[j]);
   }
   ntt16_fwd_rt(xa, my, tab.fwd, g, logn, wave);
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g + c * T;
-    xb[c] = twist_centered(crt_residue(in.B + j, LQ, N, pw, p, b), p, tab.twist[j]);
+    xb[c] = gold ? gl::mul(crt_residue_P(in.B + j, LQ, N), tab.twist[j])
+                 : twist_centered(crt_residue(in.B + j, LQ, N, pw, p, b), p, tab.twist[j]);
   }
   xsync(wave);
   ntt16_fwd_rt(xb, my, tab.fwd, g, logn, wave);
@@ -119,11 +122,10 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   ntt16_inv_rt(xa, my, tab.inv, g, logn, wave);
   if (active) {
     const uint64_t invM = rd.invMi[0][i];
-    uint32_t* y = in.Y + (size_t)i * N;
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int j = g + c * T;
-      y[j] = finish_residue(gl::mul(xa[c], tab.untwist[j]), 1, p, b, invM);
+      store_y(in.Y, N, i, j, finish_any(gold, gl::mul(xa[c], tab.untwist[j]), p, b, invM));
     }
   }
   // Publish the residues: every wave waits until its stores have reached L2

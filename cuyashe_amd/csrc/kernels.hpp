@@ -129,6 +129,37 @@ __device__ __forceinline__ uint64_t crt_residue(const uint64_t* a, int lin, size
   return mod_p(acc, p, b);
 }
 
+// ---------------------------------------------------------------- modulus 0 = P
+// The first CRT modulus is the Goldilocks prime itself: the transform gives
+// the convolution mod P exactly for any operand size, so its residues are
+// canonical values in [0, P) (not centred) and its y is 64 bits wide.
+
+// Residue mod P of a coefficient of `lin` 64-bit limbs (stride ls): Horner in
+// 2^64 = EPS (mod P).
+__device__ __forceinline__ uint64_t crt_residue_P(const uint64_t* a, int lin, size_t ls) {
+  uint64_t r = 0;
+  for (int l = lin - 1; l >= 0; --l) {
+    uint64_t v = a[(size_t)l * ls];
+    if (v >= gl::P) v -= gl::P;
+    r = gl::add(gl::mul(r, gl::EPS), v);
+  }
+  return r;
+}
+
+// Rows of the reconstruction scratch per operand: modulus 0 keeps its 64-bit
+// y in rows 0 (low half) and 1 (high half), modulus i >= 1 in row i + 1.
+__host__ __device__ constexpr int y_rows(int R) { return R + 1; }
+
+// Stores y of modulus i at coefficient j; Yp is the operand's first row.
+__device__ __forceinline__ void store_y(uint32_t* Yp, size_t N, int i, int j, uint64_t y) {
+  if (i == 0) {
+    Yp[j] = (uint32_t)y;
+    Yp[N + j] = (uint32_t)(y >> 32);
+  } else {
+    Yp[(size_t)(i + 1) * N + j] = (uint32_t)y;
+  }
+}
+
 // ---------------------------------------------------------------- NTT passes
 //
 // Transform of length N = 2^logN held by N/16 threads. A pass covers up to four
@@ -293,6 +324,26 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
   }
 }
 
+// Modulus 0 (P): exact residues mod P, twisted by psi^j, not centred.
+template <int LOGN, bool NEGA>
+__device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
+                                           const uint64_t* twist, int g) {
+  constexpr int stride = 1 << (LOGN - 4);
+#pragma unroll
+  for (int c = 0; c < E; ++c) {
+    const int j = g + c * stride;
+    const uint64_t r = j < ncoef ? crt_residue_P(a + j, lin, ls) : 0;
+    x[c] = NEGA ? gl::mul(r, twist[j]) : r;
+  }
+}
+
+// y of one residue after the inverse transform (x already scaled/untwisted):
+// modulus 0 keeps the exact residue mod P, the others reduce the signed
+// convolution value mod p_i; both are multiplied by (M/m_i)^-1 mod m_i.
+__device__ __forceinline__ uint64_t finish_any(bool gold, uint64_t v, uint64_t p, uint64_t b, uint64_t invM) {
+  return gold ? gl::mul(v, invM) : (uint64_t)finish_residue(v, 1, p, b, invM);
+}
+
 template <int LOGN, bool NEGA>
 __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDev rd, ProdArgs pa) {
   extern __shared__ uint64_t lds[];
@@ -305,8 +356,10 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   uint64_t xa[E], xb[E];
   const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
   const uint64_t* Bp = pa.b_list ? pa.b_list[pair] : pa.B + pair * pa.ostride_b;
+  const bool gold = i == 0;
   const int g1 = opaque_tid(pair);
-  load_crt<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, p, b, pa.tab.twist, g1);
+  if (gold) load_crt_P<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.tab.twist, g1);
+  else load_crt<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, p, b, pa.tab.twist, g1);
   ntt16_fwd<LOGN>(xa, lds, pa.tab.fwd, g1);
   // The second operand re-reads the tables (L1 hits) rather than keeping the
   // first transform's twiddles live in registers (the empty asm hides that
@@ -315,7 +368,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   const uint64_t* fwd2 = pa.tab.fwd;
   asm volatile("" : "+s"(twist2), "+s"(fwd2));
   const int g2 = opaque_tid(xa[E - 1]);
-  load_crt<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, p, b, twist2, g2);
+  if (gold) load_crt_P<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
+  else load_crt<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, p, b, twist2, g2);
   __syncthreads();  // every thread has left the LDS of the first transform
   ntt16_fwd<LOGN>(xb, lds, fwd2, g2);
 #pragma unroll
@@ -324,13 +378,13 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   const int g3 = opaque_tid(xa[0]);
   ntt16_inv<LOGN>(xa, lds, pa.tab.inv, g3);
   const uint64_t invM = rd.invMi[pa.set][i];
-  uint32_t* y = pa.Y + ((size_t)pair * rd.set_R[pa.set] + i) * N;
+  uint32_t* y = pa.Y + (size_t)pair * y_rows(rd.set_R[pa.set]) * N;
   constexpr int stride = N >> 4;
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g3 + c * stride;
     const uint64_t v = NEGA ? gl::mul(xa[c], pa.tab.untwist[j]) : gl::mul(xa[c], pa.tab.n_inv);
-    y[j] = finish_residue(v, 1, p, b, invM);
+    store_y(y, N, i, j, finish_any(gold, v, p, b, invM));
   }
 }
 
@@ -347,11 +401,17 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
   uint64_t ra[64], rb[64], rc[64];
   const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
   const uint64_t* Bp = pa.b_list ? pa.b_list[pair] : pa.B + pair * pa.ostride_b;
+  const bool gold = i == 0;
   for (int j = 0; j < N; ++j) {
     const uint64_t* a = A + j;
     const uint64_t* bb = Bp + j;
-    ra[j] = j < pa.ncoef_a ? center(crt_residue(a, pa.lin_a, pa.lstride_a, pw, p, b), p) : 0;
-    rb[j] = j < pa.ncoef_b ? center(crt_residue(bb, pa.lin_b, pa.lstride_b, pw, p, b), p) : 0;
+    if (gold) {
+      ra[j] = j < pa.ncoef_a ? crt_residue_P(a, pa.lin_a, pa.lstride_a) : 0;
+      rb[j] = j < pa.ncoef_b ? crt_residue_P(bb, pa.lin_b, pa.lstride_b) : 0;
+    } else {
+      ra[j] = j < pa.ncoef_a ? center(crt_residue(a, pa.lin_a, pa.lstride_a, pw, p, b), p) : 0;
+      rb[j] = j < pa.ncoef_b ? center(crt_residue(bb, pa.lin_b, pa.lstride_b, pw, p, b), p) : 0;
+    }
     rc[j] = 0;
   }
   for (int u = 0; u < N; ++u)
@@ -365,8 +425,8 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
       rc[k] = gl::add(rc[k], prod);
     }
   const uint64_t invM = rd.invMi[pa.set][i];
-  uint32_t* y = pa.Y + ((size_t)pair * rd.set_R[pa.set] + i) * N;
-  for (int j = 0; j < N; ++j) y[j] = finish_residue(rc[j], 1, p, b, invM);
+  uint32_t* y = pa.Y + (size_t)pair * y_rows(rd.set_R[pa.set]) * N;
+  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(gold, rc[j], p, b, invM));
 }
 
 // ---------------------------------------------------------------- keyswitch
@@ -405,6 +465,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_keyswitch(Rin
   const uint64_t p = rd.primes[i];
   const uint64_t b = rd.barrett[i];
   const uint64_t* c0 = ka.C + ct * ka.ostride;
+  const bool gold = i == 0;
   uint64_t acc[E], x[E];
 #pragma unroll
   for (int c = 0; c < E; ++c) acc[c] = 0;
@@ -420,8 +481,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_keyswitch(Rin
     for (int c = 0; c < E; ++c) {
       const int j = g + c * stride;
       const uint64_t d = W32 ? (uint64_t)w32[2 * j] : digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
-      const uint64_t r = mod_p(d, p, b);
-      x[c] = twist_centered(r, p, twist[j]);
+      // digits are below 2^32 < P: their residue mod P is the digit itself
+      x[c] = gold ? mul_small(d, twist[j]) : twist_centered(mod_p(d, p, b), p, twist[j]);
     }
     if (k > 0) __syncthreads();  // previous transform's last LDS reads are done
     ntt16_fwd<LOGN>(x, lds, fwd, g);
@@ -433,11 +494,11 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_keyswitch(Rin
   const int g = opaque_tid(acc[0]);
   ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g);
   const uint64_t invM = rd.invMi[1][i];
-  uint32_t* y = ka.Y + ((size_t)ct * Rks + i) * N;
+  uint32_t* y = ka.Y + (size_t)ct * y_rows(Rks) * N;
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g + c * stride;
-    y[j] = finish_residue(gl::mul(acc[c], rd.tab.untwist[j]), 1, p, b, invM);
+    store_y(y, N, i, j, finish_any(gold, gl::mul(acc[c], rd.tab.untwist[j]), p, b, invM));
   }
 }
 
@@ -451,11 +512,15 @@ __global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* gamma_r
   const uint64_t p = rd.primes[i];
   const uint64_t b = rd.barrett[i];
   const uint64_t* c0 = ka.C + ct * ka.ostride;
+  const bool gold = i == 0;
   uint64_t rc[64];
   for (int j = 0; j < N; ++j) rc[j] = 0;
   for (int k = 0; k < rd.lw; ++k) {
     uint64_t d[64];
-    for (int j = 0; j < N; ++j) d[j] = center(mod_p(digit_of(c0 + j, N, rd.LQ, k, rd.w_bits), p, b), p);
+    for (int j = 0; j < N; ++j) {
+      const uint64_t dg = digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
+      d[j] = gold ? dg : center(mod_p(dg, p, b), p);
+    }
     const uint64_t* gk = gamma_res + ((size_t)k * Rks + i) * N;
     for (int u = 0; u < N; ++u)
       for (int v = 0; v < N; ++v) {
@@ -466,8 +531,8 @@ __global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* gamma_r
       }
   }
   const uint64_t invM = rd.invMi[1][i];
-  uint32_t* y = ka.Y + ((size_t)ct * Rks + i) * N;
-  for (int j = 0; j < N; ++j) y[j] = finish_residue(rc[j], 1, p, b, invM);
+  uint32_t* y = ka.Y + (size_t)ct * y_rows(Rks) * N;
+  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(gold, rc[j], p, b, invM));
 }
 
 // Pre-transform of the keyswitch keys: gamma_k (coefficients, [lw][LQ][n]) ->
@@ -485,7 +550,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_evk_prep(Rin
   const uint64_t* g = G + (size_t)k * rd.LQ * N;
   uint64_t* o = out + ((size_t)k * Rks + i) * N;
   uint64_t x[E];
-  load_crt<LOGN, true>(x, g, rd.LQ, N, N, pw, p, b, rd.tab.twist);
+  if (i == 0) load_crt_P<LOGN, true>(x, g, rd.LQ, N, N, rd.tab.twist, threadIdx.x);
+  else load_crt<LOGN, true>(x, g, rd.LQ, N, N, pw, p, b, rd.tab.twist);
   ntt16_fwd<LOGN>(x, lds, rd.tab.fwd);
 #pragma unroll
   for (int c = 0; c < E; ++c) o[threadIdx.x * E + c] = x[c];
@@ -502,7 +568,8 @@ __global__ void k_evk_residues(RingDev rd, const uint64_t* G, uint64_t* out) {
   const int N = rd.n;
   const uint64_t* g = G + (size_t)k * rd.LQ * N;
   uint64_t* o = out + ((size_t)k * Rks + i) * N;
-  for (int j = threadIdx.x; j < N; j += blockDim.x) o[j] = center(crt_residue(g + j, rd.LQ, N, pw, p, b), p);
+  for (int j = threadIdx.x; j < N; j += blockDim.x)
+    o[j] = i == 0 ? crt_residue_P(g + j, rd.LQ, N) : center(crt_residue(g + j, rd.LQ, N, pw, p, b), p);
 }
 
 // ---------------------------------------------------------------- ICRT + epilogues
@@ -512,6 +579,23 @@ enum Epilogue { EPI_RAW = 0, EPI_MODQ = 1, EPI_SCALE = 2, EPI_DECRYPT = 3 };
 template <int NL>
 __device__ __forceinline__ void mac32(uint64_t* acc, const uint64_t* m, uint64_t y) {
   // acc += m * y (mod 2^(64 NL)), y < 2^32
+  uint64_t carry = 0;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    uint64_t lo = m[l] * y;
+    uint64_t hi = gl::umulhi(m[l], y);
+    uint64_t s = acc[l] + lo;
+    hi += (s < lo);
+    uint64_t s2 = s + carry;
+    hi += (s2 < s);
+    acc[l] = s2;
+    carry = hi;
+  }
+}
+
+template <int NL>
+__device__ __forceinline__ void mac64(uint64_t* acc, const uint64_t* m, uint64_t y) {
+  // acc += m * y (mod 2^(64 NL)), y any 64-bit value
   uint64_t carry = 0;
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
@@ -674,8 +758,9 @@ struct IcrtArgs {
   uint64_t* const* out_list;  // optional per-pair output addresses (replace out + pair*out_pstride)
 };
 
-// ICRT of one coefficient plus its epilogue. y points at the residue of
-// prime 0 (rows ystride apart), o at limb 0 of the output coefficient.
+// ICRT of one coefficient plus its epilogue. y points at the operand's first
+// scratch row (rows ystride apart, layout of store_y), o at limb 0 of the
+// output coefficient.
 template <int LQ>
 __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, size_t ystride, int set, int mode,
                                           int positive, uint64_t* o, int out_limbs, size_t out_stride) {
@@ -685,9 +770,14 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
   uint64_t acc[LW];
 #pragma unroll
   for (int l = 0; l < LW; ++l) acc[l] = 0;
-  uint64_t flo = 0, fhi = 0;
-  for (int i = 0; i < RS; ++i) {
-    uint64_t yi = y[(size_t)i * ystride];
+  // modulus 0 = P: 64-bit y (rows 0 and 1); its fixed-point fraction
+  // floor(y 2^64 / P) = y + floor(y EPS / P) is y + umulhi(y, EPS) to within
+  // two units of 2^-64 (far inside the quotient's error budget)
+  const uint64_t yP = (uint64_t)y[0] | ((uint64_t)y[ystride] << 32);
+  uint64_t flo = yP + gl::umulhi(yP, gl::EPS), fhi = 0;
+  mac64<LW>(acc, Mi, yP);
+  for (int i = 1; i < RS; ++i) {
+    uint64_t yi = y[(size_t)(i + 1) * ystride];
     // fixed-point y_i/p_i with 64 fractional bits (y_i < p_i keeps it < 2^64)
     uint64_t f = yi * rd.barrett[i];
     flo += f;
@@ -844,8 +934,8 @@ __global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
   if (j >= ia.ncoef) return;
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
-  icrt_coef<LQ>(rd, ia.Y + (size_t)pair * RS * ia.N + j, ia.N, ia.set, ia.mode, ia.positive, o + j, ia.out_limbs,
-                ia.out_stride);
+  icrt_coef<LQ>(rd, ia.Y + (size_t)pair * y_rows(RS) * ia.N + j, ia.N, ia.set, ia.mode, ia.positive, o + j,
+                ia.out_limbs, ia.out_stride);
 }
 
 // ---------------------------------------------------------------- primitives
@@ -858,7 +948,10 @@ __global__ void k_crt(RingDev rd, const uint64_t* A, int lin, int ncoef, size_t 
   const int i = blockIdx.y;
   if (j >= spacing) return;
   const uint64_t p = rd.primes[i];
-  uint64_t r = j < ncoef ? crt_residue(A + j, lin, lstride, rd.pow32 + (size_t)i * 2 * rd.LW, p, rd.barrett[i]) : 0;
+  uint64_t r = 0;
+  if (j < ncoef)
+    r = i == 0 ? crt_residue_P(A + j, lin, lstride)
+               : crt_residue(A + j, lin, lstride, rd.pow32 + (size_t)i * 2 * rd.LW, p, rd.barrett[i]);
   out[(size_t)i * spacing + j] = r;
 }
 
@@ -869,8 +962,13 @@ __global__ void k_residue_to_y(RingDev rd, const uint64_t* res, int spacing, uin
   if (j >= spacing) return;
   const uint64_t p = rd.primes[i];
   uint64_t r = res[(size_t)i * spacing + j];
-  r = mod_p(r, p, rd.barrett[i]);
-  Y[(size_t)i * spacing + j] = (uint32_t)mod_p(r * rd.invMi[set][i], p, rd.barrett[i]);
+  if (i == 0) {
+    if (r >= gl::P) r -= gl::P;
+    store_y(Y, spacing, 0, j, gl::mul(r, rd.invMi[set][0]));
+  } else {
+    r = mod_p(r, p, rd.barrett[i]);
+    store_y(Y, spacing, i, j, mod_p(r * rd.invMi[set][i], p, rd.barrett[i]));
+  }
 }
 
 // Residue-domain add/sub mod p_i (the reference's polynomialAddSub,
@@ -883,7 +981,8 @@ __global__ void k_residue_addsub(RingDev rd, uint64_t* C, const uint64_t* A, con
   size_t o = (size_t)i * spacing + j;
   uint64_t a = A[o], b = B[o];
   uint64_t r;
-  if (sub) r = a >= b ? a - b : a + p - b;
+  if (i == 0) r = sub ? gl::sub(a, b) : gl::add(a, b);  // residues mod P itself
+  else if (sub) r = a >= b ? a - b : a + p - b;
   else { r = a + b; if (r >= p) r -= p; }
   C[o] = r;
 }

@@ -18,6 +18,11 @@
 //  * The prime count makes M >= 4*B, B the largest |coefficient| the operation
 //    can produce, so the ICRT quotient k = round(sum y_i/p_i) can be found
 //    with 64-bit fixed point and no correction step.
+//  * Modulus 0 is the Goldilocks prime P itself: the transform computes the
+//    convolution mod P exactly whatever the operand sizes, so the residue mod
+//    P needs no size bound and carries 64 bits of M for the price of one
+//    small prime (the reference's CRT uses small primes only; the
+//    reconstructed integers are the same).
 #pragma once
 #include <stdint.h>
 
@@ -120,7 +125,7 @@ struct RingParams {
   int w_bits = 0;   // keyswitch base 2^w_bits (0: no keyswitch)
   int lw = 0;       // number of keyswitch digits
   uint64_t pmax = 0;
-  std::vector<uint64_t> primes;     // R primes, descending
+  std::vector<uint64_t> primes;     // R moduli: P, then small primes descending
   std::vector<uint64_t> barrett;    // floor(2^64 / p_i)
   std::vector<uint64_t> fixinv;     // floor(2^64 / p_i) used by the fixed-point quotient
   std::vector<uint64_t> pow32;      // [R][2*LW]: 2^(32k) mod p_i
@@ -142,7 +147,8 @@ struct RingParams {
     for (int i = 0; i < R; ++i) {
       BigInt Mi = s.M / BigInt(primes[i]);
       Mi.to_limbs(&s.Mi_limbs[(size_t)i * LW], LW);
-      s.invMi[i] = inv_mod_u64(Mi.mod_u64(primes[i]), primes[i]);
+      const uint64_t r = Mi.mod_u64(primes[i]);
+      s.invMi[i] = primes[i] == gl::P ? gl::inv(r) : inv_mod_u64(r, primes[i]);
     }
   }
 
@@ -172,14 +178,21 @@ struct RingParams {
     BigInt B_ks = w_bits ? BigInt(lw) * BigInt(n) * (BigInt::pow2(w_bits) - BigInt(1)) * qm1 : BigInt(0);
     BigInt need = BigInt(4) * (B_mul > B_ks ? B_mul : B_ks);
     if (need < BigInt(4)) need = BigInt(4);
-    primes = take_primes(pmax, need);
+    // modulus 0 = P, then the small primes for the rest of the range
+    const BigInt Pb(gl::P);
+    BigInt need_small = (need + Pb - BigInt(1)) / Pb;
+    primes.assign(1, gl::P);
+    if (need_small > BigInt(1)) {
+      std::vector<uint64_t> sp = take_primes(pmax, need_small);
+      primes.insert(primes.end(), sp.begin(), sp.end());
+    }
     int R = (int)primes.size();
     barrett.resize(R);
     fixinv.resize(R);
     pow32.assign((size_t)R * 2 * LW, 0);
     for (int i = 0; i < R; ++i) {
       uint64_t p = primes[i];
-      barrett[i] = (uint64_t)((((unsigned __int128)1) << 64) / p);
+      barrett[i] = i == 0 ? 1 : (uint64_t)((((unsigned __int128)1) << 64) / p);
       fixinv[i] = barrett[i];
       uint64_t v = 1;
       for (int k = 0; k < 2 * LW; ++k) {

@@ -319,7 +319,7 @@ int ring_product(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* out
                  const uint64_t* const* a_list = nullptr, const uint64_t* const* b_list = nullptr,
                  uint64_t* const* out_list = nullptr) {
   const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * yhk::y_rows(R) * n * sizeof(uint32_t));
   if (st) return st;
   yhk::ProdArgs pa{};
   pa.A = a; pa.B = b;
@@ -349,7 +349,7 @@ void launch_reduce_t(yh_ring* r, const uint64_t* c, int lin, uint64_t* out, int 
 int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out, int batch,
                    uint64_t* const* out_list = nullptr) {
   const int n = r->rd.n, LQ = r->rd.LQ, Rks = r->rd.set_R[1];
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * Rks * n * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * yhk::y_rows(Rks) * n * sizeof(uint32_t));
   if (st) return st;
   yhk::KsArgs ka{};
   ka.C = c;
@@ -651,7 +651,7 @@ int yh_icrt(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, uint64_t*
   if (!r || !d_res || !d_coefs) return fail(YH_ERR_ARG, "null argument");
   if (ncoef <= 0 || ncoef > spacing || out_limbs < 1) return fail(YH_ERR_ARG, "bad sizes");
   if (int s = set_dev(r->ctx)) return s;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)r->rd.R * spacing * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)yhk::y_rows(r->rd.R) * spacing * sizeof(uint32_t));
   if (st) return st;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
   hipLaunchKernelGGL(yhk::k_residue_to_y, grid, block, 0, r->stream, r->rd, d_res, spacing, r->d_y, 0);
@@ -709,7 +709,7 @@ int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t
   if (N > 16384) return fail(YH_ERR_UNSUPPORTED, "plain product longer than 16384");
   if (int s = set_dev(r->ctx)) return s;
   const int R = r->rd.R, LQ = r->rd.LQ;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)R * N * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)yhk::y_rows(R) * N * sizeof(uint32_t));
   if (st) return st;
   DevTables tw{};
   if (N >= 16) {
@@ -841,7 +841,7 @@ int yh_yashe_mul_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, co
 static int decrypt_impl(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch,
                         const uint64_t* const* c_list, uint64_t* const* m_list) {
   const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * yhk::y_rows(R) * n * sizeof(uint32_t));
   if (st) return st;
   yhk::ProdArgs pa{};
   pa.A = d_f; pa.B = d_c;
@@ -971,7 +971,7 @@ int yh_ring_mul_bcast(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint
   if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (int s = set_dev(r->ctx)) return s;
   const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * R * n * sizeof(uint32_t));
+  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * yhk::y_rows(R) * n * sizeof(uint32_t));
   if (st) return st;
   yhk::ProdArgs pa{};
   pa.A = d_a; pa.B = d_b;
@@ -1323,7 +1323,7 @@ int yh_batch_create(yh_ctx* ctx, const yh_batch_item* items, int count, const yh
   std::vector<size_t> o_y(count);
   for (int k = 0; k < count; ++k) {
     o_y[k] = off;
-    off = align(off + (size_t)items[k].ring->rd.R * items[k].ring->rd.n * sizeof(uint32_t));
+    off = align(off + (size_t)yhk::y_rows(items[k].ring->rd.R) * items[k].ring->rd.n * sizeof(uint32_t));
   }
   auto b = std::make_unique<yh_batch>();
   b->ctx = ctx;
@@ -1454,7 +1454,7 @@ size_t yh_mem_footprint(const yh_ring* r, int op) {
   if (!r) return 0;
   size_t n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R, Rks = r->rd.set_R[1];
   size_t ct = LQ * n * sizeof(uint64_t);
-  size_t y = R * n * sizeof(uint32_t), yks = Rks * n * sizeof(uint32_t);
+  size_t y = yhk::y_rows((int)R) * n * sizeof(uint32_t), yks = yhk::y_rows((int)Rks) * n * sizeof(uint32_t);
   switch (op) {
     case 0: return 3 * ct + y;
     case 1: return 3 * ct + y;

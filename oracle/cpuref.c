@@ -196,13 +196,14 @@ static uint64_t bn_mod_word(const uint64_t* a, int n, uint64_t p) {
   return (uint64_t)w;
 }
 static uint64_t inv_word(uint64_t a, uint64_t m) {
-  int64_t t = 0, nt = 1, r = (int64_t)m, nr = (int64_t)(a % m);
+  /* signed 128-bit extended Euclid: m may use all 64 bits */
+  __int128 t = 0, nt = 1, r = (__int128)m, nr = (__int128)(a % m);
   while (nr) {
-    int64_t q = r / nr, tmp;
+    __int128 q = r / nr, tmp;
     tmp = t - q * nt; t = nt; nt = tmp;
     tmp = r - q * nr; r = nr; nr = tmp;
   }
-  if (t < 0) t += (int64_t)m;
+  if (t < 0) t += (__int128)m;
   return (uint64_t)t;
 }
 static int is_prime_word(uint64_t x) {

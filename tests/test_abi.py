@@ -57,9 +57,13 @@ def test_no_gpu_fails_loudly():
 def test_ring_plan_bounds(n, q, t, wb):
     info, primes = ya.ring_plan(n, q, t, wb)
     assert info.R == len(primes) and info.n == n
-    assert primes == sorted(primes, reverse=True) and len(set(primes)) == len(primes)
+    # modulus 0 is the Goldilocks prime itself (its convolution is exact mod P
+    # for any operands); the small primes follow in descending order
+    assert primes[0] == P
+    small = primes[1:]
+    assert small == sorted(small, reverse=True) and len(set(small)) == len(small)
     K = max(1, info.lw)
-    for p in primes:
+    for p in small:
         assert all(p % d for d in range(2, int(p ** 0.5) + 1))
         # centred residues keep the Goldilocks convolution exact
         assert K * n * ((p - 1) // 2) ** 2 <= (P - 1) // 2
