@@ -65,6 +65,10 @@ struct RingDev {
   const uint64_t* Mi[2];    // [R][LW]
   const uint64_t* invMi[2]; // [R]
   const uint64_t* M[2];     // [LW]
+  // limb bounds of the reconstruction epilogues: |c| < M[set] has at most
+  // lim_m[set] limbs, t|c| at most lim_scale
+  int lim_m[2];
+  int lim_scale;
 };
 
 // threadIdx.x, recomputed after `dep` is ready (see the register note above).
@@ -656,11 +660,16 @@ __device__ __forceinline__ bool geq_n(const uint64_t* a, const uint64_t* b) {
   return true;
 }
 
-// Quotient and remainder of a (LA = LW+1 limbs) by q (LQ limbs) with the
-// generalised Barrett reciprocal mu = floor(2^(64 LA) / q):
-// qhat = floor(a*mu / 2^(64 LA)) is q_true or q_true - 1.
+// Quotient and remainder of a (LA = LW+1 limbs, only the low `lin` of them
+// possibly nonzero) by q (LQ limbs) with the generalised Barrett reciprocal
+// mu = floor(2^(64 LA) / q). `lin` is the same for every thread of a launch
+// (it comes from the ring's bounds), so the skipped limbs are uniform
+// branches. The partial products a_i mu_k with i + k < LA - 2 are left out:
+// together they are below 2^(64 LA - 56), so the estimate
+// floor(a*mu / 2^(64 LA)) falls short of the true quotient by at most 2 and
+// at most two corrections follow.
 template <int LQ>
-__device__ void divmod_q(const uint64_t* a, const RingDev& rd, uint64_t* quo, uint64_t* rem) {
+__device__ void divmod_q(const uint64_t* a, const RingDev& rd, uint64_t* quo, uint64_t* rem, int lin) {
   constexpr int LW = 2 * LQ + 1;
   constexpr int LA = LW + 1;
   constexpr int LMU = LW + 2 - LQ;
@@ -669,9 +678,11 @@ __device__ void divmod_q(const uint64_t* a, const RingDev& rd, uint64_t* quo, ui
   for (int l = 0; l < LA + LMU; ++l) prod[l] = 0;
 #pragma unroll
   for (int i = 0; i < LA; ++i) {
+    if (i >= lin) break;
     uint64_t carry = 0;
 #pragma unroll
     for (int k = 0; k < LMU; ++k) {
+      if (i + k < LA - 2) continue;
       uint64_t lo = a[i] * rd.mu[k];
       uint64_t hi = gl::umulhi(a[i], rd.mu[k]);
       uint64_t s = prod[i + k] + lo;
@@ -683,14 +694,17 @@ __device__ void divmod_q(const uint64_t* a, const RingDev& rd, uint64_t* quo, ui
     }
     prod[i + LMU] = carry;
   }
+  // the quotient has at most lin - LQ + 1 nonzero limbs
+  const int nq = lin - LQ + 1;
 #pragma unroll
-  for (int k = 0; k < LMU; ++k) quo[k] = prod[LA + k];
-  // r = a - quo*q over LQ+1 limbs (the true remainder is < 2q)
+  for (int k = 0; k < LMU; ++k) quo[k] = k < nq ? prod[LA + k] : 0;
+  // r = a - quo*q over LQ+1 limbs (the true remainder is < 3q)
   uint64_t qq[LQ + 1];
 #pragma unroll
   for (int l = 0; l <= LQ; ++l) qq[l] = 0;
 #pragma unroll
   for (int k = 0; k < LMU; ++k) {
+    if (k >= nq) break;
     uint64_t carry = 0;
 #pragma unroll
     for (int l = 0; l <= LQ; ++l) {
@@ -714,12 +728,15 @@ __device__ void divmod_q(const uint64_t* a, const RingDev& rd, uint64_t* quo, ui
 #pragma unroll
   for (int l = 0; l < LQ; ++l) qe[l] = rd.q[l];
   qe[LQ] = 0;
-  if (geq_n<LQ + 1>(r, qe)) {
-    sub_n<LQ + 1>(r, qe);
-    uint64_t one[LMU];
+  uint64_t one[LMU];
 #pragma unroll
-    for (int k = 0; k < LMU; ++k) one[k] = k == 0;
-    add_n<LMU>(quo, one);
+  for (int k = 0; k < LMU; ++k) one[k] = k == 0;
+#pragma unroll
+  for (int fix = 0; fix < 2; ++fix) {
+    if (geq_n<LQ + 1>(r, qe)) {
+      sub_n<LQ + 1>(r, qe);
+      add_n<LMU>(quo, one);
+    }
   }
 #pragma unroll
   for (int l = 0; l < LQ; ++l) rem[l] = r[l];
@@ -821,7 +838,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
   for (int l = 0; l < LW; ++l) a[l] = acc[l];
   a[LW] = 0;
   if (mode == EPI_MODQ || mode == EPI_DECRYPT) {
-    divmod_q<LQ>(a, rd, quo, rem);
+    divmod_q<LQ>(a, rd, quo, rem, rd.lim_m[set]);
     bool nz = false;
 #pragma unroll
     for (int l = 0; l < LQ; ++l) nz |= rem[l] != 0;
@@ -852,7 +869,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
       carry = hi;
     }
     a[LQ] = carry;
-    divmod_q<LQ>(a, rd, quo, rem);
+    divmod_q<LQ>(a, rd, quo, rem, LQ + 1);
     if (twice_gt_q<LQ>(rem, rd.q)) {
       uint64_t one[LMU];
 #pragma unroll
@@ -877,7 +894,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
     }
     a[LW] = carry;
   }
-  divmod_q<LQ>(a, rd, quo, rem);  // |t g| = quo*q + rem
+  divmod_q<LQ>(a, rd, quo, rem, rd.lim_scale);  // |t g| = quo*q + rem
   // signed floor quotient Qs and remainder rs in [0, q)
   bool nz = false;
 #pragma unroll
@@ -911,7 +928,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
 #pragma unroll
   for (int l = 0; l < LA; ++l) a2[l] = l < LMU ? quo[l] : 0;
   uint64_t quo2[LMU], r2[LQ];
-  divmod_q<LQ>(a2, rd, quo2, r2);
+  divmod_q<LQ>(a2, rd, quo2, r2, rd.lim_scale - LQ + 1);
   bool nz2 = false;
 #pragma unroll
   for (int l = 0; l < LQ; ++l) nz2 |= r2[l] != 0;
@@ -1095,7 +1112,7 @@ __global__ void k_reduce(RingDev rd, const uint64_t* C, int lin, uint64_t* out) 
   bool neg = a[LA - 1] >> 63;
   if (neg) neg_n<LA>(a);
   uint64_t quo[LMU], rem[LQ];
-  divmod_q<LQ>(a, rd, quo, rem);
+  divmod_q<LQ>(a, rd, quo, rem, lin);
   bool nz = false;
 #pragma unroll
   for (int l = 0; l < LQ; ++l) nz |= rem[l] != 0;

@@ -565,6 +565,13 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   rd.pow32 = d + o_pw; rd.q = d + o_q; rd.mu = d + o_mu;
   rd.set_R[0] = P.full.R; rd.Mi[0] = d + o_mi0; rd.invMi[0] = d + o_inv0; rd.M[0] = d + o_m0;
   rd.set_R[1] = P.ks.R; rd.Mi[1] = d + o_mi1; rd.invMi[1] = d + o_inv1; rd.M[1] = d + o_m1;
+  rd.lim_m[0] = (P.full.M.bits() + 63) / 64;
+  rd.lim_m[1] = (P.ks.M.bits() + 63) / 64;
+  {
+    int tb = 0;
+    while (tb < 64 && (P.t >> tb)) ++tb;
+    rd.lim_scale = std::min(P.LW + 1, (P.full.M.bits() + tb + 63) / 64);
+  }
   *out = r.release();
   return YH_OK;
   YH_TRY_END
