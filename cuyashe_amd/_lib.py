@@ -180,15 +180,19 @@ def array_to_ints(arr: np.ndarray) -> List[int]:
     return vals
 
 
-def ring_plan(n: int, q: int, t: int = 0, w_bits: int = 0):
-    """Host-only parameter planning: (RingInfo, primes)."""
+def ring_plan(n: int, q: int, t: int = 0, w_bits: int = 0, keyswitch: bool = False):
+    """Host-only parameter planning: (RingInfo, product moduli), or with
+    keyswitch=True (RingInfo, product moduli, keyswitch moduli)."""
     q = int(q)
     L = max(1, (q.bit_length() + 63) // 64)
     ql = (ctypes.c_uint64 * L)(*int_to_limbs(q, L))
     info = RingInfo()
     buf = (ctypes.c_uint64 * 512)()
     check(lib().yh_ring_plan(n, ql, L, t, w_bits, ctypes.byref(info), buf, 512), "yh_ring_plan")
-    return info, [int(buf[i]) for i in range(info.R)]
+    prod = [int(buf[i]) for i in range(info.R)]
+    if keyswitch:
+        return info, prod, [int(buf[info.R + i]) for i in range(info.R_ks)]
+    return info, prod
 
 
 class DeviceBuffer:
@@ -379,9 +383,16 @@ class Ring:
         self.LW = info.work_limbs
 
     def primes(self) -> List[int]:
+        """Moduli of ring products (P first)."""
         buf = (ctypes.c_uint64 * self.R)()
         check(lib().yh_ring_get_primes(self.h, buf, self.R), "yh_ring_get_primes")
         return [int(v) for v in buf]
+
+    def ks_primes(self) -> List[int]:
+        """Moduli of the keyswitch multiply-accumulate (P first)."""
+        buf = (ctypes.c_uint64 * (self.R + self.R_ks))()
+        check(lib().yh_ring_get_primes(self.h, buf, self.R + self.R_ks), "yh_ring_get_primes")
+        return [int(v) for v in buf][self.R:]
 
     def modulus(self, ks: bool = False) -> int:
         buf = (ctypes.c_uint64 * self.LW)()

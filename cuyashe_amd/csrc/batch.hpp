@@ -93,10 +93,10 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   const bool wave = logn <= 10;
   const BatchInst in = ba.inst[tk.inst];
   const int i = tk.prime;
-  const uint64_t p = rd.primes[i];
-  const uint64_t b = rd.barrett[i];
+  const uint64_t p = rd.primes[0][i];
+  const uint64_t b = rd.barrett[0][i];
   const int LQ = rd.LQ;
-  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  const uint64_t* pw = rd.pow32[0] + (size_t)i * 2 * rd.LW;
   const NttTab tab = rd.tab;
   uint64_t* my = lds + (size_t)sub * (N + T);
   uint64_t xa[E], xb[E];

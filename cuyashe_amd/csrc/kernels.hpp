@@ -55,13 +55,15 @@ struct RingDev {
   int R, w_bits, lw;
   uint64_t t;
   NttTab tab;               // length-n transform tables (negacyclic)
-  const uint64_t* primes;
-  const uint64_t* barrett;  // floor(2^64 / p_i)
-  const uint64_t* pow32;    // [R][2*LW]
   const uint64_t* q;        // [LQ]
   const uint64_t* mu;       // [LW+2-LQ]
-  // reconstruction sets: 0 = all R primes, 1 = keyswitch prefix
+  // Two modulus sets, each P followed by small primes: 0 = ring products
+  // (primes sized for one product per convolution), 1 = keyswitch (sized
+  // for the l_w-digit multiply-accumulate). Per set:
   int set_R[2];
+  const uint64_t* primes[2];
+  const uint64_t* barrett[2];  // floor(2^64 / p_i)
+  const uint64_t* pow32[2];    // [R][2*LW]: 2^(32k) mod p_i
   const uint64_t* Mi[2];    // [R][LW]
   const uint64_t* invMi[2]; // [R]
   const uint64_t* M[2];     // [LW]
@@ -354,9 +356,9 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   constexpr int N = 1 << LOGN;
   int pair, i;
   if (!xcd_map(blockIdx.x, rd.set_R[pa.set], pa.batch, pair, i)) return;
-  const uint64_t p = rd.primes[i];
-  const uint64_t b = rd.barrett[i];
-  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  const uint64_t p = rd.primes[pa.set][i];
+  const uint64_t b = rd.barrett[pa.set][i];
+  const uint64_t* pw = rd.pow32[pa.set] + (size_t)i * 2 * rd.LW;
   uint64_t xa[E], xb[E];
   const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
   const uint64_t* Bp = pa.b_list ? pa.b_list[pair] : pa.B + pair * pa.ostride_b;
@@ -399,9 +401,9 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
   const int pair = blockIdx.y;
   if (threadIdx.x != 0) return;
   const int N = 1 << pa.logN;
-  const uint64_t p = rd.primes[i];
-  const uint64_t b = rd.barrett[i];
-  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  const uint64_t p = rd.primes[pa.set][i];
+  const uint64_t b = rd.barrett[pa.set][i];
+  const uint64_t* pw = rd.pow32[pa.set] + (size_t)i * 2 * rd.LW;
   uint64_t ra[64], rb[64], rc[64];
   const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
   const uint64_t* Bp = pa.b_list ? pa.b_list[pair] : pa.B + pair * pa.ostride_b;
@@ -466,8 +468,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_keyswitch(Rin
   const int Rks = rd.set_R[1];
   int ct, i;
   if (!xcd_map(blockIdx.x, Rks, ka.batch, ct, i)) return;
-  const uint64_t p = rd.primes[i];
-  const uint64_t b = rd.barrett[i];
+  const uint64_t p = rd.primes[1][i];
+  const uint64_t b = rd.barrett[1][i];
   const uint64_t* c0 = ka.C + ct * ka.ostride;
   const bool gold = i == 0;
   uint64_t acc[E], x[E];
@@ -513,8 +515,8 @@ __global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* gamma_r
   if (threadIdx.x != 0) return;
   const int N = rd.n;
   const int Rks = rd.set_R[1];
-  const uint64_t p = rd.primes[i];
-  const uint64_t b = rd.barrett[i];
+  const uint64_t p = rd.primes[1][i];
+  const uint64_t b = rd.barrett[1][i];
   const uint64_t* c0 = ka.C + ct * ka.ostride;
   const bool gold = i == 0;
   uint64_t rc[64];
@@ -548,9 +550,9 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_evk_prep(Rin
   const int i = blockIdx.x;
   const int k = blockIdx.y;
   const int Rks = rd.set_R[1];
-  const uint64_t p = rd.primes[i];
-  const uint64_t b = rd.barrett[i];
-  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  const uint64_t p = rd.primes[1][i];
+  const uint64_t b = rd.barrett[1][i];
+  const uint64_t* pw = rd.pow32[1] + (size_t)i * 2 * rd.LW;
   const uint64_t* g = G + (size_t)k * rd.LQ * N;
   uint64_t* o = out + ((size_t)k * Rks + i) * N;
   uint64_t x[E];
@@ -566,9 +568,9 @@ __global__ void k_evk_residues(RingDev rd, const uint64_t* G, uint64_t* out) {
   const int i = blockIdx.x;
   const int k = blockIdx.y;
   const int Rks = rd.set_R[1];
-  const uint64_t p = rd.primes[i];
-  const uint64_t b = rd.barrett[i];
-  const uint64_t* pw = rd.pow32 + (size_t)i * 2 * rd.LW;
+  const uint64_t p = rd.primes[1][i];
+  const uint64_t b = rd.barrett[1][i];
+  const uint64_t* pw = rd.pow32[1] + (size_t)i * 2 * rd.LW;
   const int N = rd.n;
   const uint64_t* g = G + (size_t)k * rd.LQ * N;
   uint64_t* o = out + ((size_t)k * Rks + i) * N;
@@ -796,7 +798,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
   for (int i = 1; i < RS; ++i) {
     uint64_t yi = y[(size_t)(i + 1) * ystride];
     // fixed-point y_i/p_i with 64 fractional bits (y_i < p_i keeps it < 2^64)
-    uint64_t f = yi * rd.barrett[i];
+    uint64_t f = yi * rd.barrett[set][i];
     flo += f;
     fhi += (flo < f);
     mac32<LW>(acc, Mi + (size_t)i * LW, yi);
@@ -964,11 +966,11 @@ __global__ void k_crt(RingDev rd, const uint64_t* A, int lin, int ncoef, size_t 
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= spacing) return;
-  const uint64_t p = rd.primes[i];
+  const uint64_t p = rd.primes[0][i];
   uint64_t r = 0;
   if (j < ncoef)
     r = i == 0 ? crt_residue_P(A + j, lin, lstride)
-               : crt_residue(A + j, lin, lstride, rd.pow32 + (size_t)i * 2 * rd.LW, p, rd.barrett[i]);
+               : crt_residue(A + j, lin, lstride, rd.pow32[0] + (size_t)i * 2 * rd.LW, p, rd.barrett[0][i]);
   out[(size_t)i * spacing + j] = r;
 }
 
@@ -977,14 +979,14 @@ __global__ void k_residue_to_y(RingDev rd, const uint64_t* res, int spacing, uin
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= spacing) return;
-  const uint64_t p = rd.primes[i];
+  const uint64_t p = rd.primes[set][i];
   uint64_t r = res[(size_t)i * spacing + j];
   if (i == 0) {
     if (r >= gl::P) r -= gl::P;
     store_y(Y, spacing, 0, j, gl::mul(r, rd.invMi[set][0]));
   } else {
-    r = mod_p(r, p, rd.barrett[i]);
-    store_y(Y, spacing, i, j, mod_p(r * rd.invMi[set][i], p, rd.barrett[i]));
+    r = mod_p(r, p, rd.barrett[set][i]);
+    store_y(Y, spacing, i, j, mod_p(r * rd.invMi[set][i], p, rd.barrett[set][i]));
   }
 }
 
@@ -994,7 +996,7 @@ __global__ void k_residue_addsub(RingDev rd, uint64_t* C, const uint64_t* A, con
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= spacing) return;
-  const uint64_t p = rd.primes[i];
+  const uint64_t p = rd.primes[0][i];
   size_t o = (size_t)i * spacing + j;
   uint64_t a = A[o], b = B[o];
   uint64_t r;

@@ -124,18 +124,49 @@ struct RingParams {
   uint64_t t = 0;   // plaintext modulus
   int w_bits = 0;   // keyswitch base 2^w_bits (0: no keyswitch)
   int lw = 0;       // number of keyswitch digits
-  uint64_t pmax = 0;
-  std::vector<uint64_t> primes;     // R moduli: P, then small primes descending
-  std::vector<uint64_t> barrett;    // floor(2^64 / p_i)
-  std::vector<uint64_t> fixinv;     // floor(2^64 / p_i) used by the fixed-point quotient
-  std::vector<uint64_t> pow32;      // [R][2*LW]: 2^(32k) mod p_i
+  uint64_t pmax = 0;                // largest small prime of the product set
+  uint64_t pmax_ks = 0;             // ... of the keyswitch set
+  // Product set (index 0): R moduli, P then small primes descending, sized
+  // for one product per convolution. Keyswitch set (index 1): P then smaller
+  // primes sized for the l_w-term multiply-accumulate.
+  std::vector<uint64_t> primes, primes_ks;
+  std::vector<uint64_t> barrett, barrett_ks;  // floor(2^64 / p_i) (1 for P, unused)
+  std::vector<uint64_t> pow32, pow32_ks;      // [R][2*LW]: 2^(32k) mod p_i
   IcrtSet full, ks;
   std::vector<uint64_t> q_limbs;    // [LQ]
   std::vector<uint64_t> mu_limbs;   // [LW+2-LQ]: floor(2^(64*(LW+1)) / q)
 
   static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
-  void build_set(IcrtSet& s, int R) {
+  // P followed by descending primes <= pm until the product reaches `need`.
+  static std::vector<uint64_t> moduli_for(uint64_t pm, const BigInt& need) {
+    const BigInt Pb(gl::P);
+    BigInt need_small = (need + Pb - BigInt(1)) / Pb;
+    std::vector<uint64_t> out(1, gl::P);
+    if (need_small > BigInt(1)) {
+      std::vector<uint64_t> sp = take_primes(pm, need_small);
+      out.insert(out.end(), sp.begin(), sp.end());
+    }
+    return out;
+  }
+
+  void tables_for(const std::vector<uint64_t>& pr, std::vector<uint64_t>& bar, std::vector<uint64_t>& pw) {
+    const int R = (int)pr.size();
+    bar.resize(R);
+    pw.assign((size_t)R * 2 * LW, 0);
+    for (int i = 0; i < R; ++i) {
+      uint64_t p = pr[i];
+      bar[i] = p == gl::P ? 1 : (uint64_t)((((unsigned __int128)1) << 64) / p);
+      uint64_t v = 1;
+      for (int k = 0; k < 2 * LW; ++k) {
+        pw[(size_t)i * 2 * LW + k] = v;
+        v = (uint64_t)(((unsigned __int128)v << 32) % p);
+      }
+    }
+  }
+
+  void build_set(IcrtSet& s, const std::vector<uint64_t>& primes) {
+    const int R = (int)primes.size();
     s.R = R;
     s.M = BigInt(1);
     for (int i = 0; i < R; ++i) s.M *= BigInt(primes[i]);
@@ -169,44 +200,26 @@ struct RingParams {
     if (LQ > 8) throw std::invalid_argument("q wider than 512 bits is not supported");
     LW = 2 * LQ + 1;
     lw = w_bits ? (q.bits() + w_bits - 1) / w_bits : 0;
-    int K = lw > 1 ? lw : 1;
-    pmax = max_prime_for(n, K);
     BigInt qm1 = q - BigInt(1);
     // Ring products: |c_j| <= n (q-1)^2 for operands in [0, q).
     BigInt B_mul = BigInt(n) * qm1 * qm1;
-    // Keyswitch: |c_j| <= l_w n (w-1)(q-1).
-    BigInt B_ks = w_bits ? BigInt(lw) * BigInt(n) * (BigInt::pow2(w_bits) - BigInt(1)) * qm1 : BigInt(0);
-    BigInt need = BigInt(4) * (B_mul > B_ks ? B_mul : B_ks);
+    BigInt need = BigInt(4) * B_mul;
     if (need < BigInt(4)) need = BigInt(4);
-    // modulus 0 = P, then the small primes for the rest of the range
-    const BigInt Pb(gl::P);
-    BigInt need_small = (need + Pb - BigInt(1)) / Pb;
-    primes.assign(1, gl::P);
-    if (need_small > BigInt(1)) {
-      std::vector<uint64_t> sp = take_primes(pmax, need_small);
-      primes.insert(primes.end(), sp.begin(), sp.end());
-    }
-    int R = (int)primes.size();
-    barrett.resize(R);
-    fixinv.resize(R);
-    pow32.assign((size_t)R * 2 * LW, 0);
-    for (int i = 0; i < R; ++i) {
-      uint64_t p = primes[i];
-      barrett[i] = i == 0 ? 1 : (uint64_t)((((unsigned __int128)1) << 64) / p);
-      fixinv[i] = barrett[i];
-      uint64_t v = 1;
-      for (int k = 0; k < 2 * LW; ++k) {
-        pow32[(size_t)i * 2 * LW + k] = v;
-        v = (uint64_t)(((unsigned __int128)v << 32) % p);
-      }
-    }
-    build_set(full, R);
-    int Rks = 0;
+    pmax = max_prime_for(n, 1);
+    primes = moduli_for(pmax, need);
+    tables_for(primes, barrett, pow32);
+    build_set(full, primes);
     if (w_bits) {
-      BigInt need_ks = BigInt(4) * B_ks, M(1);
-      while (M < need_ks) M *= BigInt(primes[Rks++]);
+      // Keyswitch: |c_j| <= l_w n (w-1)(q-1), l_w products summed per convolution.
+      BigInt B_ks = BigInt(lw) * BigInt(n) * (BigInt::pow2(w_bits) - BigInt(1)) * qm1;
+      pmax_ks = max_prime_for(n, lw > 1 ? lw : 1);
+      primes_ks = moduli_for(pmax_ks, BigInt(4) * B_ks);
+    } else {
+      pmax_ks = pmax;
+      primes_ks = primes;
     }
-    build_set(ks, Rks > 0 ? Rks : R);
+    tables_for(primes_ks, barrett_ks, pow32_ks);
+    build_set(ks, primes_ks);
     q_limbs.assign(LQ, 0);
     q.to_limbs(q_limbs.data(), LQ);
     int LMU = LW + 2 - LQ;

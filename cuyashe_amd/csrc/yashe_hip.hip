@@ -547,8 +547,9 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   // one table buffer: pack every array, remember offsets
   std::vector<uint64_t> h;
   auto put = [&h](const std::vector<uint64_t>& v) { size_t o = h.size(); h.insert(h.end(), v.begin(), v.end()); return o; };
-  size_t o_pr = put(P.primes), o_b = put(P.barrett);
-  size_t o_pw = put(P.pow32), o_q = put(P.q_limbs), o_mu = put(P.mu_limbs);
+  size_t o_pr = put(P.primes), o_b = put(P.barrett), o_pw = put(P.pow32);
+  size_t o_pr1 = put(P.primes_ks), o_b1 = put(P.barrett_ks), o_pw1 = put(P.pow32_ks);
+  size_t o_q = put(P.q_limbs), o_mu = put(P.mu_limbs);
   size_t o_mi0 = put(P.full.Mi_limbs), o_inv0 = put(P.full.invMi), o_m0 = put(P.full.M_limbs);
   size_t o_mi1 = put(P.ks.Mi_limbs), o_inv1 = put(P.ks.invMi), o_m1 = put(P.ks.M_limbs);
   YH_HIP(hipMalloc(&r->d_tables, h.size() * sizeof(uint64_t)));
@@ -561,8 +562,9 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   if (P.n >= 16) {
     if (int st = upload_ntt16(P.n, &r->d_ntt, &rd.tab)) return st;
   }
-  rd.primes = d + o_pr; rd.barrett = d + o_b;
-  rd.pow32 = d + o_pw; rd.q = d + o_q; rd.mu = d + o_mu;
+  rd.primes[0] = d + o_pr; rd.barrett[0] = d + o_b; rd.pow32[0] = d + o_pw;
+  rd.primes[1] = d + o_pr1; rd.barrett[1] = d + o_b1; rd.pow32[1] = d + o_pw1;
+  rd.q = d + o_q; rd.mu = d + o_mu;
   rd.set_R[0] = P.full.R; rd.Mi[0] = d + o_mi0; rd.invMi[0] = d + o_inv0; rd.M[0] = d + o_m0;
   rd.set_R[1] = P.ks.R; rd.Mi[1] = d + o_mi1; rd.invMi[1] = d + o_inv1; rd.M[1] = d + o_m1;
   rd.lim_m[0] = (P.full.M.bits() + 63) / 64;
@@ -588,6 +590,16 @@ int yh_ring_destroy(yh_ring* r) {
   if (r->d_y) (void)hipFree(r->d_y);
   if (r->d_tmp) (void)hipFree(r->d_tmp);
   delete r;
+  return YH_OK;
+}
+
+// The R product moduli, then (when cap allows) the R_ks keyswitch moduli.
+static int copy_moduli(const yh::RingParams& P, uint64_t* out, int cap) {
+  const int R = (int)P.primes.size(), Rk = P.w_bits ? (int)P.primes_ks.size() : 0;
+  if (cap < R) return fail(YH_ERR_ARG, "capacity too small");
+  for (int i = 0; i < R; ++i) out[i] = P.primes[i];
+  if (cap >= R + Rk)
+    for (int i = 0; i < Rk; ++i) out[R + i] = P.primes_ks[i];
   return YH_OK;
 }
 
@@ -619,8 +631,7 @@ int yh_ring_plan(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, 
   yh::RingParams P(n, yh::BigInt::from_limbs(q, q_limbs), t, w_bits);
   fill_info(P, info);
   if (primes) {
-    if (cap < (int)P.primes.size()) return fail(YH_ERR_ARG, "capacity too small");
-    for (size_t i = 0; i < P.primes.size(); ++i) primes[i] = P.primes[i];
+    if (int st = copy_moduli(P, primes, cap)) return st;
   }
   return YH_OK;
   YH_TRY_END
@@ -628,10 +639,7 @@ int yh_ring_plan(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, 
 
 int yh_ring_get_primes(const yh_ring* r, uint64_t* out, int cap) {
   if (!r || !out) return fail(YH_ERR_ARG, "null argument");
-  int R = (int)r->P->primes.size();
-  if (cap < R) return fail(YH_ERR_ARG, "capacity too small");
-  for (int i = 0; i < R; ++i) out[i] = r->P->primes[i];
-  return YH_OK;
+  return copy_moduli(*r->P, out, cap);
 }
 
 int yh_ring_get_modulus(const yh_ring* r, int set, uint64_t* out, int cap) {

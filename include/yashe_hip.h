@@ -102,13 +102,18 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
                    yh_ring** out);
 int yh_ring_destroy(yh_ring* ring);
 /* Host-only planning of the same parameters (no device needed): fills info
- * and up to `cap` primes. Used for admission control and CPU-side checks. */
+ * and the moduli as yh_ring_get_primes does. Used for admission control and
+ * CPU-side checks. */
 int yh_ring_plan(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, yh_ring_info* info,
                  uint64_t* primes, int cap);
 int yh_ring_get_info(const yh_ring* ring, yh_ring_info* info);
-/* The R CRT primes, descending (Polynomial::CRTPrimes). */
+/* The R CRT moduli of ring products (Polynomial::CRTPrimes): modulus 0 is
+ * P = 2^64 - 2^32 + 1 itself, then small primes descending. When cap is at
+ * least R + R_ks the R_ks keyswitch moduli follow (P, then smaller primes
+ * sized for the l_w-digit multiply-accumulate). */
 int yh_ring_get_primes(const yh_ring* ring, uint64_t* out, int cap);
-/* M as limbs: set 0 = all R primes (Polynomial::CRTProduct), 1 = keyswitch. */
+/* M as limbs: set 0 = the product moduli (Polynomial::CRTProduct), 1 = the
+ * keyswitch moduli. */
 int yh_ring_get_modulus(const yh_ring* ring, int set, uint64_t* out, int cap);
 
 /* ------------------------------------------------------------ primitives */

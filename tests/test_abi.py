@@ -55,34 +55,41 @@ def test_no_gpu_fails_loudly():
     (2, 1171313591017775093490277364417, 0, 0),
 ])
 def test_ring_plan_bounds(n, q, t, wb):
-    info, primes = ya.ring_plan(n, q, t, wb)
+    info, primes, ks_primes = ya.ring_plan(n, q, t, wb, keyswitch=True)
     assert info.R == len(primes) and info.n == n
     # modulus 0 is the Goldilocks prime itself (its convolution is exact mod P
-    # for any operands); the small primes follow in descending order
-    assert primes[0] == P
-    small = primes[1:]
-    assert small == sorted(small, reverse=True) and len(set(small)) == len(small)
-    K = max(1, info.lw)
-    for p in small:
-        assert all(p % d for d in range(2, int(p ** 0.5) + 1))
-        # centred residues keep the Goldilocks convolution exact
-        assert K * n * ((p - 1) // 2) ** 2 <= (P - 1) // 2
-        assert p < (1 << 28)
+    # for any operands); the small primes follow in descending order, sized
+    # for one product per convolution (K = 1) in the product set and for the
+    # l_w-term multiply-accumulate in the keyswitch set
+    for mods, K in ((primes, 1), (ks_primes, max(1, info.lw))):
+        if not mods:
+            continue
+        assert mods[0] == P
+        small = mods[1:]
+        assert small == sorted(small, reverse=True) and len(set(small)) == len(small)
+        for p in small:
+            assert all(p % d for d in range(2, int(p ** 0.5) + 1))
+            # centred residues keep the Goldilocks convolution exact
+            assert K * n * ((p - 1) // 2) ** 2 <= (P - 1) // 2
+            assert p < (1 << 28)
     M = 1
     for p in primes:
         M *= p
     assert M >= 4 * n * (q - 1) ** 2
     assert M.bit_length() == info.m_bits
     assert M.bit_length() + 2 <= 64 * info.work_limbs
+    M_short = M // primes[-1]
+    assert M_short < 4 * n * (q - 1) ** 2  # no more primes than needed
     if wb:
         lw = (q.bit_length() + wb - 1) // wb
-        assert info.lw == lw
+        assert info.lw == lw and len(ks_primes) == info.R_ks
         Mk = 1
-        for p in primes[:info.R_ks]:
+        for p in ks_primes:
             Mk *= p
         assert Mk >= 4 * lw * n * ((1 << wb) - 1) * (q - 1)
-        Mk_short = Mk // primes[info.R_ks - 1]
-        assert Mk_short < 4 * lw * n * ((1 << wb) - 1) * (q - 1)
+        assert Mk.bit_length() == info.m_ks_bits
+        if len(ks_primes) > 1:
+            assert Mk // ks_primes[-1] < 4 * lw * n * ((1 << wb) - 1) * (q - 1)
 
 
 def test_ring_plan_rejects_bad_parameters():
