@@ -37,13 +37,17 @@ struct BatchTask {
   int prime;
 };
 
-// Every task of a bundle belongs to the same ring, so the ring's tables are
-// workgroup-uniform (scalar loads); only the instance and prime vary.
+// Every task of a bundle belongs to the same ring and either all or none
+// are modulus 0, so the ring's tables and the residue path are
+// workgroup-uniform (scalar loads and branches); only instance and prime
+// vary.
 struct BatchBundle {
   int logn;
   int count;  // tasks in this bundle (<= W / (n/16))
   int task0;  // first task index
   int ring;   // index into BatchArgs::rings
+  int gold;   // 1: every task is modulus 0 (P); 0: none is
+  int pad;
 };
 
 struct BatchArgs {
@@ -79,6 +83,7 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 // task finished the last prime of its product.
 // Transforms of n <= 1024 lie inside one wavefront each, so their LDS
 // exchanges synchronise the wavefront only.
+template <bool GOLD>
 __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBundle& bd, const RingDev& rd,
                                              uint64_t* lds, int* s_last) {
   const int logn = bd.logn;
@@ -100,7 +105,7 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   const NttTab tab = rd.tab;
   uint64_t* my = lds + (size_t)sub * (N + T);
   uint64_t xa[E], xb[E];
-  const bool gold = i == 0;  // modulus 0 is P itself (exact residues, not centred)
+  constexpr bool gold = GOLD;  // modulus 0 is P itself (exact residues, not centred)
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g + c * T;
@@ -179,7 +184,8 @@ __global__ void __launch_bounds__(BATCH_MAX_THREADS, WAVES) k_batch_prod(BatchAr
     if (bi < 0) break;
     const BatchBundle bd = ba.bundles[bi];
     const RingDev& rd = ba.rings[bd.ring];
-    batch_bundle(ba, bd, rd, lds, s_last);
+    if (bd.gold) batch_bundle<true>(ba, bd, rd, lds, s_last);
+    else batch_bundle<false>(ba, bd, rd, lds, s_last);
     // products completed by this bundle: ICRT by the whole workgroup (the
     // bundle's ring is workgroup-uniform, so is its limb count)
     const int n = 1 << bd.logn;

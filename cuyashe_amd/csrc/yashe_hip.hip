@@ -1309,16 +1309,20 @@ int yh_batch_create(yh_ctx* ctx, const yh_batch_item* items, int count, const yh
     for (int ri : ring_order) {
       const int ln = rings[ri]->rd.logn;
       const int cap = W / ((1 << ln) / 16);
-      int open = -1;
-      for (int k : qinst[q]) {
-        if (items[k].ring != rings[ri]) continue;
-        for (int i = 0; i < rings[ri]->rd.R; ++i) {
-          if (open < 0 || bundles[open].count == cap) {
-            bundles.push_back({ln, 0, (int)tasks.size(), ri});
-            open = (int)bundles.size() - 1;
+      // tasks of modulus 0 (P, a different residue path) get bundles of
+      // their own, so every bundle takes one path uniformly
+      for (int gold = 1; gold >= 0; --gold) {
+        int open = -1;
+        for (int k : qinst[q]) {
+          if (items[k].ring != rings[ri]) continue;
+          for (int i = gold ? 0 : 1; i < (gold ? 1 : rings[ri]->rd.R); ++i) {
+            if (open < 0 || bundles[open].count == cap) {
+              bundles.push_back({ln, 0, (int)tasks.size(), ri, gold});
+              open = (int)bundles.size() - 1;
+            }
+            tasks.push_back({k, i});
+            bundles[open].count++;
           }
-          tasks.push_back({k, i});
-          bundles[open].count++;
         }
       }
     }
