@@ -9,5 +9,5 @@ from ._lib import (  # noqa: F401
     Batch, Event, batch_autotune, batch_default_config,
     ints_to_array, array_to_ints, int_to_limbs, ring_plan,
     OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD,
-    KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH,
+    KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH, KERNEL_KS_DIGITS,
 )

@@ -23,7 +23,7 @@ YH_ADD = 0
 YH_SUB = 1
 
 OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD = range(5)
-KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH = range(4)
+KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH, KERNEL_KS_DIGITS = range(5)
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p

@@ -16,7 +16,8 @@
 //                         mod-q / scale-and-round epilogues of k_icrt
 //  * Ciphertext::operator* rounding  src/ciphertext.cpp:55-63 -> EPI_SCALE
 //  * Ciphertext::worddecomp          src/ciphertext.cpp:109-130 -> digit loads
-//                         inside k_keyswitch (no digit polynomials in HBM)
+//                         inside k_ks_digits (each digit transformed once,
+//                         shared by every keyswitch modulus)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -437,14 +438,22 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
 
 // ---------------------------------------------------------------- keyswitch
 //
-// One workgroup per (ciphertext, prime of the keyswitch set): digit k of each
-// coefficient (bits [k*w, (k+1)*w) of c in [0, q)) is read straight from the
-// ciphertext limbs, reduced mod p, transformed, multiplied by the resident
-// NTT(gamma_k mod p) and accumulated; one inverse transform per prime.
+// Two launches per batch. k_ks_digits: one workgroup per (ciphertext, digit
+// k); digit k of each coefficient (bits [k*w, (k+1)*w) of c in [0, q)) is
+// read straight from the ciphertext limbs, twisted and transformed once,
+// unreduced, and the transform is written to scratch. k_ks_mac: one workgroup
+// per (ciphertext, modulus of the keyswitch set) multiplies the l_w digit
+// transforms by the resident NTT(gamma_k mod m_i) and accumulates, then runs
+// one inverse transform. Sharing the digit transforms between moduli is exact
+// because the keyswitch primes are sized for unreduced digits
+// (l_w n (2^w - 1) (p-1)/2 <= (P-1)/2): the accumulated value is the integer
+// sum_k d_k * (gamma_k mod p_i), which the epilogue reduces mod p_i. The
+// reference (src/ciphertext.cpp:81-130) runs one full CRT product per digit.
 struct KsArgs {
   const uint64_t* C;        // [ct][LQ][n]
   size_t ostride;
   const uint64_t* evk;      // [lw][R_ks][n] transform domain
+  uint64_t* D;              // [ct][lw][n] digit transforms (scratch)
   uint32_t* Y;              // [ct][R_ks][n]
   int batch;
 };
@@ -458,52 +467,62 @@ __device__ __forceinline__ uint64_t digit_of(const uint64_t* a, int n, int lq, i
 }
 
 // W32: digits are the 32-bit words of the limbs (w = 2^32), read directly.
-// The multiply-accumulate sum stays in registers (126 VGPRs with the opaque
-// thread index, four waves per SIMD, no spills at any n).
+// Output layout: the transform's output permutation, thread g's 16 values
+// contiguous (the layout of the resident keys).
 template <int LOGN, bool W32>
-__global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_keyswitch(RingDev rd, KsArgs ka) {
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_ks_digits(RingDev rd, KsArgs ka) {
+  extern __shared__ uint64_t lds[];
+  constexpr int N = 1 << LOGN;
+  constexpr int stride = N >> 4;
+  int ct, k;
+  if (!xcd_map(blockIdx.x, rd.lw, ka.batch, ct, k)) return;
+  const uint64_t* c0 = ka.C + ct * ka.ostride;
+  const int g = threadIdx.x;
+  const uint32_t* w32 = reinterpret_cast<const uint32_t*>(c0 + (size_t)(k >> 1) * N) + (k & 1);
+  uint64_t x[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) {
+    const int j = g + c * stride;
+    const uint64_t d = W32 ? (uint64_t)w32[2 * j] : digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
+    // d < 2^32 < P: the digit is its own Goldilocks element
+    x[c] = mul_small(d, rd.tab.twist[j]);
+  }
+  ntt16_fwd<LOGN>(x, lds, rd.tab.fwd, g);
+  uint64_t* o = ka.D + ((size_t)ct * rd.lw + k) * N + (size_t)g * E;
+#pragma unroll
+  for (int c = 0; c < E; ++c) o[c] = x[c];
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_ks_mac(RingDev rd, KsArgs ka) {
   extern __shared__ uint64_t lds[];
   constexpr int N = 1 << LOGN;
   constexpr int stride = N >> 4;
   const int Rks = rd.set_R[1];
   int ct, i;
   if (!xcd_map(blockIdx.x, Rks, ka.batch, ct, i)) return;
+  const int g = threadIdx.x;
+  const uint64_t* dg = ka.D + (size_t)ct * rd.lw * N + (size_t)g * E;
+  const uint64_t* ev = ka.evk + (size_t)i * N + (size_t)g * E;
+  uint64_t acc[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) acc[c] = gl::mul(dg[c], ev[c]);
+  for (int k = 1; k < rd.lw; ++k) {
+    dg += N;
+    ev += (size_t)Rks * N;
+#pragma unroll
+    for (int c = 0; c < E; ++c) acc[c] = gl::add(acc[c], gl::mul(dg[c], ev[c]));
+  }
+  const int g2 = opaque_tid(acc[0]);
+  ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g2);
   const uint64_t p = rd.primes[1][i];
   const uint64_t b = rd.barrett[1][i];
-  const uint64_t* c0 = ka.C + ct * ka.ostride;
-  const bool gold = i == 0;
-  uint64_t acc[E], x[E];
-#pragma unroll
-  for (int c = 0; c < E; ++c) acc[c] = 0;
-  for (int k = 0; k < rd.lw; ++k) {
-    // Re-read the tables for each digit rather than keeping loop-invariant
-    // table values live across the loop (the empty asm hides the invariance).
-    const uint64_t* twist = rd.tab.twist;
-    const uint64_t* fwd = rd.tab.fwd;
-    asm volatile("" : "+s"(twist), "+s"(fwd));
-    const int g = opaque_tid(k);
-    const uint32_t* w32 = reinterpret_cast<const uint32_t*>(c0 + (size_t)(k >> 1) * N) + (k & 1);
-#pragma unroll
-    for (int c = 0; c < E; ++c) {
-      const int j = g + c * stride;
-      const uint64_t d = W32 ? (uint64_t)w32[2 * j] : digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
-      // digits are below 2^32 < P: their residue mod P is the digit itself
-      x[c] = gold ? mul_small(d, twist[j]) : twist_centered(mod_p(d, p, b), p, twist[j]);
-    }
-    if (k > 0) __syncthreads();  // previous transform's last LDS reads are done
-    ntt16_fwd<LOGN>(x, lds, fwd, g);
-    const uint64_t* ev = ka.evk + ((size_t)k * Rks + i) * N + g * E;
-#pragma unroll
-    for (int c = 0; c < E; ++c) acc[c] = gl::add(acc[c], gl::mul(x[c], ev[c]));
-  }
-  __syncthreads();
-  const int g = opaque_tid(acc[0]);
-  ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g);
   const uint64_t invM = rd.invMi[1][i];
+  const bool gold = i == 0;
   uint32_t* y = ka.Y + (size_t)ct * y_rows(Rks) * N;
 #pragma unroll
   for (int c = 0; c < E; ++c) {
-    const int j = g + c * stride;
+    const int j = g2 + c * stride;
     store_y(y, N, i, j, finish_any(gold, gl::mul(acc[c], rd.tab.untwist[j]), p, b, invM));
   }
 }

@@ -12,9 +12,10 @@
 // exactness holes (its 29-bit primes overflow the Goldilocks convolution for
 // n > 64):
 //  * The prime size is the largest that keeps the Goldilocks convolution exact:
-//    with residues centred in (-p/2, p/2], K*n*((p-1)/2)^2 <= (P-1)/2, where K
-//    is the number of products summed before the inverse transform (1 for a
-//    ring product, l_w for the keyswitch multiply-accumulate).
+//    for a ring product (residues of both operands centred in (-p/2, p/2]),
+//    n*((p-1)/2)^2 <= (P-1)/2; for the keyswitch, whose digits are transformed
+//    unreduced once and shared by every modulus,
+//    l_w*n*(2^w_bits - 1)*((p-1)/2) <= (P-1)/2.
 //  * The prime count makes M >= 4*B, B the largest |coefficient| the operation
 //    can produce, so the ICRT quotient k = round(sum y_i/p_i) can be found
 //    with 64-bit fixed point and no correction step.
@@ -89,6 +90,22 @@ inline uint64_t max_prime_for(int n, int K) {
     if (v <= lim) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+// Largest prime size for the keyswitch set. The keyswitch transforms every
+// digit d (0 <= d < 2^w_bits) once, unreduced, and shares that transform
+// between all moduli; only the key residue is reduced mod p and centred. The
+// l_w-term multiply-accumulate stays exact in the Goldilocks field when
+// l_w * n * (2^w_bits - 1) * ((p-1)/2) <= (P-1)/2.
+inline uint64_t max_prime_ks(int n, int lw, int w_bits) {
+  const unsigned __int128 lim = (unsigned __int128)((gl::P - 1) / 2);
+  const unsigned __int128 per = (unsigned __int128)(lw > 1 ? lw : 1) * (unsigned __int128)n *
+                                (unsigned __int128)((1ull << w_bits) - 1);
+  const unsigned __int128 h = lim / per;  // largest (p-1)/2
+  const unsigned __int128 cap = (1ull << 28) - 1;
+  const unsigned __int128 pm = 2 * h + 1;
+  if (pm < 3) throw std::invalid_argument("keyswitch digits too wide for exact transforms");
+  return (uint64_t)(pm < cap ? pm : cap);
 }
 
 // Descending primes <= pmax until their product reaches `need`.
@@ -212,7 +229,7 @@ struct RingParams {
     if (w_bits) {
       // Keyswitch: |c_j| <= l_w n (w-1)(q-1), l_w products summed per convolution.
       BigInt B_ks = BigInt(lw) * BigInt(n) * (BigInt::pow2(w_bits) - BigInt(1)) * qm1;
-      pmax_ks = max_prime_for(n, lw > 1 ? lw : 1);
+      pmax_ks = max_prime_ks(n, lw, w_bits);
       primes_ks = moduli_for(pmax_ks, BigInt(4) * B_ks);
     } else {
       pmax_ks = pmax;

@@ -68,8 +68,8 @@ struct yh_ctx {
   bool timing = false;
   std::vector<TimedLaunch> pending;
   std::vector<hipEvent_t> event_pool;
-  double total_ms[4] = {0, 0, 0, 0};
-  long launches[4] = {0, 0, 0, 0};
+  double total_ms[YH_KERNEL_COUNT] = {};
+  long launches[YH_KERNEL_COUNT] = {};
   int cus = 0;           // compute units of the device
   std::string arch;      // gcnArchName
   bool capturing = false;
@@ -91,6 +91,8 @@ struct yh_ring {
   size_t y_bytes = 0;
   uint64_t* d_tmp = nullptr;   // ring-element scratch (mul+keyswitch intermediate)
   size_t tmp_bytes = 0;
+  uint64_t* d_dig = nullptr;   // keyswitch digit transforms [ct][lw][n]
+  size_t dig_bytes = 0;
   std::map<int, yh_ring*> sub;  // rings x^m + 1 over the same q, m < n (inverse recursion)
   hipStream_t stream = nullptr;  // the context stream, or the ring's own (yh_ring_use_own_stream)
   bool own_stream = false;
@@ -199,11 +201,6 @@ int ilog2i(long v) { int l = 0; while ((1L << l) < v) ++l; return l; }
 
 size_t lds_bytes(int N) { return (size_t)(N + N / 16) * sizeof(uint64_t); }
 
-template <int LOGN>
-size_t ks_lds_bytes() {
-  return lds_bytes(1 << LOGN);
-}
-
 // Every transform kernel is instantiated per log2(N) in [4, 14].
 #define YH_LOGN_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
 
@@ -212,10 +209,11 @@ void set_lds_attrs() {
   const int bytes = (int)lds_bytes(1 << LOGN);
   (void)hipFuncSetAttribute((const void*)yhk::k_prod<LOGN, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_prod<LOGN, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_keyswitch<LOGN, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)ks_lds_bytes<LOGN>());
-  (void)hipFuncSetAttribute((const void*)yhk::k_keyswitch<LOGN, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)ks_lds_bytes<LOGN>());
+  (void)hipFuncSetAttribute((const void*)yhk::k_ks_digits<LOGN, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes);
+  (void)hipFuncSetAttribute((const void*)yhk::k_ks_digits<LOGN, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes);
+  (void)hipFuncSetAttribute((const void*)yhk::k_ks_mac<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_evk_prep<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ntt_cyclic<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
@@ -357,26 +355,42 @@ int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out
   ka.evk = k->d;
   ka.Y = r->d_y;
   ka.batch = batch;
-  dim3 grid(Rks, batch);
-  dim3 grid_x(yhk::xcd_grid(batch, Rks));
-  LaunchTimer* timer = new LaunchTimer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
   if (n >= 16) {
-    switch (r->rd.logn) {
-#define X(L)                                                                                                  \
-  case L:                                                                                                       \
-    if (r->rd.w_bits == 32)                                                                                     \
-      hipLaunchKernelGGL((yhk::k_keyswitch<L, true>), grid_x, dim3(n / 16), ks_lds_bytes<L>(), r->stream, r->rd, ka); \
-    else                                                                                                        \
-      hipLaunchKernelGGL((yhk::k_keyswitch<L, false>), grid_x, dim3(n / 16), ks_lds_bytes<L>(), r->stream, r->rd, ka); \
+    st = ensure(r->ctx, r->stream, (void**)&r->d_dig, &r->dig_bytes, (size_t)batch * r->rd.lw * n * sizeof(uint64_t));
+    if (st) return st;
+    ka.D = r->d_dig;
+    const bool w32 = r->rd.w_bits == 32;
+    {
+      LaunchTimer timer(r->ctx, YH_KERNEL_KS_DIGITS, r->stream);
+      dim3 grid(yhk::xcd_grid(batch, r->rd.lw));
+      switch (r->rd.logn) {
+#define X(L)                                                                                                   \
+  case L:                                                                                                        \
+    if (w32)                                                                                                     \
+      hipLaunchKernelGGL((yhk::k_ks_digits<L, true>), grid, dim3(n / 16), lds_bytes(n), r->stream, r->rd, ka);  \
+    else                                                                                                         \
+      hipLaunchKernelGGL((yhk::k_ks_digits<L, false>), grid, dim3(n / 16), lds_bytes(n), r->stream, r->rd, ka); \
     break;
-      YH_LOGN_CASES(X)
+        YH_LOGN_CASES(X)
 #undef X
-      default: delete timer; return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
+        default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
+      }
+    }
+    YH_HIP(hipGetLastError());
+    {
+      LaunchTimer timer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
+      dim3 grid(yhk::xcd_grid(batch, Rks));
+      switch (r->rd.logn) {
+#define X(L) case L: hipLaunchKernelGGL((yhk::k_ks_mac<L>), grid, dim3(n / 16), lds_bytes(n), r->stream, r->rd, ka); break;
+        YH_LOGN_CASES(X)
+#undef X
+        default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
+      }
     }
   } else {
-    hipLaunchKernelGGL(yhk::k_keyswitch_small, grid, dim3(64), 0, r->stream, r->rd, ka, k->d);
+    LaunchTimer timer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
+    hipLaunchKernelGGL(yhk::k_keyswitch_small, dim3(Rks, batch), dim3(64), 0, r->stream, r->rd, ka, k->d);
   }
-  delete timer;
   YH_HIP(hipGetLastError());
   yhk::IcrtArgs ia = icrt_args(r, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0);
   ia.out_list = out_list;
@@ -468,12 +482,12 @@ int yh_ctx_reset_timing(yh_ctx* ctx) {
   if (!ctx) return fail(YH_ERR_ARG, "null ctx");
   if (int s = set_dev(ctx)) return s;
   if (int s = drain_timing(ctx)) return s;
-  for (int k = 0; k < 4; ++k) { ctx->total_ms[k] = 0; ctx->launches[k] = 0; }
+  for (int k = 0; k < YH_KERNEL_COUNT; ++k) { ctx->total_ms[k] = 0; ctx->launches[k] = 0; }
   return YH_OK;
 }
 
 int yh_ctx_kernel_time(yh_ctx* ctx, int kernel_id, double* total_ms, long* launches) {
-  if (!ctx || kernel_id < 0 || kernel_id > 3) return fail(YH_ERR_ARG, "bad argument");
+  if (!ctx || kernel_id < 0 || kernel_id >= YH_KERNEL_COUNT) return fail(YH_ERR_ARG, "bad argument");
   if (int s = set_dev(ctx)) return s;
   if (int s = drain_timing(ctx)) return s;
   if (total_ms) *total_ms = ctx->total_ms[kernel_id];
@@ -589,6 +603,7 @@ int yh_ring_destroy(yh_ring* r) {
   if (r->d_ntt) (void)hipFree(r->d_ntt);
   if (r->d_y) (void)hipFree(r->d_y);
   if (r->d_tmp) (void)hipFree(r->d_tmp);
+  if (r->d_dig) (void)hipFree(r->d_dig);
   delete r;
   return YH_OK;
 }
@@ -1474,6 +1489,8 @@ size_t yh_mem_footprint(const yh_ring* r, int op) {
   size_t n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R, Rks = r->rd.set_R[1];
   size_t ct = LQ * n * sizeof(uint64_t);
   size_t y = yhk::y_rows((int)R) * n * sizeof(uint32_t), yks = yhk::y_rows((int)Rks) * n * sizeof(uint32_t);
+  // keyswitch: reconstruction rows plus the digit transforms
+  yks += (size_t)r->rd.lw * n * sizeof(uint64_t);
   switch (op) {
     case 0: return 3 * ct + y;
     case 1: return 3 * ct + y;

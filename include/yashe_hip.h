@@ -300,11 +300,14 @@ int yh_batch_autotune(yh_ctx* ctx, const yh_batch_item* items, int count, const 
  * every launch of the fused kernels (no counterpart in the reference, which
  * times whole operators with clock_gettime, src/benchmark_yashe.cpp:152-291).
  * kernel ids: 0 product (CRT + NTTs + pointwise + INTT), 1 keyswitch
- * multiply-accumulate, 2 ICRT epilogue. */
+ * multiply-accumulate + inverse transform, 2 ICRT epilogue, 3 persistent
+ * batch, 4 keyswitch digit transforms. */
 #define YH_KERNEL_PROD 0
 #define YH_KERNEL_KEYSWITCH 1
 #define YH_KERNEL_ICRT 2
 #define YH_KERNEL_BATCH 3
+#define YH_KERNEL_KS_DIGITS 4
+#define YH_KERNEL_COUNT 5
 int yh_ctx_set_timing(yh_ctx* ctx, int enable);
 /* Synchronises, then returns the summed milliseconds and launch count of
  * kernel `kernel_id` since timing was enabled (or last reset). */

@@ -59,19 +59,34 @@ def test_ring_plan_bounds(n, q, t, wb):
     assert info.R == len(primes) and info.n == n
     # modulus 0 is the Goldilocks prime itself (its convolution is exact mod P
     # for any operands); the small primes follow in descending order, sized
-    # for one product per convolution (K = 1) in the product set and for the
-    # l_w-term multiply-accumulate in the keyswitch set
-    for mods, K in ((primes, 1), (ks_primes, max(1, info.lw))):
+    # for one product of centred residues per convolution in the product set,
+    # and in the keyswitch set for l_w products of an unreduced digit
+    # (< 2^wb, transformed once for every modulus) by a centred key residue
+    def exact_prod(p):
+        return n * ((p - 1) // 2) ** 2 <= (P - 1) // 2
+
+    def exact_ks(p):
+        return max(1, info.lw) * n * ((1 << wb) - 1) * ((p - 1) // 2) <= (P - 1) // 2
+
+    def is_prime(p):
+        return p > 1 and all(p % d for d in range(2, int(p ** 0.5) + 1))
+
+    for mods, exact in ((primes, exact_prod), (ks_primes, exact_ks)):
         if not mods:
             continue
         assert mods[0] == P
         small = mods[1:]
         assert small == sorted(small, reverse=True) and len(set(small)) == len(small)
         for p in small:
-            assert all(p % d for d in range(2, int(p ** 0.5) + 1))
-            # centred residues keep the Goldilocks convolution exact
-            assert K * n * ((p - 1) // 2) ** 2 <= (P - 1) // 2
+            assert is_prime(p)
+            assert exact(p)
             assert p < (1 << 28)
+        if small and small[0] < (1 << 28) - 64:
+            # the first small prime is the largest the bound allows
+            nxt = small[0] + 2
+            while not is_prime(nxt):
+                nxt += 2
+            assert not exact(nxt) or nxt >= (1 << 28)
     M = 1
     for p in primes:
         M *= p

@@ -326,6 +326,45 @@ def test_keyswitch(gpu_ctx, n, q, wb):
     ring.close()
 
 
+def _crt_max_residue(primes, q):
+    """An integer below q whose residue mod each of the leading primes is
+    (p-1)/2, the largest centred residue (as many primes as fit below q)."""
+    x, m = 0, 1
+    for p in primes:
+        if m * p >= q:
+            break
+        # x' = x (mod m), x' = (p-1)/2 (mod p)
+        t = ((p - 1) // 2 - x) * pow(m, -1, p) % p
+        x, m = x + m * t, m * p
+    return x
+
+
+@pytest.mark.parametrize("n,wb", [(16, 32), (4096, 32), (1024, 16)])
+def test_keyswitch_extremes(gpu_ctx, n, wb):
+    # The digit transforms are shared by every keyswitch modulus, so the
+    # multiply-accumulate of unreduced digits must stay inside the Goldilocks
+    # bound: all-(q-1) ciphertexts give digits near 2^wb - 1, and keys whose
+    # residues are the largest centred value make every term of the last
+    # coefficient's negacyclic sum positive.
+    q = Q127
+    rng = random.Random(n + wb)
+    ring = ya.Ring(gpu_ctx, n, q, t=1024, w_bits=wb)
+    lw = ring.lw
+    G = _crt_max_residue(ring.ks_primes()[1:], q)
+    cases = [([q - 1] * n, [[G] * n] * lw),
+             ([q - 1] * n, [[q - 1] * n] * lw),
+             ([q - 1] * n, [rand_poly(rng, n, q) for _ in range(lw)]),
+             ([0] * (n - 1) + [q - 1], [[G] * n] * lw)]
+    for ci, (c, gammas) in enumerate(cases):
+        evk = ring.evk(gpu_ctx.to_device(ring.pack(gammas)))
+        dc = upload_polys(gpu_ctx, ring, [c])
+        out = gpu_ctx.alloc(ring.L * n * 8)
+        ring.keyswitch(evk, dc, out, 1)
+        assert download_polys(out, ring, 1)[0] == cpuref.keyswitch(c, gammas, n, q, wb), f"case {ci}"
+        evk.close()
+    ring.close()
+
+
 def test_mul_keyswitch_fused_equals_steps(gpu_ctx):
     rng = random.Random(13)
     n, q, t = 4096, Q127, 1024
