@@ -97,6 +97,7 @@ _SIGS = {
     "yh_yashe_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_mem_footprint": (ctypes.c_size_t, [_vp, ctypes.c_int]),
     "yh_ctx_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "yh_ctx_set_timing_mask": (ctypes.c_int, [_vp, ctypes.c_uint]),
     "yh_ctx_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_long)]),
     "yh_ctx_reset_timing": (ctypes.c_int, [_vp]),
@@ -265,6 +266,11 @@ class Context:
 
     def set_timing(self, enable: bool):
         check(lib().yh_ctx_set_timing(self.h, 1 if enable else 0), "yh_ctx_set_timing")
+
+    def set_timing_mask(self, kernel_ids):
+        """Time only these kernel ids while timing is on (None: all)."""
+        mask = 0xFFFFFFFF if kernel_ids is None else sum(1 << k for k in kernel_ids)
+        check(lib().yh_ctx_set_timing_mask(self.h, mask), "yh_ctx_set_timing_mask")
 
     def reset_timing(self):
         check(lib().yh_ctx_reset_timing(self.h), "yh_ctx_reset_timing")

@@ -49,6 +49,11 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_KS_WAVES
 #define YH_KS_WAVES 4
 #endif
+// The keyswitch multiply-accumulate (k_ks_mac) loads l_w digit and key
+// transforms per thread before its inverse transform.
+#ifndef YH_KSMAC_WAVES
+#define YH_KSMAC_WAVES 4
+#endif
 
 // Device view of one ring's tables (all pointers are device memory).
 struct RingDev {
@@ -68,10 +73,16 @@ struct RingDev {
   const uint64_t* Mi[2];    // [R][LW]
   const uint64_t* invMi[2]; // [R]
   const uint64_t* M[2];     // [LW]
-  // limb bounds of the reconstruction epilogues: |c| < M[set] has at most
-  // lim_m[set] limbs, t|c| at most lim_scale
-  int lim_m[2];
-  int lim_scale;
+  // Epilogues mod q without the full integer (per set): M_i mod q [R][LQ]
+  // and q - (M mod q) [LQ]; scale-and-round splits of the product set
+  // t M_i = q alpha_i + beta_i: alpha_i mod q and beta_i [R][LQ],
+  // q - beta_M and q - ((alpha_M + 1) mod q) [LQ].
+  const uint64_t* Mq[2];
+  const uint64_t* negMq[2];
+  const uint64_t* alpha;
+  const uint64_t* beta;
+  const uint64_t* negbetaM;
+  const uint64_t* negalpha1;
 };
 
 // threadIdx.x, recomputed after `dep` is ready (see the register note above).
@@ -494,7 +505,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_ks_digits(Rin
 }
 
 template <int LOGN>
-__global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_ks_mac(RingDev rd, KsArgs ka) {
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(RingDev rd, KsArgs ka) {
   extern __shared__ uint64_t lds[];
   constexpr int N = 1 << LOGN;
   constexpr int stride = N >> 4;
@@ -796,27 +807,194 @@ struct IcrtArgs {
   uint64_t* const* out_list;  // optional per-pair output addresses (replace out + pair*out_pstride)
 };
 
+// 32-bit word w of a little-endian limb array.
+__device__ __forceinline__ uint32_t word_of(const uint64_t* a, int w) {
+  return (uint32_t)(a[w >> 1] >> ((w & 1) * 32));
+}
+
+// A (NA 32-bit words) += y * X * 2^(32 OFF), X of NX words; the callers'
+// bounds keep the sum below 2^(32 NA).
+template <int NX, int NA, int OFF>
+__device__ __forceinline__ void mac_words(uint32_t* A, const uint64_t* X, uint32_t y) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int w = 0; w < NX; ++w) {
+    const uint64_t t = (uint64_t)y * word_of(X, w) + A[w + OFF] + c;  // < 2^64
+    A[w + OFF] = (uint32_t)t;
+    c = (uint32_t)(t >> 32);
+  }
+#pragma unroll
+  for (int w = NX + OFF; w < NA; ++w) {
+    const uint64_t t = (uint64_t)A[w] + c;
+    A[w] = (uint32_t)t;
+    c = (uint32_t)(t >> 32);
+  }
+}
+
+// Modulus-0 y and the rounded quotient k = round(sum_i y_i / m_i) of the
+// reconstruction c = sum_i y_i M_i - k M (|c| <= M/4: round to nearest).
+__device__ __forceinline__ uint32_t icrt_quotient(const RingDev& rd, const uint32_t* y, size_t ystride, int set,
+                                                  uint64_t& yP) {
+  const int RS = rd.set_R[set];
+  // modulus 0 = P: floor(y 2^64 / P) = y + floor(y EPS / P) is y + umulhi(y,
+  // EPS) to within two units of 2^-64 (far inside the quotient's error budget)
+  yP = (uint64_t)y[0] | ((uint64_t)y[ystride] << 32);
+  uint64_t flo = yP + gl::umulhi(yP, gl::EPS), fhi = 0;
+  for (int i = 1; i < RS; ++i) {
+    // fixed-point y_i/p_i with 64 fractional bits (y_i < p_i keeps it < 2^64)
+    const uint64_t f = (uint64_t)y[(size_t)(i + 1) * ystride] * rd.barrett[set][i];
+    flo += f;
+    fhi += (flo < f);
+  }
+  return (uint32_t)(fhi + (flo >> 63));
+}
+
+// V = sum_i y_i X_i + k Xk as LQ + 2 limbs (limbs past the value zero), for
+// X_i, Xk below 2^(64 LQ): the reconstruction folded through per-modulus
+// constants (M_i mod q, or the scale-and-round splits of t M_i), never the
+// full integer. Small-prime terms use one 64-bit column sum per 32-bit word
+// (sum_i y_i X_i[w] < 2^64 because sum_i p_i < 2^32, checked when the ring
+// is built); the 64-bit y of modulus 0 and k then go through carrying
+// multiply-accumulates.
+template <int LQ>
+__device__ __forceinline__ void fold_sum(const RingDev& rd, const uint32_t* y, size_t ystride, int set, uint64_t yP,
+                                         const uint64_t* X, const uint64_t* Xk, uint32_t k, uint64_t* V) {
+  constexpr int NX = 2 * LQ, NA = NX + 3;
+  const int RS = rd.set_R[set];
+  uint64_t S[NX];
+#pragma unroll
+  for (int w = 0; w < NX; ++w) S[w] = 0;
+  for (int i = 1; i < RS; ++i) {
+    const uint64_t yi = y[(size_t)(i + 1) * ystride];
+    const uint64_t* Xi = X + (size_t)i * LQ;
+#pragma unroll
+    for (int w = 0; w < NX; ++w) S[w] += yi * word_of(Xi, w);
+  }
+  uint32_t A[NA];
+  uint64_t c = 0;
+#pragma unroll
+  for (int w = 0; w < NX; ++w) {
+    const uint64_t cur = S[w] + c;  // S[w] < 2^64 - 2^33, c < 2^32
+    A[w] = (uint32_t)cur;
+    c = cur >> 32;
+  }
+  A[NX] = (uint32_t)c;
+  A[NX + 1] = (uint32_t)(c >> 32);
+  A[NX + 2] = 0;
+  mac_words<NX, NA, 0>(A, X, (uint32_t)yP);
+  mac_words<NX, NA, 1>(A, X, (uint32_t)(yP >> 32));
+  mac_words<NX, NA, 0>(A, Xk, k);
+#pragma unroll
+  for (int l = 0; l < LQ + 1; ++l) V[l] = (uint64_t)A[2 * l] | ((uint64_t)A[2 * l + 1] << 32);
+  V[LQ + 1] = A[NA - 1];
+}
+
+// c mod q in [0, q) for the centred reconstruction c (set `set`).
+template <int LQ>
+__device__ __forceinline__ void icrt_mod_q(const RingDev& rd, const uint32_t* y, size_t ystride, int set,
+                                           uint64_t* rem) {
+  constexpr int LW = 2 * LQ + 1;
+  constexpr int LA = LW + 1;
+  constexpr int LMU = LW + 2 - LQ;
+  uint64_t yP;
+  const uint32_t k = icrt_quotient(rd, y, ystride, set, yP);
+  uint64_t a[LA], quo[LMU];
+#pragma unroll
+  for (int l = 0; l < LA; ++l) a[l] = 0;
+  // c = sum y_i M_i - k M  ==  sum y_i (M_i mod q) + k (q - M mod q)  (mod q)
+  fold_sum<LQ>(rd, y, ystride, set, yP, rd.Mq[set], rd.negMq[set], k, a);
+  divmod_q<LQ>(a, rd, quo, rem, LQ + 2);
+}
+
 // ICRT of one coefficient plus its epilogue. y points at the operand's first
 // scratch row (rows ystride apart, layout of store_y), o at limb 0 of the
-// output coefficient.
+// output coefficient. EPI_RAW writes the reconstructed integer itself (in
+// [0, M) when `positive`, else centred); the other epilogues need only
+// residues mod q of it and never form it.
 template <int LQ>
 __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, size_t ystride, int set, int mode,
                                           int positive, uint64_t* o, int out_limbs, size_t out_stride) {
   constexpr int LW = 2 * LQ + 1;
+  constexpr int LA = LW + 1;
+  constexpr int LMU = LW + 2 - LQ;
+  if (mode == EPI_MODQ) {
+    uint64_t rem[LQ];
+    icrt_mod_q<LQ>(rd, y, ystride, set, rem);
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = rem[l];
+    return;
+  }
+  if (mode == EPI_DECRYPT) {
+    // m = round(t * [g]_q / q) mod t, [g]_q in [0, q)
+    // (src/yashe.cpp:167-197 on every coefficient)
+    uint64_t rem[LQ], a[LA], quo[LMU];
+    icrt_mod_q<LQ>(rd, y, ystride, set, rem);
+#pragma unroll
+    for (int l = 0; l < LA; ++l) a[l] = 0;
+    uint64_t carry = 0;
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) {
+      uint64_t lo = rem[l] * rd.t, hi = gl::umulhi(rem[l], rd.t);
+      uint64_t s = lo + carry;
+      hi += (s < lo);
+      a[l] = s;
+      carry = hi;
+    }
+    a[LQ] = carry;
+    divmod_q<LQ>(a, rd, quo, rem, LQ + 1);
+    if (twice_gt_q<LQ>(rem, rd.q)) {
+      uint64_t one[LMU];
+#pragma unroll
+      for (int k2 = 0; k2 < LMU; ++k2) one[k2] = k2 == 0;
+      add_n<LMU>(quo, one);
+    }
+    // quo <= t here
+    o[0] = quo[0] == rd.t ? 0 : quo[0];
+    return;
+  }
+  if (mode == EPI_SCALE) {
+    // round(t c / q) mod q for the signed c (src/ciphertext.cpp:55-63
+    // semantics applied to t*c: floor division, non-negative remainder, ties
+    // down). With t M_i = q alpha_i + beta_i and t M = q alpha_M + beta_M:
+    //   t c = q (sum y_i alpha_i - k alpha_M) + S,  S = sum y_i beta_i - k beta_M,
+    // so floor(t c / q) = sum y_i alpha_i - k alpha_M + floor(S / q) and
+    // (t c) mod q = S mod q. S' = S + k q = sum y_i beta_i + k (q - beta_M)
+    // is non-negative: floor(S / q) = floor(S' / q) - k.
+    uint64_t yP;
+    const uint32_t k = icrt_quotient(rd, y, ystride, set, yP);
+    uint64_t a[LA], quo[LMU], rem[LQ];
+#pragma unroll
+    for (int l = 0; l < LA; ++l) a[l] = 0;
+    fold_sum<LQ>(rd, y, ystride, set, yP, rd.beta, rd.negbetaM, k, a);
+    divmod_q<LQ>(a, rd, quo, rem, LQ + 2);  // S' = quo q + rem
+    const uint64_t up = twice_gt_q<LQ>(rem, rd.q) ? 1 : 0;
+    // result = sum y_i alpha_i + k (q - (alpha_M + 1)) + quo + up  (mod q)
+#pragma unroll
+    for (int l = 0; l < LA; ++l) a[l] = 0;
+    fold_sum<LQ>(rd, y, ystride, set, yP, rd.alpha, rd.negalpha1, k, a);
+    uint64_t add[LQ + 2];
+#pragma unroll
+    for (int l = 0; l < LQ + 2; ++l) add[l] = l < LMU ? quo[l] : 0;
+    add_n<LQ + 2>(a, add);
+#pragma unroll
+    for (int l = 0; l < LQ + 2; ++l) add[l] = l == 0 ? up : 0;
+    add_n<LQ + 2>(a, add);
+    divmod_q<LQ>(a, rd, quo, rem, LQ + 2);
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = rem[l];
+    return;
+  }
+  // EPI_RAW: the integer itself.
   const int RS = rd.set_R[set];
   const uint64_t* Mi = rd.Mi[set];
   uint64_t acc[LW];
 #pragma unroll
   for (int l = 0; l < LW; ++l) acc[l] = 0;
-  // modulus 0 = P: 64-bit y (rows 0 and 1); its fixed-point fraction
-  // floor(y 2^64 / P) = y + floor(y EPS / P) is y + umulhi(y, EPS) to within
-  // two units of 2^-64 (far inside the quotient's error budget)
   const uint64_t yP = (uint64_t)y[0] | ((uint64_t)y[ystride] << 32);
   uint64_t flo = yP + gl::umulhi(yP, gl::EPS), fhi = 0;
   mac64<LW>(acc, Mi, yP);
   for (int i = 1; i < RS; ++i) {
     uint64_t yi = y[(size_t)(i + 1) * ystride];
-    // fixed-point y_i/p_i with 64 fractional bits (y_i < p_i keeps it < 2^64)
     uint64_t f = yi * rd.barrett[set][i];
     flo += f;
     fhi += (flo < f);
@@ -841,128 +1019,13 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
   bool neg = acc[LW - 1] >> 63;
   if (positive) {
     if (neg) add_n<LW>(acc, rd.M[set]);
-    neg = false;
   } else if (neg) {
     neg_n<LW>(acc);
   }
-  if (mode == EPI_RAW) {
 #pragma unroll
-    for (int l = 0; l < LW; ++l)
-      if (l < out_limbs) o[(size_t)l * out_stride] = acc[l];
-    for (int l = LW; l < out_limbs; ++l) o[(size_t)l * out_stride] = 0;
-    return;
-  }
-  constexpr int LA = LW + 1;
-  constexpr int LMU = LW + 2 - LQ;
-  uint64_t a[LA], quo[LMU], rem[LQ];
-#pragma unroll
-  for (int l = 0; l < LW; ++l) a[l] = acc[l];
-  a[LW] = 0;
-  if (mode == EPI_MODQ || mode == EPI_DECRYPT) {
-    divmod_q<LQ>(a, rd, quo, rem, rd.lim_m[set]);
-    bool nz = false;
-#pragma unroll
-    for (int l = 0; l < LQ; ++l) nz |= rem[l] != 0;
-    if (neg && nz) {
-      uint64_t qv[LQ];
-#pragma unroll
-      for (int l = 0; l < LQ; ++l) qv[l] = rd.q[l];
-      sub_n<LQ>(qv, rem);
-#pragma unroll
-      for (int l = 0; l < LQ; ++l) rem[l] = qv[l];
-    }
-    if (mode == EPI_MODQ) {
-#pragma unroll
-      for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = rem[l];
-      return;
-    }
-    // decrypt: m = round(t * [g]_q / q) mod t, [g]_q in [0, q)
-    // (src/yashe.cpp:167-197 on every coefficient)
-#pragma unroll
-    for (int l = 0; l < LA; ++l) a[l] = 0;
-    uint64_t carry = 0;
-#pragma unroll
-    for (int l = 0; l < LQ; ++l) {
-      uint64_t lo = rem[l] * rd.t, hi = gl::umulhi(rem[l], rd.t);
-      uint64_t s = lo + carry;
-      hi += (s < lo);
-      a[l] = s;
-      carry = hi;
-    }
-    a[LQ] = carry;
-    divmod_q<LQ>(a, rd, quo, rem, LQ + 1);
-    if (twice_gt_q<LQ>(rem, rd.q)) {
-      uint64_t one[LMU];
-#pragma unroll
-      for (int k2 = 0; k2 < LMU; ++k2) one[k2] = k2 == 0;
-      add_n<LMU>(quo, one);
-    }
-    // quo <= t here
-    o[0] = quo[0] == rd.t ? 0 : quo[0];
-    return;
-  }
-  // EPI_SCALE: round(t * g / q) mod q for the signed g (src/ciphertext.cpp:55-63
-  // semantics applied to t*g: floor division, non-negative remainder, ties down).
-  {
-    uint64_t carry = 0;
-#pragma unroll
-    for (int l = 0; l < LW; ++l) {
-      uint64_t lo = acc[l] * rd.t, hi = gl::umulhi(acc[l], rd.t);
-      uint64_t s = lo + carry;
-      hi += (s < lo);
-      a[l] = s;
-      carry = hi;
-    }
-    a[LW] = carry;
-  }
-  divmod_q<LQ>(a, rd, quo, rem, rd.lim_scale);  // |t g| = quo*q + rem
-  // signed floor quotient Qs and remainder rs in [0, q)
-  bool nz = false;
-#pragma unroll
-  for (int l = 0; l < LQ; ++l) nz |= rem[l] != 0;
-  uint64_t one[LMU];
-#pragma unroll
-  for (int k2 = 0; k2 < LMU; ++k2) one[k2] = k2 == 0;
-  bool qneg = false;
-  if (neg) {
-    qneg = true;  // Qs = -(quo) or -(quo+1)
-    if (nz) {
-      add_n<LMU>(quo, one);
-      uint64_t qv[LQ];
-#pragma unroll
-      for (int l = 0; l < LQ; ++l) qv[l] = rd.q[l];
-      sub_n<LQ>(qv, rem);
-#pragma unroll
-      for (int l = 0; l < LQ; ++l) rem[l] = qv[l];
-    }
-  }
-  if (twice_gt_q<LQ>(rem, rd.q)) {
-    // Qs += 1
-    if (!qneg) {
-      add_n<LMU>(quo, one);
-    } else {
-      sub_n<LMU>(quo, one);  // -(m) + 1 = -(m - 1); m >= 1 here
-    }
-  }
-  // result = Qs mod q
-  uint64_t a2[LA];
-#pragma unroll
-  for (int l = 0; l < LA; ++l) a2[l] = l < LMU ? quo[l] : 0;
-  uint64_t quo2[LMU], r2[LQ];
-  divmod_q<LQ>(a2, rd, quo2, r2, rd.lim_scale - LQ + 1);
-  bool nz2 = false;
-#pragma unroll
-  for (int l = 0; l < LQ; ++l) nz2 |= r2[l] != 0;
-  if (qneg && nz2) {
-    uint64_t qv[LQ];
-#pragma unroll
-    for (int l = 0; l < LQ; ++l) qv[l] = rd.q[l];
-    sub_n<LQ>(qv, r2);
-#pragma unroll
-    for (int l = 0; l < LQ; ++l) r2[l] = qv[l];
-  }
-#pragma unroll
-  for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = r2[l];
+  for (int l = 0; l < LW; ++l)
+    if (l < out_limbs) o[(size_t)l * out_stride] = acc[l];
+  for (int l = LW; l < out_limbs; ++l) o[(size_t)l * out_stride] = 0;
 }
 
 template <int LQ>

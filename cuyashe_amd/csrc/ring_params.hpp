@@ -131,6 +131,21 @@ struct IcrtSet {
   std::vector<uint64_t> Mi_limbs;   // [R][LW]
   std::vector<uint64_t> invMi;      // [R]  (M/p_i)^-1 mod p_i
   std::vector<uint64_t> M_limbs;    // [LW]
+  // Reconstruction mod q without the full integer: c = sum y_i M_i - k M, so
+  // c mod q = (sum y_i (M_i mod q) + k (q - M mod q)) mod q.
+  std::vector<uint64_t> Mq_limbs;   // [R][LQ]: M_i mod q
+  std::vector<uint64_t> negMq_limbs;  // [LQ]: q - (M mod q), in (0, q]
+};
+
+// Scale-and-round tables of the product set: t M_i = q alpha_i + beta_i with
+// beta_i in [0, q), t M = q alpha_M + beta_M. For c = sum y_i M_i - k M,
+//   t c / q = sum y_i alpha_i - k alpha_M + S / q,  S = sum y_i beta_i - k beta_M,
+// so round(t c / q) mod q follows from two short accumulations.
+struct ScaleTables {
+  std::vector<uint64_t> alpha_limbs;  // [R][LQ]: alpha_i mod q
+  std::vector<uint64_t> beta_limbs;   // [R][LQ]: beta_i
+  std::vector<uint64_t> negbetaM;     // [LQ]: q - beta_M, in (0, q]
+  std::vector<uint64_t> negalpha1;    // [LQ]: q - ((alpha_M + 1) mod q), in (0, q]
 };
 
 struct RingParams {
@@ -150,21 +165,31 @@ struct RingParams {
   std::vector<uint64_t> barrett, barrett_ks;  // floor(2^64 / p_i) (1 for P, unused)
   std::vector<uint64_t> pow32, pow32_ks;      // [R][2*LW]: 2^(32k) mod p_i
   IcrtSet full, ks;
+  ScaleTables scale;
   std::vector<uint64_t> q_limbs;    // [LQ]
   std::vector<uint64_t> mu_limbs;   // [LW+2-LQ]: floor(2^(64*(LW+1)) / q)
 
   static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
   // P followed by descending primes <= pm until the product reaches `need`.
-  static std::vector<uint64_t> moduli_for(uint64_t pm, const BigInt& need) {
+  // The reconstruction epilogues sum y_i * (32-bit word) over the small
+  // primes in one 64-bit column, exact while sum_i p_i < 2^32; a set that
+  // would exceed it (very short rings with wide q) takes smaller primes.
+  static std::vector<uint64_t> moduli_for(uint64_t& pm, const BigInt& need) {
     const BigInt Pb(gl::P);
     BigInt need_small = (need + Pb - BigInt(1)) / Pb;
-    std::vector<uint64_t> out(1, gl::P);
-    if (need_small > BigInt(1)) {
-      std::vector<uint64_t> sp = take_primes(pm, need_small);
-      out.insert(out.end(), sp.begin(), sp.end());
+    for (;;) {
+      std::vector<uint64_t> out(1, gl::P);
+      if (need_small > BigInt(1)) {
+        std::vector<uint64_t> sp = take_primes(pm, need_small);
+        out.insert(out.end(), sp.begin(), sp.end());
+      }
+      uint64_t psum = 0;
+      for (size_t i = 1; i < out.size(); ++i) psum += out[i];
+      if (psum < (1ull << 32)) return out;
+      pm = pm / 8 * 7;
+      if (pm < 1024) throw std::runtime_error("no CRT prime set fits the reconstruction bounds");
     }
-    return out;
   }
 
   void tables_for(const std::vector<uint64_t>& pr, std::vector<uint64_t>& bar, std::vector<uint64_t>& pw) {
@@ -184,6 +209,9 @@ struct RingParams {
 
   void build_set(IcrtSet& s, const std::vector<uint64_t>& primes) {
     const int R = (int)primes.size();
+    uint64_t psum = 0;  // bound of the folded epilogues (moduli_for)
+    for (int i = 1; i < R; ++i) psum += primes[i];
+    if (psum >= (1ull << 32)) throw std::logic_error("small CRT primes exceed the folded-epilogue bound");
     s.R = R;
     s.M = BigInt(1);
     for (int i = 0; i < R; ++i) s.M *= BigInt(primes[i]);
@@ -192,12 +220,34 @@ struct RingParams {
     s.invMi.assign(R, 0);
     s.M_limbs.assign(LW, 0);
     s.M.to_limbs(s.M_limbs.data(), LW);
+    s.Mq_limbs.assign((size_t)R * LQ, 0);
     for (int i = 0; i < R; ++i) {
       BigInt Mi = s.M / BigInt(primes[i]);
       Mi.to_limbs(&s.Mi_limbs[(size_t)i * LW], LW);
       const uint64_t r = Mi.mod_u64(primes[i]);
       s.invMi[i] = primes[i] == gl::P ? gl::inv(r) : inv_mod_u64(r, primes[i]);
+      (Mi % q).to_limbs(&s.Mq_limbs[(size_t)i * LQ], LQ);
     }
+    s.negMq_limbs.assign(LQ, 0);
+    (q - s.M % q).to_limbs(s.negMq_limbs.data(), LQ);
+  }
+
+  void build_scale(const IcrtSet& s) {
+    const int R = s.R;
+    scale.alpha_limbs.assign((size_t)R * LQ, 0);
+    scale.beta_limbs.assign((size_t)R * LQ, 0);
+    scale.negbetaM.assign(LQ, 0);
+    scale.negalpha1.assign(LQ, 0);
+    if (t == 0) return;
+    const BigInt tb(t);
+    for (int i = 0; i < R; ++i) {
+      const BigInt tMi = tb * (s.M / BigInt(primes[i]));
+      ((tMi / q) % q).to_limbs(&scale.alpha_limbs[(size_t)i * LQ], LQ);
+      (tMi % q).to_limbs(&scale.beta_limbs[(size_t)i * LQ], LQ);
+    }
+    const BigInt tM = tb * s.M;
+    (q - tM % q).to_limbs(scale.negbetaM.data(), LQ);
+    (q - (tM / q + BigInt(1)) % q).to_limbs(scale.negalpha1.data(), LQ);
   }
 
   // n: ring degree (power of two, x^n + 1); q: modulus (odd, > 1);
@@ -226,6 +276,7 @@ struct RingParams {
     primes = moduli_for(pmax, need);
     tables_for(primes, barrett, pow32);
     build_set(full, primes);
+    build_scale(full);
     if (w_bits) {
       // Keyswitch: |c_j| <= l_w n (w-1)(q-1), l_w products summed per convolution.
       BigInt B_ks = BigInt(lw) * BigInt(n) * (BigInt::pow2(w_bits) - BigInt(1)) * qm1;

@@ -66,6 +66,7 @@ struct yh_ctx {
   std::map<int, DevTables> cyclic;  // N -> twiddles of the cyclic transform
   std::mutex mu;
   bool timing = false;
+  unsigned timing_mask = ~0u;  // kernel ids (bit k = id k) timed while timing is on
   std::vector<TimedLaunch> pending;
   std::vector<hipEvent_t> event_pool;
   double total_ms[YH_KERNEL_COUNT] = {};
@@ -128,7 +129,7 @@ struct LaunchTimer {
   hipStream_t stream;
   hipEvent_t start = nullptr;
   LaunchTimer(yh_ctx* c, int k, hipStream_t s) : ctx(c), kernel(k), stream(s) {
-    if (ctx->timing && !ctx->capturing) {
+    if (ctx->timing && !ctx->capturing && (ctx->timing_mask >> kernel & 1u)) {
       start = take_event(ctx);
       if (start) (void)hipEventRecord(start, stream);
     }
@@ -478,6 +479,14 @@ int yh_ctx_set_timing(yh_ctx* ctx, int enable) {
   return YH_OK;
 }
 
+int yh_ctx_set_timing_mask(yh_ctx* ctx, unsigned mask) {
+  if (!ctx) return fail(YH_ERR_ARG, "null ctx");
+  if (int s = set_dev(ctx)) return s;
+  if (int s = drain_timing(ctx)) return s;
+  ctx->timing_mask = mask;
+  return YH_OK;
+}
+
 int yh_ctx_reset_timing(yh_ctx* ctx) {
   if (!ctx) return fail(YH_ERR_ARG, "null ctx");
   if (int s = set_dev(ctx)) return s;
@@ -566,6 +575,10 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   size_t o_q = put(P.q_limbs), o_mu = put(P.mu_limbs);
   size_t o_mi0 = put(P.full.Mi_limbs), o_inv0 = put(P.full.invMi), o_m0 = put(P.full.M_limbs);
   size_t o_mi1 = put(P.ks.Mi_limbs), o_inv1 = put(P.ks.invMi), o_m1 = put(P.ks.M_limbs);
+  size_t o_mq0 = put(P.full.Mq_limbs), o_nmq0 = put(P.full.negMq_limbs);
+  size_t o_mq1 = put(P.ks.Mq_limbs), o_nmq1 = put(P.ks.negMq_limbs);
+  size_t o_al = put(P.scale.alpha_limbs), o_be = put(P.scale.beta_limbs);
+  size_t o_nbm = put(P.scale.negbetaM), o_na1 = put(P.scale.negalpha1);
   YH_HIP(hipMalloc(&r->d_tables, h.size() * sizeof(uint64_t)));
   YH_HIP(hipMemcpy(r->d_tables, h.data(), h.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   uint64_t* d = r->d_tables;
@@ -581,13 +594,9 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   rd.q = d + o_q; rd.mu = d + o_mu;
   rd.set_R[0] = P.full.R; rd.Mi[0] = d + o_mi0; rd.invMi[0] = d + o_inv0; rd.M[0] = d + o_m0;
   rd.set_R[1] = P.ks.R; rd.Mi[1] = d + o_mi1; rd.invMi[1] = d + o_inv1; rd.M[1] = d + o_m1;
-  rd.lim_m[0] = (P.full.M.bits() + 63) / 64;
-  rd.lim_m[1] = (P.ks.M.bits() + 63) / 64;
-  {
-    int tb = 0;
-    while (tb < 64 && (P.t >> tb)) ++tb;
-    rd.lim_scale = std::min(P.LW + 1, (P.full.M.bits() + tb + 63) / 64);
-  }
+  rd.Mq[0] = d + o_mq0; rd.negMq[0] = d + o_nmq0;
+  rd.Mq[1] = d + o_mq1; rd.negMq[1] = d + o_nmq1;
+  rd.alpha = d + o_al; rd.beta = d + o_be; rd.negbetaM = d + o_nbm; rd.negalpha1 = d + o_na1;
   *out = r.release();
   return YH_OK;
   YH_TRY_END

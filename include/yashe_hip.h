@@ -309,6 +309,10 @@ int yh_batch_autotune(yh_ctx* ctx, const yh_batch_item* items, int count, const 
 #define YH_KERNEL_KS_DIGITS 4
 #define YH_KERNEL_COUNT 5
 int yh_ctx_set_timing(yh_ctx* ctx, int enable);
+/* Restricts timing to the kernels whose bit (1 << kernel_id) is set in
+ * `mask` (default: all), so a measured loop pays for event pairs around the
+ * launches it reports only. */
+int yh_ctx_set_timing_mask(yh_ctx* ctx, unsigned mask);
 /* Synchronises, then returns the summed milliseconds and launch count of
  * kernel `kernel_id` since timing was enabled (or last reset). */
 int yh_ctx_kernel_time(yh_ctx* ctx, int kernel_id, double* total_ms, long* launches);

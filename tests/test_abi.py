@@ -53,6 +53,7 @@ def test_no_gpu_fails_loudly():
     (16384, (1 << 511) - 187, 1024, 32),
     (8, 77287149995192912462927307869, 35951, 32),
     (2, 1171313591017775093490277364417, 0, 0),
+    (16, (1 << 511) - 187, 1024, 32),
 ])
 def test_ring_plan_bounds(n, q, t, wb):
     info, primes, ks_primes = ya.ring_plan(n, q, t, wb, keyswitch=True)
@@ -81,7 +82,10 @@ def test_ring_plan_bounds(n, q, t, wb):
             assert is_prime(p)
             assert exact(p)
             assert p < (1 << 28)
-        if small and small[0] < (1 << 28) - 64:
+        # the reconstruction epilogues sum y_i * (32-bit word) over the small
+        # primes in one 64-bit column
+        assert sum(small) < (1 << 32)
+        if small and small[0] < (1 << 28) - 64 and sum(small) < (1 << 31):
             # the first small prime is the largest the bound allows
             nxt = small[0] + 2
             while not is_prime(nxt):
