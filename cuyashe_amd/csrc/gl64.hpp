@@ -77,6 +77,46 @@ GL_HD uint64_t mul(uint64_t a, uint64_t b) {
   return reduce128(hi, lo);
 }
 
+// x * 2^e mod P for canonical x, canonical result. Meant for exponents
+// known at compile time (the twiddles inside the 16-, 8-, 4- and 2-point
+// DFTs): after inlining only one branch remains. 2^96 = -1 and 2^192 = 1, so
+// e reduces to (0, 96) with a sign; a negated product negates x first
+// (P - x; x = 0 gives P, which the reductions below map to 0).
+//  * 0 < e <= 32:  x 2^e = lo + hi 2^64 with hi < 2^32, and 2^64 = EPS.
+//  * 64 <= e < 96: with k = 96 - e <= 32, x = xh 2^k + xl,
+//      x 2^e = -x 2^-k = -(xh - xl 2^(96-k)) = zh EPS - xh,
+//    zh = xl 2^(32-k) < 2^32 (xl 2^(96-k) = zh 2^64): one subtraction of
+//    canonical values instead of a 128-bit reduction.
+//  * 32 < e < 64:  with s = 64 - e and y = y1 2^32 + y0, y0 2^e = h 2^32 +
+//    (y0 >> s) 2^64 (h = y0 2^(32-s) mod 2^32) and, by the case above,
+//    y1 2^(e+32) = (y1 2^(32-s) mod 2^32) EPS - (y1 >> s); the two EPS
+//    factors are disjoint bit fields of m = bits [s, s+32) of y, so
+//    y 2^e = h 2^32 + m EPS - (y1 >> s), all three terms canonical.
+GL_HD uint64_t mul_pow2(uint64_t x, int e) {
+  e %= 192;
+  if (e < 0) e += 192;
+  if (e == 0) return x;
+  if (e == 96) return neg(x);
+  const bool ng = e > 96;
+  if (ng) e -= 96;
+  if (e >= 64) {
+    const int k = 96 - e;
+    const uint64_t zh = (uint64_t)(uint32_t)((uint32_t)x << (32 - k));
+    const uint64_t a = (zh << 32) - zh;  // zh * EPS < P
+    const uint64_t xh = x >> k;           // < 2^(64-k) < P
+    return ng ? sub(xh, a) : sub(a, xh);
+  }
+  const uint64_t y = ng ? P - x : x;
+  if (e > 32) {
+    const int s = 64 - e;
+    const uint64_t h = (uint64_t)(uint32_t)((uint32_t)y << (32 - s)) << 32;  // <= P - 1
+    const uint64_t m = (uint64_t)(uint32_t)(y >> s);
+    const uint64_t r = (y >> 32) >> s;
+    return sub(add(h, (m << 32) - m), r);
+  }
+  return reduce128(y >> (64 - e), y << e);
+}
+
 GL_HD uint64_t pow(uint64_t b, uint64_t e) {
   uint64_t r = 1;
   while (e) {

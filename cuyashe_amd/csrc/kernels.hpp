@@ -478,8 +478,9 @@ __device__ __forceinline__ uint64_t digit_of(const uint64_t* a, int n, int lq, i
 }
 
 // W32: digits are the 32-bit words of the limbs (w = 2^32), read directly.
-// Output layout: the transform's output permutation, thread g's 16 values
-// contiguous (the layout of the resident keys).
+// Output layout (shared with the resident keys): slot c of thread g at
+// c * n/16 + g, so each store or load instruction of a wavefront covers
+// 512 contiguous bytes.
 template <int LOGN, bool W32>
 __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_ks_digits(RingDev rd, KsArgs ka) {
   extern __shared__ uint64_t lds[];
@@ -499,9 +500,9 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_ks_digits(Rin
     x[c] = mul_small(d, rd.tab.twist[j]);
   }
   ntt16_fwd<LOGN>(x, lds, rd.tab.fwd, g);
-  uint64_t* o = ka.D + ((size_t)ct * rd.lw + k) * N + (size_t)g * E;
+  uint64_t* o = ka.D + ((size_t)ct * rd.lw + k) * N + g;
 #pragma unroll
-  for (int c = 0; c < E; ++c) o[c] = x[c];
+  for (int c = 0; c < E; ++c) o[c * stride] = x[c];
 }
 
 template <int LOGN>
@@ -513,16 +514,16 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
   int ct, i;
   if (!xcd_map(blockIdx.x, Rks, ka.batch, ct, i)) return;
   const int g = threadIdx.x;
-  const uint64_t* dg = ka.D + (size_t)ct * rd.lw * N + (size_t)g * E;
-  const uint64_t* ev = ka.evk + (size_t)i * N + (size_t)g * E;
+  const uint64_t* dg = ka.D + (size_t)ct * rd.lw * N + g;
+  const uint64_t* ev = ka.evk + (size_t)i * N + g;
   uint64_t acc[E];
 #pragma unroll
-  for (int c = 0; c < E; ++c) acc[c] = gl::mul(dg[c], ev[c]);
+  for (int c = 0; c < E; ++c) acc[c] = gl::mul(dg[c * stride], ev[c * stride]);
   for (int k = 1; k < rd.lw; ++k) {
     dg += N;
     ev += (size_t)Rks * N;
 #pragma unroll
-    for (int c = 0; c < E; ++c) acc[c] = gl::add(acc[c], gl::mul(dg[c], ev[c]));
+    for (int c = 0; c < E; ++c) acc[c] = gl::add(acc[c], gl::mul(dg[c * stride], ev[c * stride]));
   }
   const int g2 = opaque_tid(acc[0]);
   ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g2);
@@ -590,7 +591,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_evk_prep(Rin
   else load_crt<LOGN, true>(x, g, rd.LQ, N, N, pw, p, b, rd.tab.twist);
   ntt16_fwd<LOGN>(x, lds, rd.tab.fwd);
 #pragma unroll
-  for (int c = 0; c < E; ++c) o[threadIdx.x * E + c] = x[c];
+  for (int c = 0; c < E; ++c) o[c * (N >> 4) + threadIdx.x] = x[c];
 }
 
 // Keyswitch keys of small rings (n < 16): centred residues, no transform.

@@ -53,26 +53,8 @@ __host__ __device__ constexpr int pass_offset(int logn, int p) {
   return off;
 }
 
-// x * 2^e mod P; e is a compile-time constant after unrolling, so the
-// branches fold away.
-__device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int e) {
-  e %= 192;
-  if (e == 0) return x;
-  const bool neg = e >= 96;  // 2^96 = -1
-  if (neg) e -= 96;
-  uint64_t r;
-  if (e == 0) {
-    r = x;
-  } else if (e < 64) {
-    r = gl::reduce128(x >> (64 - e), x << e);
-  } else {
-    // 2^e = 2^(e-64) * 2^64 and 2^64 = 2^32 - 1
-    const int f = e - 64;
-    const uint64_t t = f == 0 ? x : gl::reduce128(x >> (64 - f), x << f);
-    r = gl::sub(gl::reduce128(t >> 32, t << 32), t);
-  }
-  return neg ? gl::neg(r) : r;
-}
+// x * 2^e mod P for a compile-time e (gl::mul_pow2).
+__device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int e) { return gl::mul_pow2(x, e); }
 
 // a * b mod P for a < 2^32.
 __device__ __forceinline__ uint64_t mul_small(uint64_t a, uint64_t b) {
