@@ -10,4 +10,6 @@ from ._lib import (  # noqa: F401
     ints_to_array, array_to_ints, int_to_limbs, ring_plan,
     OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD,
     KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH, KERNEL_KS_DIGITS,
+    WIRE_CIPHERTEXT, WIRE_KEYSWITCH_KEY, WIRE_PUBLIC_KEY, WIRE_SECRET_KEY, WIRE_PLAINTEXT,
+    wire_encode, wire_decode, wire_peek, wire_blob_size,
 )

@@ -24,6 +24,7 @@ YH_SUB = 1
 
 OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD = range(5)
 KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH, KERNEL_KS_DIGITS = range(5)
+WIRE_CIPHERTEXT, WIRE_KEYSWITCH_KEY, WIRE_PUBLIC_KEY, WIRE_SECRET_KEY, WIRE_PLAINTEXT = range(1, 6)
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
@@ -39,6 +40,15 @@ class RingInfo(ctypes.Structure):
         ("R_ks", ctypes.c_int), ("w_bits", ctypes.c_int), ("lw", ctypes.c_int), ("work_limbs", ctypes.c_int),
         ("m_bits", ctypes.c_int), ("m_ks_bits", ctypes.c_int), ("t", ctypes.c_uint64), ("prime_max", ctypes.c_uint64),
     ]
+
+
+class WireHeader(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_int), ("kind", ctypes.c_int), ("n", ctypes.c_int), ("q_limbs", ctypes.c_int),
+                ("t", ctypes.c_uint64), ("w_bits", ctypes.c_int), ("count", ctypes.c_int),
+                ("q", ctypes.c_uint64 * 8)]
+
+    def modulus(self) -> int:
+        return sum(int(self.q[l]) << (64 * l) for l in range(self.q_limbs))
 
 
 class BatchItem(ctypes.Structure):
@@ -96,6 +106,12 @@ _SIGS = {
     "yh_yashe_mul_keyswitch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_yashe_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_mem_footprint": (ctypes.c_size_t, [_vp, ctypes.c_int]),
+    "yh_wire_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "yh_wire_encode": (ctypes.c_int, [ctypes.c_int, _u64p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                      _u64p, ctypes.c_int, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "yh_wire_decode": (ctypes.c_int, [ctypes.c_int, _u64p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                      _vp, ctypes.c_size_t, _u64p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "yh_wire_peek": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(WireHeader)]),
     "yh_ctx_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "yh_ctx_set_timing_mask": (ctypes.c_int, [_vp, ctypes.c_uint]),
     "yh_ctx_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -194,6 +210,56 @@ def ring_plan(n: int, q: int, t: int = 0, w_bits: int = 0, keyswitch: bool = Fal
     if keyswitch:
         return info, prod, [int(buf[info.R + i]) for i in range(info.R_ks)]
     return info, prod
+
+
+# ------------------------------------------------------------------ wire format
+
+def wire_encode(n: int, q: int, t: int, w_bits: int, kind: int, polys: np.ndarray) -> bytes:
+    """Blob of `polys` ([count][L][n] uint64 limbs, or [L][n] for one) for
+    the ring (n, q, t, w_bits); see yh_wire_encode."""
+    q = int(q)
+    L = max(1, (q.bit_length() + 63) // 64)
+    arr = np.ascontiguousarray(polys, dtype=np.uint64)
+    if arr.ndim == 2:
+        arr = arr[None]
+    if arr.shape[1:] != (L, n):
+        raise YashError(f"expected [count][{L}][{n}] limbs, got {arr.shape}")
+    ql = (ctypes.c_uint64 * L)(*int_to_limbs(q, L))
+    size = lib().yh_wire_size(n, L, arr.shape[0])
+    out = ctypes.create_string_buffer(size)
+    got = ctypes.c_size_t(0)
+    check(lib().yh_wire_encode(n, ql, L, t, w_bits, kind, arr.ctypes.data_as(_u64p), arr.shape[0], out, size,
+                               ctypes.byref(got)), "yh_wire_encode")
+    return out.raw[:got.value]
+
+
+def wire_peek(data: bytes) -> WireHeader:
+    h = WireHeader()
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    check(lib().yh_wire_peek(buf, len(data), ctypes.byref(h)), "yh_wire_peek")
+    return h
+
+
+def wire_blob_size(data: bytes, offset: int = 0) -> int:
+    """Size of the blob starting at `offset` (blobs can be concatenated)."""
+    h = wire_peek(bytes(data[offset:offset + 96]))
+    return int(lib().yh_wire_size(h.n, h.q_limbs, h.count))
+
+
+def wire_decode(n: int, q: int, t: int, w_bits: int, kind: int, data: bytes) -> np.ndarray:
+    """[count][L][n] uint64 limbs from a blob, after every check of
+    yh_wire_decode (ring parameters, kind, length, checksum, ranges)."""
+    q = int(q)
+    L = max(1, (q.bit_length() + 63) // 64)
+    ql = (ctypes.c_uint64 * L)(*int_to_limbs(q, L))
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    count = ctypes.c_int(0)
+    check(lib().yh_wire_decode(n, ql, L, t, w_bits, kind, buf, len(data), None, 0, ctypes.byref(count)),
+          "yh_wire_decode")
+    out = np.empty((count.value, L, n), dtype=np.uint64)
+    check(lib().yh_wire_decode(n, ql, L, t, w_bits, kind, buf, len(data), out.ctypes.data_as(_u64p), count.value,
+                               ctypes.byref(count)), "yh_wire_decode")
+    return out
 
 
 class DeviceBuffer:
@@ -399,6 +465,19 @@ class Ring:
         buf = (ctypes.c_uint64 * (self.R + self.R_ks))()
         check(lib().yh_ring_get_primes(self.h, buf, self.R + self.R_ks), "yh_ring_get_primes")
         return [int(v) for v in buf][self.R:]
+
+    def export(self, kind: int, buf: "DeviceBuffer", count: int) -> bytes:
+        """Blob (wire format) of `count` device polynomials of this ring."""
+        return wire_encode(self.n, self.q, self.t, self.w_bits, kind, buf.download((count, self.L, self.n)))
+
+    def import_(self, kind: int, data: bytes, out: Optional["DeviceBuffer"] = None) -> "DeviceBuffer":
+        """Device polynomials from a blob of this ring (checked on the host
+        before anything is uploaded)."""
+        arr = wire_decode(self.n, self.q, self.t, self.w_bits, kind, data)
+        if out is None:
+            return self.ctx.to_device(arr)
+        out.upload(arr)
+        return out
 
     def modulus(self, ks: bool = False) -> int:
         buf = (ctypes.c_uint64 * self.LW)()

@@ -18,6 +18,7 @@
 #include "batch.hpp"
 #include "kernels.hpp"
 #include "ring_params.hpp"
+#include "wire.hpp"
 
 namespace {
 
@@ -1490,6 +1491,58 @@ int yh_batch_autotune(yh_ctx* ctx, const yh_batch_item* items, int count, const 
   if (rc) return rc;
   *out = best;
   cache_store(cache_path, key, best, best_ms);
+  return YH_OK;
+}
+
+static int wire_params(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, yhw::Params* p) {
+  if (!q || q_limbs < 1 || q_limbs > yhw::MAX_LIMBS) return fail(YH_ERR_ARG, "q must have 1 to 8 limbs");
+  p->n = n;
+  p->L = q_limbs;
+  for (int l = 0; l < q_limbs; ++l) p->q[l] = q[l];
+  p->t = t;
+  p->w_bits = w_bits;
+  std::string e = yhw::check_params(*p);
+  if (!e.empty()) return fail(YH_ERR_ARG, e.c_str());
+  return YH_OK;
+}
+
+size_t yh_wire_size(int n, int q_limbs, int count) {
+  if (n < 1 || q_limbs < 1 || count < 0) return 0;
+  return yhw::encoded_size(n, q_limbs, count);
+}
+
+int yh_wire_encode(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, int kind, const uint64_t* h_src,
+                   int count, void* out, size_t cap, size_t* len) {
+  yhw::Params p;
+  if (int st = wire_params(n, q, q_limbs, t, w_bits, &p)) return st;
+  if (out && count > 0 && !h_src) return fail(YH_ERR_ARG, "null source");
+  std::string e = yhw::encode(p, kind, kind == YH_WIRE_PLAINTEXT, h_src, count, (uint8_t*)out, cap, len);
+  if (!e.empty()) return fail(YH_ERR_ARG, e.c_str());
+  return YH_OK;
+}
+
+int yh_wire_decode(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, int kind, const void* in,
+                   size_t len, uint64_t* h_dst, int cap, int* count) {
+  yhw::Params p;
+  if (int st = wire_params(n, q, q_limbs, t, w_bits, &p)) return st;
+  std::string e = yhw::decode(p, kind, kind == YH_WIRE_PLAINTEXT, (const uint8_t*)in, len, h_dst, cap, count);
+  if (!e.empty()) return fail(YH_ERR_ARG, e.c_str());
+  return YH_OK;
+}
+
+int yh_wire_peek(const void* in, size_t len, yh_wire_header* hdr) {
+  if (!hdr) return fail(YH_ERR_ARG, "null header");
+  yhw::Header h;
+  std::string e = yhw::peek((const uint8_t*)in, len, &h);
+  if (!e.empty()) return fail(YH_ERR_ARG, e.c_str());
+  hdr->version = h.version;
+  hdr->kind = h.kind;
+  hdr->n = h.p.n;
+  hdr->q_limbs = h.p.L;
+  hdr->t = h.p.t;
+  hdr->w_bits = h.p.w_bits;
+  hdr->count = h.count;
+  for (int l = 0; l < 8; ++l) hdr->q[l] = l < h.p.L ? h.p.q[l] : 0;
   return YH_OK;
 }
 

@@ -21,7 +21,8 @@ from typing import Optional
 
 import numpy as np
 
-from ._lib import Context, DeviceBuffer, Ring, YashError, ints_to_array
+from ._lib import (Context, DeviceBuffer, Ring, YashError, ints_to_array, WIRE_CIPHERTEXT, WIRE_KEYSWITCH_KEY,
+                   WIRE_PUBLIC_KEY, WIRE_SECRET_KEY, wire_blob_size, wire_peek)
 
 GAUSS_SIGMA = 8.0   # src/distribution.h: discrete Gaussian with sigma 8
 GAUSS_BOUND = 48    # truncated at 6 sigma
@@ -103,6 +104,51 @@ class Yashe:
             self.evk = ring.evk(gam)
             self.gamma = gam
         return self
+
+    # ------------------------------------------------------------ key files
+    def save_keys(self, path: str, secret: bool = True):
+        """Writes h, the keyswitch key gamma_0..gamma_{lw-1} and (secret=True)
+        f as concatenated wire-format blobs of this ring."""
+        ring = self.ring
+        blobs = [ring.export(WIRE_PUBLIC_KEY, self.h, 1)]
+        if self.lw:
+            blobs.append(ring.export(WIRE_KEYSWITCH_KEY, self.gamma, self.lw))
+        if secret:
+            blobs.append(ring.export(WIRE_SECRET_KEY, self.f, 1))
+        with open(path, "wb") as fh:
+            for b in blobs:
+                fh.write(b)
+
+    def load_keys(self, path: str):
+        """Reads a key file written by save_keys for the same ring: h, the
+        resident keyswitch key (rebuilt from gamma_k) and f if present."""
+        with open(path, "rb") as fh:
+            data = fh.read()
+        off = 0
+        ring = self.ring
+        while off < len(data):
+            size = wire_blob_size(data, off)
+            blob = data[off:off + size]
+            kind = wire_peek(blob[:96]).kind
+            if kind == WIRE_PUBLIC_KEY:
+                self.h = ring.import_(WIRE_PUBLIC_KEY, blob)
+            elif kind == WIRE_KEYSWITCH_KEY:
+                self.gamma = ring.import_(WIRE_KEYSWITCH_KEY, blob)
+                self.evk = ring.evk(self.gamma)
+            elif kind == WIRE_SECRET_KEY:
+                self.f = ring.import_(WIRE_SECRET_KEY, blob)
+                self.ff = self.ctx.alloc(self.elt_bytes)
+                ring.ring_mul(self.f, self.f, self.ff, 1)
+            else:
+                raise YashError(f"unexpected element kind {kind} in key file")
+            off += size
+        return self
+
+    def export_ciphertexts(self, c: DeviceBuffer, batch: int) -> bytes:
+        return self.ring.export(WIRE_CIPHERTEXT, c, batch)
+
+    def import_ciphertexts(self, data: bytes) -> DeviceBuffer:
+        return self.ring.import_(WIRE_CIPHERTEXT, data)
 
     # ------------------------------------------------------------ ciphertexts
     def encrypt(self, m: np.ndarray, out: Optional[DeviceBuffer] = None) -> DeviceBuffer:

@@ -318,6 +318,48 @@ int yh_ctx_set_timing_mask(yh_ctx* ctx, unsigned mask);
 int yh_ctx_kernel_time(yh_ctx* ctx, int kernel_id, double* total_ms, long* launches);
 int yh_ctx_reset_timing(yh_ctx* ctx);
 
+/* ------------------------------------------------------------ wire format
+ * Ring elements as bytes, for clients of a serving process and for keeping
+ * keys across runs. No counterpart in the reference, whose only serialisation
+ * is the Python 2 to_JSON of experiments/modq/polynomial.py:158-165.
+ * Little-endian: "YHW1", u16 version (1), u16 kind, u32 n, u32 q_limbs,
+ * u64 t, u32 w_bits, u32 count, q (q_limbs x u64), then count x q_limbs x n
+ * u64 coefficients in the device layout ([count][L][n]), then the CRC-32
+ * (IEEE, as zlib.crc32) of every preceding byte. Host memory on both sides:
+ * move data to and from the device with yh_memcpy_*. Encoding refuses
+ * coefficients outside [0, q) ([0, t) for plaintexts); decoding checks the
+ * ring parameters (n, q, t, w_bits) against the caller's, the kind (unless
+ * 0), the exact length, the checksum and every coefficient's range. The
+ * resident keyswitch key is stored as its coefficient-domain gamma_k
+ * (YH_WIRE_KEYSWITCH_KEY, l_w polynomials) and rebuilt with yh_evk_create. */
+#define YH_WIRE_CIPHERTEXT 1
+#define YH_WIRE_KEYSWITCH_KEY 2
+#define YH_WIRE_PUBLIC_KEY 3
+#define YH_WIRE_SECRET_KEY 4
+#define YH_WIRE_PLAINTEXT 5
+typedef struct yh_wire_header {
+  int version;
+  int kind;
+  int n;
+  int q_limbs;
+  uint64_t t;
+  int w_bits;
+  int count;
+  uint64_t q[8];
+} yh_wire_header;
+/* Bytes of a blob holding `count` polynomials. */
+size_t yh_wire_size(int n, int q_limbs, int count);
+/* Writes `count` polynomials from h_src ([count][q_limbs][n]) into out;
+ * *len receives the blob size (out NULL: size query only). */
+int yh_wire_encode(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, int kind, const uint64_t* h_src,
+                   int count, void* out, size_t cap, size_t* len);
+/* Reads a blob into h_dst ([cap][q_limbs][n]); *count receives the number of
+ * polynomials (h_dst NULL: validate and count only). */
+int yh_wire_decode(int n, const uint64_t* q, int q_limbs, uint64_t t, int w_bits, int kind, const void* in,
+                   size_t len, uint64_t* h_dst, int cap, int* count);
+/* Header of a blob (to pick the ring or size buffers before decoding). */
+int yh_wire_peek(const void* in, size_t len, yh_wire_header* hdr);
+
 /* ------------------------------------------------------------ batching helpers */
 /* Device bytes one operation of `op` (0 ring_mul, 1 yashe_mul, 2 keyswitch,
  * 3 mul+keyswitch, 4 add) needs per batch element, inputs and outputs
