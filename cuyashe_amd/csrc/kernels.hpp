@@ -832,45 +832,57 @@ __device__ __forceinline__ void mac_words(uint32_t* A, const uint64_t* X, uint32
   }
 }
 
-// Modulus-0 y and the rounded quotient k = round(sum_i y_i / m_i) of the
-// reconstruction c = sum_i y_i M_i - k M (|c| <= M/4: round to nearest).
-__device__ __forceinline__ uint32_t icrt_quotient(const RingDev& rd, const uint32_t* y, size_t ystride, int set,
-                                                  uint64_t& yP) {
+// One pass over the moduli of set `set` for one coefficient: the 64-bit y of
+// modulus 0, the rounded quotient k = round(sum_i y_i / m_i) of the
+// reconstruction c = sum_i y_i M_i - k M (|c| <= M/4: round to nearest), and
+// for each of the NF tables X[f] ([R][LQ] constants below 2^(64 LQ)) the
+// column sums S[f][w] = sum_{i>=1} y_i X[f]_i[w] over the 32-bit words w
+// (each < 2^64 because sum_i p_i < 2^32, which the prime choice guarantees).
+// The y rows are loaded eight at a time so the loads overlap.
+template <int LQ, int NF>
+__device__ __forceinline__ uint32_t icrt_pass(const RingDev& rd, const uint32_t* y, size_t ystride, int set,
+                                              const uint64_t* const* X, uint64_t (&S)[NF][2 * LQ], uint64_t& yP) {
+  constexpr int NX = 2 * LQ;
   const int RS = rd.set_R[set];
+  const uint64_t* bar = rd.barrett[set];
   // modulus 0 = P: floor(y 2^64 / P) = y + floor(y EPS / P) is y + umulhi(y,
   // EPS) to within two units of 2^-64 (far inside the quotient's error budget)
   yP = (uint64_t)y[0] | ((uint64_t)y[ystride] << 32);
   uint64_t flo = yP + gl::umulhi(yP, gl::EPS), fhi = 0;
-  for (int i = 1; i < RS; ++i) {
-    // fixed-point y_i/p_i with 64 fractional bits (y_i < p_i keeps it < 2^64)
-    const uint64_t f = (uint64_t)y[(size_t)(i + 1) * ystride] * rd.barrett[set][i];
-    flo += f;
-    fhi += (flo < f);
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int w = 0; w < NX; ++w) S[f][w] = 0;
+  for (int i0 = 1; i0 < RS; i0 += 8) {
+    uint64_t yy[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) yy[k] = i0 + k < RS ? y[(size_t)(i0 + k + 1) * ystride] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = i0 + k;
+      if (i >= RS) break;
+      // fixed-point y_i/p_i with 64 fractional bits (y_i < p_i keeps it < 2^64)
+      const uint64_t fr = yy[k] * bar[i];
+      flo += fr;
+      fhi += (flo < fr);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const uint64_t* Xi = X[f] + (size_t)i * LQ;
+#pragma unroll
+        for (int w = 0; w < NX; ++w) S[f][w] += yy[k] * word_of(Xi, w);
+      }
+    }
   }
   return (uint32_t)(fhi + (flo >> 63));
 }
 
-// V = sum_i y_i X_i + k Xk as LQ + 2 limbs (limbs past the value zero), for
-// X_i, Xk below 2^(64 LQ): the reconstruction folded through per-modulus
-// constants (M_i mod q, or the scale-and-round splits of t M_i), never the
-// full integer. Small-prime terms use one 64-bit column sum per 32-bit word
-// (sum_i y_i X_i[w] < 2^64 because sum_i p_i < 2^32, checked when the ring
-// is built); the 64-bit y of modulus 0 and k then go through carrying
-// multiply-accumulates.
+// V = sum_i y_i X_i + k Xk as LQ + 2 limbs from the column sums S of the
+// small-prime terms (icrt_pass): normalise the columns, then add the 64-bit
+// y of modulus 0 times X_0 and k times Xk with carrying multiply-accumulates.
 template <int LQ>
-__device__ __forceinline__ void fold_sum(const RingDev& rd, const uint32_t* y, size_t ystride, int set, uint64_t yP,
-                                         const uint64_t* X, const uint64_t* Xk, uint32_t k, uint64_t* V) {
+__device__ __forceinline__ void fold_finish(const uint64_t (&S)[2 * LQ], const uint64_t* X, const uint64_t* Xk,
+                                            uint64_t yP, uint32_t k, uint64_t* V) {
   constexpr int NX = 2 * LQ, NA = NX + 3;
-  const int RS = rd.set_R[set];
-  uint64_t S[NX];
-#pragma unroll
-  for (int w = 0; w < NX; ++w) S[w] = 0;
-  for (int i = 1; i < RS; ++i) {
-    const uint64_t yi = y[(size_t)(i + 1) * ystride];
-    const uint64_t* Xi = X + (size_t)i * LQ;
-#pragma unroll
-    for (int w = 0; w < NX; ++w) S[w] += yi * word_of(Xi, w);
-  }
   uint32_t A[NA];
   uint64_t c = 0;
 #pragma unroll
@@ -890,20 +902,21 @@ __device__ __forceinline__ void fold_sum(const RingDev& rd, const uint32_t* y, s
   V[LQ + 1] = A[NA - 1];
 }
 
-// c mod q in [0, q) for the centred reconstruction c (set `set`).
+// c mod q in [0, q) for the centred reconstruction c (set `set`):
+// c = sum y_i M_i - k M  ==  sum y_i (M_i mod q) + k (q - M mod q)  (mod q).
 template <int LQ>
 __device__ __forceinline__ void icrt_mod_q(const RingDev& rd, const uint32_t* y, size_t ystride, int set,
                                            uint64_t* rem) {
   constexpr int LW = 2 * LQ + 1;
   constexpr int LA = LW + 1;
   constexpr int LMU = LW + 2 - LQ;
-  uint64_t yP;
-  const uint32_t k = icrt_quotient(rd, y, ystride, set, yP);
+  uint64_t yP, S[1][2 * LQ];
+  const uint64_t* X[1] = {rd.Mq[set]};
+  const uint32_t k = icrt_pass<LQ, 1>(rd, y, ystride, set, X, S, yP);
   uint64_t a[LA], quo[LMU];
 #pragma unroll
   for (int l = 0; l < LA; ++l) a[l] = 0;
-  // c = sum y_i M_i - k M  ==  sum y_i (M_i mod q) + k (q - M mod q)  (mod q)
-  fold_sum<LQ>(rd, y, ystride, set, yP, rd.Mq[set], rd.negMq[set], k, a);
+  fold_finish<LQ>(S[0], rd.Mq[set], rd.negMq[set], yP, k, a);
   divmod_q<LQ>(a, rd, quo, rem, LQ + 2);
 }
 
@@ -961,18 +974,19 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
     // so floor(t c / q) = sum y_i alpha_i - k alpha_M + floor(S / q) and
     // (t c) mod q = S mod q. S' = S + k q = sum y_i beta_i + k (q - beta_M)
     // is non-negative: floor(S / q) = floor(S' / q) - k.
-    uint64_t yP;
-    const uint32_t k = icrt_quotient(rd, y, ystride, set, yP);
+    uint64_t yP, S[2][2 * LQ];
+    const uint64_t* X[2] = {rd.beta, rd.alpha};
+    const uint32_t k = icrt_pass<LQ, 2>(rd, y, ystride, set, X, S, yP);
     uint64_t a[LA], quo[LMU], rem[LQ];
 #pragma unroll
     for (int l = 0; l < LA; ++l) a[l] = 0;
-    fold_sum<LQ>(rd, y, ystride, set, yP, rd.beta, rd.negbetaM, k, a);
+    fold_finish<LQ>(S[0], rd.beta, rd.negbetaM, yP, k, a);
     divmod_q<LQ>(a, rd, quo, rem, LQ + 2);  // S' = quo q + rem
     const uint64_t up = twice_gt_q<LQ>(rem, rd.q) ? 1 : 0;
     // result = sum y_i alpha_i + k (q - (alpha_M + 1)) + quo + up  (mod q)
 #pragma unroll
     for (int l = 0; l < LA; ++l) a[l] = 0;
-    fold_sum<LQ>(rd, y, ystride, set, yP, rd.alpha, rd.negalpha1, k, a);
+    fold_finish<LQ>(S[1], rd.alpha, rd.negalpha1, yP, k, a);
     uint64_t add[LQ + 2];
 #pragma unroll
     for (int l = 0; l < LQ + 2; ++l) add[l] = l < LMU ? quo[l] : 0;
