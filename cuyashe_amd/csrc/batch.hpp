@@ -126,11 +126,11 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   xsync(wave);
   ntt16_inv_rt(xa, my, tab.inv, g, logn, wave);
   if (active) {
-    const uint64_t invM = rd.invMi[0][i];
+
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int j = g + c * T;
-      store_y(in.Y, N, i, j, finish_any(gold, gl::mul(xa[c], tab.untwist[j]), p, b, invM));
+      store_y(in.Y, N, i, j, finish_any(rd, 0, i, gl::mul(xa[c], tab.untwist[j])));
     }
   }
   // Publish the residues: every wave waits until its stores have reached L2

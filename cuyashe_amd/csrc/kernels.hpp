@@ -23,6 +23,7 @@
 #include <stdint.h>
 
 #include "gl64.hpp"
+#include "modp.hpp"
 #include "ntt16.hpp"
 
 namespace yhk {
@@ -72,6 +73,7 @@ struct RingDev {
   const uint64_t* pow32[2];    // [R][2*LW]: 2^(32k) mod p_i
   const uint64_t* Mi[2];    // [R][LW]
   const uint64_t* invMi[2]; // [R]
+  const SmallPrime* sp[2];  // [R]: constants of the small primes (row 0 unused)
   const uint64_t* M[2];     // [LW]
   // Epilogues mod q without the full integer (per set): M_i mod q [R][LQ]
   // and q - (M mod q) [LQ]; scale-and-round splits of the product set
@@ -125,11 +127,11 @@ __device__ __forceinline__ uint64_t center(uint64_t r, uint64_t p) {
   return r > (p >> 1) ? gl::P - (p - r) : r;
 }
 
-// centred residue r times w (the negacyclic twist), as a Goldilocks element
+// centred residue r times w (the negacyclic twist), as a Goldilocks element;
+// a negative residue multiplies |r| by -w (w != 0, so P - w is canonical)
 __device__ __forceinline__ uint64_t twist_centered(uint64_t r, uint64_t p, uint64_t w) {
   const bool neg = r > (p >> 1);
-  const uint64_t y = mul_small(neg ? p - r : r, w);
-  return neg ? gl::neg(y) : y;
+  return mul_small((uint32_t)(neg ? p - r : r), neg ? gl::P - w : w);
 }
 
 // CRT of one coefficient (LIN 64-bit limbs, limb-major with stride `ls`):
@@ -290,17 +292,6 @@ __device__ __forceinline__ void ntt2_inv(uint64_t* x, uint64_t* lds, const uint6
   }
 }
 
-// After the inverse transform: value v = x * N^-1 mod P is the exact signed
-// convolution coefficient; reduce it mod p and fold in (M/p)^-1 mod p.
-__device__ __forceinline__ uint32_t finish_residue(uint64_t x, uint64_t ninv, uint64_t p, uint64_t b,
-                                                   uint64_t invM) {
-  int64_t v = gl::lift(gl::mul(x, ninv));
-  uint64_t mag = v < 0 ? (uint64_t)(-v) : (uint64_t)v;
-  uint64_t r = mod_p(mag, p, b);
-  if (v < 0 && r) r = p - r;
-  return (uint32_t)mod_p(r * invM, p, b);
-}
-
 // ---------------------------------------------------------------- fused products
 //
 // k_prod: one workgroup per (pair, prime). CRT both operands on load, forward
@@ -355,11 +346,12 @@ __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int l
   }
 }
 
-// y of one residue after the inverse transform (x already scaled/untwisted):
-// modulus 0 keeps the exact residue mod P, the others reduce the signed
-// convolution value mod p_i; both are multiplied by (M/m_i)^-1 mod m_i.
-__device__ __forceinline__ uint64_t finish_any(bool gold, uint64_t v, uint64_t p, uint64_t b, uint64_t invM) {
-  return gold ? gl::mul(v, invM) : (uint64_t)finish_residue(v, 1, p, b, invM);
+// y of residue i of set `set` after the inverse transform (v already
+// scaled/untwisted): modulus 0 keeps the exact residue mod P, the others
+// reduce the signed convolution value mod p_i (finish_small); both are
+// multiplied by (M/m_i)^-1 mod m_i.
+__device__ __forceinline__ uint64_t finish_any(const RingDev& rd, int set, int i, uint64_t v) {
+  return i == 0 ? gl::mul(v, rd.invMi[set][0]) : (uint64_t)finish_small(v, rd.sp[set][i]);
 }
 
 template <int LOGN, bool NEGA>
@@ -395,14 +387,13 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   __syncthreads();
   const int g3 = opaque_tid(xa[0]);
   ntt16_inv<LOGN>(xa, lds, pa.tab.inv, g3);
-  const uint64_t invM = rd.invMi[pa.set][i];
   uint32_t* y = pa.Y + (size_t)pair * y_rows(rd.set_R[pa.set]) * N;
   constexpr int stride = N >> 4;
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g3 + c * stride;
     const uint64_t v = NEGA ? gl::mul(xa[c], pa.tab.untwist[j]) : gl::mul(xa[c], pa.tab.n_inv);
-    store_y(y, N, i, j, finish_any(gold, v, p, b, invM));
+    store_y(y, N, i, j, finish_any(rd, pa.set, i, v));
   }
 }
 
@@ -442,9 +433,8 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
       }
       rc[k] = gl::add(rc[k], prod);
     }
-  const uint64_t invM = rd.invMi[pa.set][i];
   uint32_t* y = pa.Y + (size_t)pair * y_rows(rd.set_R[pa.set]) * N;
-  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(gold, rc[j], p, b, invM));
+  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(rd, pa.set, i, rc[j]));
 }
 
 // ---------------------------------------------------------------- keyswitch
@@ -527,15 +517,11 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
   }
   const int g2 = opaque_tid(acc[0]);
   ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g2);
-  const uint64_t p = rd.primes[1][i];
-  const uint64_t b = rd.barrett[1][i];
-  const uint64_t invM = rd.invMi[1][i];
-  const bool gold = i == 0;
   uint32_t* y = ka.Y + (size_t)ct * y_rows(Rks) * N;
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g2 + c * stride;
-    store_y(y, N, i, j, finish_any(gold, gl::mul(acc[c], rd.tab.untwist[j]), p, b, invM));
+    store_y(y, N, i, j, finish_any(rd, 1, i, gl::mul(acc[c], rd.tab.untwist[j])));
   }
 }
 
@@ -567,9 +553,8 @@ __global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* gamma_r
         rc[kk] = gl::add(rc[kk], prod);
       }
   }
-  const uint64_t invM = rd.invMi[1][i];
   uint32_t* y = ka.Y + (size_t)ct * y_rows(Rks) * N;
-  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(gold, rc[j], p, b, invM));
+  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(rd, 1, i, rc[j]));
 }
 
 // Pre-transform of the keyswitch keys: gamma_k (coefficients, [lw][LQ][n]) ->

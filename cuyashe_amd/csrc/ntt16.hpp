@@ -56,11 +56,11 @@ __host__ __device__ constexpr int pass_offset(int logn, int p) {
 // x * 2^e mod P for a compile-time e (gl::mul_pow2).
 __device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int e) { return gl::mul_pow2(x, e); }
 
-// a * b mod P for a < 2^32.
-__device__ __forceinline__ uint64_t mul_small(uint64_t a, uint64_t b) {
-  const uint64_t p0 = a * (b & gl::EPS);
-  const uint64_t p1 = a * (b >> 32) + (p0 >> 32);
-  return gl::reduce128(p1 >> 32, (p1 << 32) | (p0 & gl::EPS));
+// a * b mod P for a < 2^32 (the high word of the 96-bit product is < 2^32).
+__device__ __forceinline__ uint64_t mul_small(uint32_t a, uint64_t b) {
+  const uint64_t p0 = (uint64_t)a * (uint32_t)b;
+  const uint64_t p1 = (uint64_t)a * (uint32_t)(b >> 32) + (p0 >> 32);
+  return gl::reduce128(p1 >> 32, (p1 << 32) | (uint32_t)p0);
 }
 
 // In-register DIF of R slots: natural order in, bit-reversed out.

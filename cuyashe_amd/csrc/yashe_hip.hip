@@ -579,6 +579,16 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   size_t o_mq0 = put(P.full.Mq_limbs), o_nmq0 = put(P.full.negMq_limbs);
   size_t o_mq1 = put(P.ks.Mq_limbs), o_nmq1 = put(P.ks.negMq_limbs);
   size_t o_al = put(P.scale.alpha_limbs), o_be = put(P.scale.beta_limbs);
+  // constants of the small primes, one 32-byte row per modulus (row 0 = P unused)
+  auto small_rows = [](const std::vector<uint64_t>& pr, const std::vector<uint64_t>& inv) {
+    std::vector<uint64_t> w(pr.size() * sizeof(yhk::SmallPrime) / sizeof(uint64_t), 0);
+    for (size_t i = 1; i < pr.size(); ++i) {
+      const yhk::SmallPrime k = yhk::make_small_prime((uint32_t)pr[i], (uint32_t)inv[i]);
+      memcpy(&w[i * sizeof(yhk::SmallPrime) / sizeof(uint64_t)], &k, sizeof k);
+    }
+    return w;
+  };
+  size_t o_sp0 = put(small_rows(P.primes, P.full.invMi)), o_sp1 = put(small_rows(P.primes_ks, P.ks.invMi));
   size_t o_nbm = put(P.scale.negbetaM), o_na1 = put(P.scale.negalpha1);
   YH_HIP(hipMalloc(&r->d_tables, h.size() * sizeof(uint64_t)));
   YH_HIP(hipMemcpy(r->d_tables, h.data(), h.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -598,6 +608,8 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   rd.Mq[0] = d + o_mq0; rd.negMq[0] = d + o_nmq0;
   rd.Mq[1] = d + o_mq1; rd.negMq[1] = d + o_nmq1;
   rd.alpha = d + o_al; rd.beta = d + o_be; rd.negbetaM = d + o_nbm; rd.negalpha1 = d + o_na1;
+  rd.sp[0] = reinterpret_cast<const yhk::SmallPrime*>(d + o_sp0);
+  rd.sp[1] = reinterpret_cast<const yhk::SmallPrime*>(d + o_sp1);
   *out = r.release();
   return YH_OK;
   YH_TRY_END

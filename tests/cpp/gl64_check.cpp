@@ -1,10 +1,12 @@
-// Host check of the Goldilocks helpers the kernels use (cuyashe_amd/csrc/gl64.hpp):
-// every result against 128-bit arithmetic, for every twiddle exponent of the
-// shift-only DFTs and the edge values of the field. Prints "ok" and exits 0.
+// Host check of the Goldilocks and small-prime helpers the kernels use
+// (cuyashe_amd/csrc/gl64.hpp, modp.hpp): every result against 128-bit
+// arithmetic, for every twiddle exponent of the shift-only DFTs, the edge
+// values of the field and a spread of CRT primes. Prints "ok" and exits 0.
 #include <stdint.h>
 #include <stdio.h>
 
 #include "gl64.hpp"
+#include "modp.hpp"
 
 static const unsigned __int128 PP = gl::P;
 
@@ -66,6 +68,29 @@ int main() {
       return 1;
     }
     ++checks;
+  }
+  // finish_small: (lift(x) mod p) * inv mod p, lift(x) = x - P [x > P/2]
+  const uint32_t primes[] = {3, 5, 7, 65537, 262139, 524287, 94906249, 94906247, 134217689, 268435399};
+  for (uint32_t p : primes) {
+    for (int rep = 0; rep < 4; ++rep) {
+      const uint32_t inv = rep == 0 ? 1 : rep == 1 ? p - 1 : (uint32_t)(next() % (p - 1)) + 1;
+      const yhk::SmallPrime k = yhk::make_small_prime(p, inv);
+      for (int i = 0; i < NE + 50000; ++i) {
+        uint64_t x = i < NE ? edges[i] : next() % P;
+        if (i == NE) x = P >> 1;
+        if (i == NE + 1) x = (P >> 1) + 1;
+        __int128 c = x > (P >> 1) ? (__int128)x - (__int128)P : (__int128)x;
+        __int128 cm = c % p;
+        if (cm < 0) cm += p;
+        const uint32_t want = (uint32_t)((cm * inv) % p);
+        const uint32_t got = yhk::finish_small(x, k);
+        if (got != want) {
+          printf("finish_small(%llu) mod %u = %u, want %u\n", (unsigned long long)x, p, got, want);
+          return 1;
+        }
+        ++checks;
+      }
+    }
   }
   printf("ok %ld\n", checks);
   return 0;
