@@ -46,19 +46,24 @@ GL_HD uint64_t sub(uint64_t a, uint64_t b) {
 
 GL_HD uint64_t neg(uint64_t a) { return a == 0 ? 0 : P - a; }
 
-// Reduce hi * 2^64 + lo modulo P, canonical result.
-// hi = hh * 2^32 + hl; 2^64 = EPS and 2^96 = -1 (mod P).
-GL_HD uint64_t reduce128(uint64_t hi, uint64_t lo) {
-  uint64_t hh = hi >> 32;
-  uint64_t hl = hi & EPS;
+// Reduce top 2^128 + hi 2^64 + lo modulo P, canonical result, for top < 2^4
+// (0 for a 128-bit value; up to 15 for a sum of up to 16 products).
+// hi = hh' 2^32 + hl; 2^64 = EPS, 2^96 = -1 and 2^128 = -2^32 (mod P), so
+// with hh = top 2^32 + hh' (< 2^36 < P) the value is lo - hh + hl EPS.
+GL_HD uint64_t reduce_wide(uint32_t top, uint64_t hi, uint64_t lo) {
+  const uint64_t hh = ((uint64_t)top << 32) | (hi >> 32);
+  const uint64_t hl = hi & EPS;
   uint64_t t0 = lo - hh;
   if (lo < hh) t0 -= EPS;              // borrow: add P = 2^64 - EPS
-  uint64_t t1 = (hl << 32) - hl;       // hl * EPS < 2^64
+  const uint64_t t1 = (hl << 32) - hl;  // hl * EPS < 2^64
   uint64_t t2 = t0 + t1;
   if (t2 < t0) t2 += EPS;              // carry: 2^64 = EPS
   if (t2 >= P) t2 -= P;
   return t2;
 }
+
+// Reduce hi * 2^64 + lo modulo P, canonical result.
+GL_HD uint64_t reduce128(uint64_t hi, uint64_t lo) { return reduce_wide(0, hi, lo); }
 
 // Full 64x64 -> 128 product from four 32x32 -> 64 partial products
 // (v_mad_u64_u32 on CDNA); no partial sum overflows 64 bits.

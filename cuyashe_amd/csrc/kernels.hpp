@@ -42,10 +42,12 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_MIN_WAVES
 #define YH_MIN_WAVES 2
 #endif
-// The fused product kernel fits three waves per SIMD without spills (140
-// VGPRs at n = 4096, 116 at n >= 8192); the keyswitch fits four (126).
+// The fused product kernel runs at four waves per SIMD (128 VGPRs; at
+// n = 4096 the compiler spills 68 bytes per lane, and four waves still
+// measure ~6 % faster than three without spills at 140 VGPRs); the keyswitch
+// kernels fit four or more without spills.
 #ifndef YH_PROD_WAVES
-#define YH_PROD_WAVES 3
+#define YH_PROD_WAVES 4
 #endif
 #ifndef YH_KS_WAVES
 #define YH_KS_WAVES 4
@@ -506,14 +508,33 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
   const int g = threadIdx.x;
   const uint64_t* dg = ka.D + (size_t)ct * rd.lw * N + g;
   const uint64_t* ev = ka.evk + (size_t)i * N + g;
+  // The l_w products of a slot are summed exactly as 128-bit values (top
+  // counts the carries out of bit 128, < l_w <= 16) and reduced once.
   uint64_t acc[E];
 #pragma unroll
-  for (int c = 0; c < E; ++c) acc[c] = gl::mul(dg[c * stride], ev[c * stride]);
-  for (int k = 1; k < rd.lw; ++k) {
-    dg += N;
-    ev += (size_t)Rks * N;
+  for (int h = 0; h < E; h += 8) {
+    uint64_t lo[8], hi[8];
+    uint32_t top[8];
 #pragma unroll
-    for (int c = 0; c < E; ++c) acc[c] = gl::add(acc[c], gl::mul(dg[c * stride], ev[c * stride]));
+    for (int c = 0; c < 8; ++c) {
+      lo[c] = hi[c] = 0;
+      top[c] = 0;
+    }
+    for (int k = 0; k < rd.lw; ++k) {
+      const uint64_t* dk = dg + (size_t)k * N;
+      const uint64_t* ek = ev + (size_t)k * Rks * N;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        uint64_t ph, pl;
+        gl::mul128(dk[(h + c) * stride], ek[(h + c) * stride], ph, pl);
+        lo[c] += pl;
+        const uint64_t t = ph + (lo[c] < pl);  // ph <= 2^64 - 2
+        hi[c] += t;
+        top[c] += hi[c] < t;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[h + c] = gl::reduce_wide(top[c], hi[c], lo[c]);
   }
   const int g2 = opaque_tid(acc[0]);
   ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g2);

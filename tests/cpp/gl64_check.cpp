@@ -69,6 +69,22 @@ int main() {
     }
     ++checks;
   }
+  // reduce_wide: top 2^128 + hi 2^64 + lo for top < 16
+  {
+    const unsigned __int128 e = ((unsigned __int128)1 << 64) % PP;  // 2^64 mod P
+    const uint64_t two128 = (uint64_t)(e * e % PP);
+    for (int i = 0; i < 300000; ++i) {
+      const uint32_t top = (uint32_t)(next() % 16);
+      const uint64_t hi = i < NE ? edges[i] : next(), lo = i < NE ? ~edges[i] : next();
+      const unsigned __int128 v = ((unsigned __int128)hi << 64) | lo;
+      const uint64_t want = (uint64_t)(((unsigned __int128)top * two128 + v % PP) % PP);
+      if (gl::reduce_wide(top, hi, lo) != want) {
+        printf("reduce_wide mismatch\n");
+        return 1;
+      }
+      ++checks;
+    }
+  }
   // finish_small: (lift(x) mod p) * inv mod p, lift(x) = x - P [x > P/2]
   const uint32_t primes[] = {3, 5, 7, 65537, 262139, 524287, 94906249, 94906247, 134217689, 268435399};
   for (uint32_t p : primes) {
