@@ -49,13 +49,19 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_PROD_WAVES
 #define YH_PROD_WAVES 4
 #endif
-#ifndef YH_KS_WAVES
-#define YH_KS_WAVES 4
-#endif
 // The keyswitch multiply-accumulate (k_ks_mac) loads l_w digit and key
 // transforms per thread before its inverse transform.
 #ifndef YH_KSMAC_WAVES
 #define YH_KSMAC_WAVES 4
+#endif
+// The digit transforms (k_ks_digits, 77 VGPRs) and the reconstruction
+// (k_icrt, 66 VGPRs) run at the occupancy their registers allow; forcing
+// eight waves (64 VGPRs, small spills) measured slower for both.
+#ifndef YH_KSDIG_WAVES
+#define YH_KSDIG_WAVES 6
+#endif
+#ifndef YH_ICRT_WAVES
+#define YH_ICRT_WAVES 7
 #endif
 
 // Device view of one ring's tables (all pointers are device memory).
@@ -474,7 +480,7 @@ __device__ __forceinline__ uint64_t digit_of(const uint64_t* a, int n, int lq, i
 // c * n/16 + g, so each store or load instruction of a wavefront covers
 // 512 contiguous bytes.
 template <int LOGN, bool W32>
-__global__ void __launch_bounds__((1 << LOGN) / 16, YH_KS_WAVES) k_ks_digits(RingDev rd, KsArgs ka) {
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSDIG_WAVES) k_ks_digits(RingDev rd, KsArgs ka) {
   extern __shared__ uint64_t lds[];
   constexpr int N = 1 << LOGN;
   constexpr int stride = N >> 4;
@@ -1050,7 +1056,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
 }
 
 template <int LQ>
-__global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
+__global__ void __launch_bounds__(256, YH_ICRT_WAVES) k_icrt(RingDev rd, IcrtArgs ia) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int pair = blockIdx.y;
   if (j >= ia.ncoef) return;
