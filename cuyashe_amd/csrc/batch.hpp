@@ -122,7 +122,7 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   xsync(wave);
   ntt16_fwd_rt(xb, my, tab.fwd, g, logn, wave);
 #pragma unroll
-  for (int c = 0; c < E; ++c) xa[c] = gl::mul(xa[c], xb[c]);
+  for (int c = 0; c < E; ++c) xa[c] = gl::mont_mul(xa[c], xb[c]);  // x 2^-64, cancelled by untwist_pw
   xsync(wave);
   ntt16_inv_rt(xa, my, tab.inv, g, logn, wave);
   if (active) {
@@ -130,7 +130,7 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int j = g + c * T;
-      store_y(in.Y, N, i, j, finish_any(rd, 0, i, gl::mul(xa[c], tab.untwist[j])));
+      store_y(in.Y, N, i, j, finish_any(rd, 0, i, gl::mont_mul(xa[c], tab.untwist_pw[j])));
     }
   }
   // Publish the residues: every wave waits until its stores have reached L2

@@ -82,6 +82,66 @@ GL_HD uint64_t mul(uint64_t a, uint64_t b) {
   return reduce128(hi, lo);
 }
 
+// 32-bit add/subtract with carry/borrow in and out (v_add_co/v_sub_co chains
+// on the device; plain arithmetic in a host build).
+GL_HD uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t* cout) {
+#if defined(__clang__)
+  unsigned c;
+  const uint32_t r = __builtin_addc(a, b, cin, &c);
+  *cout = c;
+  return r;
+#else
+  const uint64_t s = (uint64_t)a + b + cin;
+  *cout = (uint32_t)(s >> 32);
+  return (uint32_t)s;
+#endif
+}
+GL_HD uint32_t subc32(uint32_t a, uint32_t b, uint32_t bin, uint32_t* bout) {
+#if defined(__clang__)
+  unsigned c;
+  const uint32_t r = __builtin_subc(a, b, bin, &c);
+  *bout = c;
+  return r;
+#else
+  const uint64_t d = (uint64_t)a - b - bin;
+  *bout = (uint32_t)(d >> 63);
+  return (uint32_t)d;
+#endif
+}
+
+// Montgomery reduction with R = 2^64: hi 2^64 + lo (< P 2^64) times 2^-64
+// mod P, canonical. P^-1 = 1 + 2^32 (mod 2^64), so the quotient digit is
+// m = lo (1 + 2^32) mod 2^64 with no multiply, and floor(m P / 2^64) =
+// m - (m >> 32) - [m_lo > m_hi], where [m_lo > m_hi] is the carry e of
+// lo + (lo << 32). The result hi - floor(m P / 2^64) lies in (-P, P).
+GL_HD uint64_t mont_reduce(uint64_t hi, uint64_t lo) {
+  const uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32);
+  uint32_t e, br1, br2, x;
+  const uint32_t m1 = addc32(l1, l0, 0u, &e);  // m = (m1 : l0)
+  const uint32_t t0 = subc32(l0, 0u, e, &br1);
+  const uint32_t b0 = subc32(t0, m1, 0u, &br2);
+  uint32_t b1 = subc32(m1, 0u, br1, &x);
+  b1 = subc32(b1, 0u, br2, &x);
+  uint32_t c1, c2, c3;
+  const uint32_t r0 = subc32((uint32_t)hi, b0, 0u, &c1);
+  const uint32_t r1 = subc32((uint32_t)(hi >> 32), b1, c1, &c2);
+  // negative: add P = 2^64 - EPS, i.e. subtract EPS modulo 2^64
+  const uint32_t s0 = subc32(r0, c2 ? 0xFFFFFFFFu : 0u, 0u, &c3);
+  const uint32_t s1 = subc32(r1, 0u, c3, &x);
+  return ((uint64_t)s1 << 32) | s0;
+}
+
+// a b 2^-64 mod P: with b = w 2^64 mod P (a constant stored in Montgomery
+// form) this is a w mod P, six instructions cheaper than mul().
+GL_HD uint64_t mont_mul(uint64_t a, uint64_t b) {
+  uint64_t hi, lo;
+  mul128(a, b, hi, lo);
+  return mont_reduce(hi, lo);
+}
+
+// w 2^64 mod P (Montgomery form of a canonical w).
+GL_HD uint64_t to_mont(uint64_t w) { return mul(w, EPS); }
+
 // x * 2^e mod P for canonical x, canonical result. Meant for exponents
 // known at compile time (the twiddles inside the 16-, 8-, 4- and 2-point
 // DFTs): after inlining only one branch remains. 2^96 = -1 and 2^192 = 1, so

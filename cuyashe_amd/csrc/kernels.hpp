@@ -391,7 +391,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   __syncthreads();  // every thread has left the LDS of the first transform
   ntt16_fwd<LOGN>(xb, lds, fwd2, g2);
 #pragma unroll
-  for (int c = 0; c < E; ++c) xa[c] = gl::mul(xa[c], xb[c]);
+  for (int c = 0; c < E; ++c) xa[c] = gl::mont_mul(xa[c], xb[c]);  // x 2^-64, cancelled by *_pw
   __syncthreads();
   const int g3 = opaque_tid(xa[0]);
   ntt16_inv<LOGN>(xa, lds, pa.tab.inv, g3);
@@ -400,7 +400,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g3 + c * stride;
-    const uint64_t v = NEGA ? gl::mul(xa[c], pa.tab.untwist[j]) : gl::mul(xa[c], pa.tab.n_inv);
+    const uint64_t v = NEGA ? gl::mont_mul(xa[c], pa.tab.untwist_pw[j]) : gl::mont_mul(xa[c], pa.tab.n_inv_pw);
     store_y(y, N, i, j, finish_any(rd, pa.set, i, v));
   }
 }
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g2 + c * stride;
-    store_y(y, N, i, j, finish_any(rd, 1, i, gl::mul(acc[c], rd.tab.untwist[j])));
+    store_y(y, N, i, j, finish_any(rd, 1, i, gl::mont_mul(acc[c], rd.tab.untwist_m[j])));
   }
 }
 

@@ -27,13 +27,17 @@
 
 namespace yhk {
 
-// Device pointers of the tables of one transform length.
+// Device pointers of the tables of one transform length. The inter-pass
+// twiddles are stored in Montgomery form (w 2^64 mod P) and applied with
+// gl::mont_mul. A pointwise product of two transforms taken with mont_mul
+// carries an extra 2^-64, which the *_pw constants cancel.
 struct NttTab {
-  const uint64_t* fwd;      // inter-pass twiddles, forward
-  const uint64_t* inv;      // inter-pass twiddles, inverse
-  const uint64_t* twist;    // psi^j
-  const uint64_t* untwist;  // psi^-j * N^-1
-  uint64_t n_inv;
+  const uint64_t* fwd;         // inter-pass twiddles, forward (Montgomery form)
+  const uint64_t* inv;         // inter-pass twiddles, inverse (Montgomery form)
+  const uint64_t* twist;       // psi^j
+  const uint64_t* untwist_pw;  // psi^-j N^-1 2^128 (after a mont_mul pointwise product)
+  const uint64_t* untwist_m;   // psi^-j N^-1 2^64 (Montgomery form)
+  uint64_t n_inv_pw;           // N^-1 2^128 (cyclic products)
 };
 
 __host__ __device__ constexpr int brev4(int c) {
@@ -139,7 +143,7 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
       const int s = 1 << b0;
       const uint64_t* t = tw + pass_offset(LOGN, p) + (g & (s - 1));
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mul(x[c], t[(brev4(c) - 1) * s]);
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[(brev4(c) - 1) * s]);
     }
     if (p < NFULL - 1 || SHORT) {
       st16(lds, x, base, b0);
@@ -181,7 +185,7 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
       const int s = 1 << b0;
       const uint64_t* t = twi + pass_offset(LOGN, p) + (g & (s - 1));
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mul(x[c], t[(brev4(c) - 1) * s]);
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[(brev4(c) - 1) * s]);
     }
     dit_r<16>(x);
     if (p > 0) {
@@ -251,7 +255,7 @@ __device__ __forceinline__ void ntt16_fwd_rt(uint64_t* x, uint64_t* lds, const u
       const int s = 1 << b0;
       const uint64_t* t = tw + off + (g & (s - 1));
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mul(x[c], t[(brev4(c) - 1) * s]);
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[(brev4(c) - 1) * s]);
       off += 15 * s;
     }
     if (p < nfull - 1 || sh) {
@@ -292,7 +296,7 @@ __device__ __forceinline__ void ntt16_inv_rt(uint64_t* x, uint64_t* lds, const u
       off -= 15 * s;
       const uint64_t* t = twi + off + (g & (s - 1));
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mul(x[c], t[(brev4(c) - 1) * s]);
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[(brev4(c) - 1) * s]);
     }
     dit_r<16>(x);
     if (p > 0) {

@@ -301,9 +301,12 @@ struct RingParams {
 // ntt16.hpp): inter-pass twiddles w_L^(o*k) (k = 1..15, o < s, per pass with
 // s = 2^b0 > 1), their inverses, the negacyclic twist psi^j and the untwist
 // psi^-j * N^-1, with w_N = 7^((P-1)/N) and psi = 7^((P-1)/(2N)).
+// fwd/inv in Montgomery form (applied with gl::mont_mul); untwist_pw and
+// n_inv_pw carry 2^128 (Montgomery form and the 2^-64 of a mont_mul
+// pointwise product), untwist_m 2^64.
 struct Ntt16Tables {
-  std::vector<uint64_t> fwd, inv, twist, untwist;
-  uint64_t n_inv = 0;
+  std::vector<uint64_t> fwd, inv, twist, untwist_pw, untwist_m;
+  uint64_t n_inv_pw = 0;
 };
 
 inline Ntt16Tables ntt16_tables(int N) {
@@ -321,23 +324,26 @@ inline Ntt16Tables ntt16_tables(int N) {
       const uint64_t step = gl::pow(wL, (uint64_t)k), stepi = gl::pow(wLi, (uint64_t)k);
       uint64_t a = 1, ai = 1;
       for (int o = 0; o < s; ++o) {
-        t.fwd.push_back(a);
-        t.inv.push_back(ai);
+        t.fwd.push_back(gl::to_mont(a));
+        t.inv.push_back(gl::to_mont(ai));
         a = gl::mul(a, step);
         ai = gl::mul(ai, stepi);
       }
     }
   }
-  if (t.fwd.empty()) { t.fwd.push_back(1); t.inv.push_back(1); }
+  if (t.fwd.empty()) { t.fwd.push_back(gl::to_mont(1)); t.inv.push_back(gl::to_mont(1)); }
   const uint64_t psi = gl::pow(7, (gl::P - 1) / (2 * (uint64_t)N));
   const uint64_t psii = gl::inv(psi);
-  t.n_inv = gl::inv((uint64_t)N);
+  const uint64_t n_inv = gl::inv((uint64_t)N);
+  t.n_inv_pw = gl::to_mont(gl::to_mont(n_inv));
   t.twist.resize(N);
-  t.untwist.resize(N);
-  uint64_t a = 1, ai = t.n_inv;
+  t.untwist_pw.resize(N);
+  t.untwist_m.resize(N);
+  uint64_t a = 1, ai = n_inv;
   for (int j = 0; j < N; ++j) {
     t.twist[j] = a;
-    t.untwist[j] = ai;
+    t.untwist_m[j] = gl::to_mont(ai);
+    t.untwist_pw[j] = gl::to_mont(t.untwist_m[j]);
     a = gl::mul(a, psi);
     ai = gl::mul(ai, psii);
   }

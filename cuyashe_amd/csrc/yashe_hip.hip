@@ -170,14 +170,16 @@ int upload_ntt16(int N, uint64_t** buf, yhk::NttTab* tab) {
   size_t o_f = h.size(); h.insert(h.end(), t.fwd.begin(), t.fwd.end());
   size_t o_i = h.size(); h.insert(h.end(), t.inv.begin(), t.inv.end());
   size_t o_t = h.size(); h.insert(h.end(), t.twist.begin(), t.twist.end());
-  size_t o_u = h.size(); h.insert(h.end(), t.untwist.begin(), t.untwist.end());
+  size_t o_u = h.size(); h.insert(h.end(), t.untwist_pw.begin(), t.untwist_pw.end());
+  size_t o_m = h.size(); h.insert(h.end(), t.untwist_m.begin(), t.untwist_m.end());
   YH_HIP(hipMalloc(buf, h.size() * sizeof(uint64_t)));
   YH_HIP(hipMemcpy(*buf, h.data(), h.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   tab->fwd = *buf + o_f;
   tab->inv = *buf + o_i;
   tab->twist = *buf + o_t;
-  tab->untwist = *buf + o_u;
-  tab->n_inv = t.n_inv;
+  tab->untwist_pw = *buf + o_u;
+  tab->untwist_m = *buf + o_m;
+  tab->n_inv_pw = t.n_inv_pw;
   return YH_OK;
 }
 

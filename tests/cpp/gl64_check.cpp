@@ -85,6 +85,32 @@ int main() {
       ++checks;
     }
   }
+  // mont_mul: a * to_mont(w) = a w mod P; mont_reduce(hi, lo) = (hi 2^64 + lo) 2^-64 mod P
+  {
+    const unsigned __int128 e = ((unsigned __int128)1 << 64) % PP;
+    // 2^-64 mod P by exponentiation: (2^64)^(P-2)
+    unsigned __int128 inv64 = 1, base = e;
+    for (uint64_t k = P - 2; k; k >>= 1) {
+      if (k & 1) inv64 = inv64 * base % PP;
+      base = base * base % PP;
+    }
+    for (int i = 0; i < 400000; ++i) {
+      const uint64_t a = i < NE * NE ? edges[i % NE] : next() % P;
+      const uint64_t w = i < NE * NE ? edges[i / NE] : next() % P;
+      if (gl::mont_mul(a, gl::to_mont(w)) != ref_mul(a, w)) {
+        printf("mont_mul mismatch %llu %llu\n", (unsigned long long)a, (unsigned long long)w);
+        return 1;
+      }
+      // any 128-bit value below P 2^64
+      const uint64_t hi = (i < NE ? edges[i] : next()) % P, lo = next();
+      const unsigned __int128 v = ((unsigned __int128)hi << 64) | lo;
+      if (gl::mont_reduce(hi, lo) != (uint64_t)(v % PP * inv64 % PP)) {
+        printf("mont_reduce mismatch\n");
+        return 1;
+      }
+      checks += 2;
+    }
+  }
   // finish_small: (lift(x) mod p) * inv mod p, lift(x) = x - P [x > P/2]
   const uint32_t primes[] = {3, 5, 7, 65537, 262139, 524287, 94906249, 94906247, 134217689, 268435399};
   for (uint32_t p : primes) {
