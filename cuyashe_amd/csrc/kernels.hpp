@@ -93,6 +93,10 @@ struct RingDev {
   const uint64_t* beta;
   const uint64_t* negbetaM;
   const uint64_t* negalpha1;
+  // one-word division of the epilogues (rings with bits(q) >= 64 LQ - 26)
+  int fast_div;
+  int q_shift;
+  uint64_t qmu;
 };
 
 // threadIdx.x, recomputed after `dep` is ready (see the register note above).
@@ -888,13 +892,15 @@ __device__ __forceinline__ uint32_t icrt_pass(const RingDev& rd, const uint32_t*
   return (uint32_t)(fhi + (flo >> 63));
 }
 
-// V = sum_i y_i X_i + k Xk as LQ + 2 limbs from the column sums S of the
-// small-prime terms (icrt_pass): normalise the columns, then add the 64-bit
-// y of modulus 0 times X_0 and k times Xk with carrying multiply-accumulates.
+// V = sum_i y_i X_i + k Xk (< 2^(64 LQ + 34)) as LQ + 1 limbs from the column
+// sums S of the small-prime terms (icrt_pass): normalise the columns, then
+// add the two 32-bit halves of the y of modulus 0 (rows 0 and RS of X: the
+// high half's row is 2^32 X_0 mod q, or the scale-and-round split of it) and
+// k times Xk with carrying multiply-accumulates.
 template <int LQ>
-__device__ __forceinline__ void fold_finish(const uint64_t (&S)[2 * LQ], const uint64_t* X, const uint64_t* Xk,
-                                            uint64_t yP, uint32_t k, uint64_t* V) {
-  constexpr int NX = 2 * LQ, NA = NX + 3;
+__device__ __forceinline__ void fold_finish(const uint64_t (&S)[2 * LQ], const uint64_t* X, int RS,
+                                            const uint64_t* Xk, uint64_t yP, uint32_t k, uint64_t* V) {
+  constexpr int NX = 2 * LQ, NA = NX + 2;
   uint32_t A[NA];
   uint64_t c = 0;
 #pragma unroll
@@ -904,14 +910,64 @@ __device__ __forceinline__ void fold_finish(const uint64_t (&S)[2 * LQ], const u
     c = cur >> 32;
   }
   A[NX] = (uint32_t)c;
-  A[NX + 1] = (uint32_t)(c >> 32);
-  A[NX + 2] = 0;
+  A[NX + 1] = 0;
   mac_words<NX, NA, 0>(A, X, (uint32_t)yP);
-  mac_words<NX, NA, 1>(A, X, (uint32_t)(yP >> 32));
+  mac_words<NX, NA, 0>(A, X + (size_t)RS * LQ, (uint32_t)(yP >> 32));
   mac_words<NX, NA, 0>(A, Xk, k);
 #pragma unroll
   for (int l = 0; l < LQ + 1; ++l) V[l] = (uint64_t)A[2 * l] | ((uint64_t)A[2 * l + 1] << 32);
-  V[LQ + 1] = A[NA - 1];
+}
+
+// V (LQ + 1 limbs) = quo q + rem, rem in [0, q). On rings with fast_div
+// (bits(q) >= 64 LQ - 26, so V / q < 2^62 for V < 2^(64 LQ + 35)) one 64-bit
+// Barrett step: floor(floor(V / 2^s) qmu / 2^64) with s = bits(q) - 1 and
+// qmu = floor(2^(s+64) / q) is the quotient or falls short by at most 2.
+// Other rings take the general truncated Barrett.
+template <int LQ>
+__device__ __forceinline__ void divmod_v(const uint64_t* V, const RingDev& rd, uint64_t* quo, uint64_t* rem) {
+  constexpr int LW = 2 * LQ + 1;
+  constexpr int LA = LW + 1;
+  constexpr int LMU = LW + 2 - LQ;
+  if (rd.fast_div) {
+    const int o = rd.q_shift;  // bit s of the value lies in limb LQ - 1
+    uint64_t vhi = V[LQ - 1] >> o;
+    if (o) vhi |= V[LQ] << (64 - o);
+    uint64_t qe = gl::umulhi(vhi, rd.qmu);
+    uint64_t qq[LQ + 1], r[LQ + 1], qext[LQ + 1];
+    uint64_t carry = 0;
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) {
+      const uint64_t lo = qe * rd.q[l];
+      uint64_t hi = gl::umulhi(qe, rd.q[l]);
+      const uint64_t s = lo + carry;
+      hi += (s < lo);
+      qq[l] = s;
+      carry = hi;
+      qext[l] = rd.q[l];
+      r[l] = V[l];
+    }
+    qq[LQ] = carry;
+    qext[LQ] = 0;
+    r[LQ] = V[LQ];
+    sub_n<LQ + 1>(r, qq);  // V - qe q in [0, 3q)
+#pragma unroll
+    for (int fix = 0; fix < 2; ++fix) {
+      if (geq_n<LQ + 1>(r, qext)) {
+        sub_n<LQ + 1>(r, qext);
+        ++qe;
+      }
+    }
+    quo[0] = qe;
+#pragma unroll
+    for (int k = 1; k < LMU; ++k) quo[k] = 0;
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) rem[l] = r[l];
+  } else {
+    uint64_t a[LA];
+#pragma unroll
+    for (int l = 0; l < LA; ++l) a[l] = l <= LQ ? V[l] : 0;
+    divmod_q<LQ>(a, rd, quo, rem, LQ + 1);
+  }
 }
 
 // c mod q in [0, q) for the centred reconstruction c (set `set`):
@@ -919,17 +975,13 @@ __device__ __forceinline__ void fold_finish(const uint64_t (&S)[2 * LQ], const u
 template <int LQ>
 __device__ __forceinline__ void icrt_mod_q(const RingDev& rd, const uint32_t* y, size_t ystride, int set,
                                            uint64_t* rem) {
-  constexpr int LW = 2 * LQ + 1;
-  constexpr int LA = LW + 1;
-  constexpr int LMU = LW + 2 - LQ;
+  constexpr int LMU = 2 * LQ + 3 - LQ;
   uint64_t yP, S[1][2 * LQ];
   const uint64_t* X[1] = {rd.Mq[set]};
   const uint32_t k = icrt_pass<LQ, 1>(rd, y, ystride, set, X, S, yP);
-  uint64_t a[LA], quo[LMU];
-#pragma unroll
-  for (int l = 0; l < LA; ++l) a[l] = 0;
-  fold_finish<LQ>(S[0], rd.Mq[set], rd.negMq[set], yP, k, a);
-  divmod_q<LQ>(a, rd, quo, rem, LQ + 2);
+  uint64_t v[LQ + 1], quo[LMU];
+  fold_finish<LQ>(S[0], rd.Mq[set], rd.set_R[set], rd.negMq[set], yP, k, v);
+  divmod_v<LQ>(v, rd, quo, rem);
 }
 
 // ICRT of one coefficient plus its epilogue. y points at the operand's first
@@ -988,25 +1040,23 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
     // is non-negative: floor(S / q) = floor(S' / q) - k.
     uint64_t yP, S[2][2 * LQ];
     const uint64_t* X[2] = {rd.beta, rd.alpha};
+    const int RS = rd.set_R[set];
     const uint32_t k = icrt_pass<LQ, 2>(rd, y, ystride, set, X, S, yP);
-    uint64_t a[LA], quo[LMU], rem[LQ];
-#pragma unroll
-    for (int l = 0; l < LA; ++l) a[l] = 0;
-    fold_finish<LQ>(S[0], rd.beta, rd.negbetaM, yP, k, a);
-    divmod_q<LQ>(a, rd, quo, rem, LQ + 2);  // S' = quo q + rem
+    uint64_t v[LQ + 1], quo[LMU], rem[LQ];
+    fold_finish<LQ>(S[0], rd.beta, RS, rd.negbetaM, yP, k, v);
+    divmod_v<LQ>(v, rd, quo, rem);  // S' = quo q + rem
     const uint64_t up = twice_gt_q<LQ>(rem, rd.q) ? 1 : 0;
-    // result = sum y_i alpha_i + k (q - (alpha_M + 1)) + quo + up  (mod q)
+    // result = sum y_i alpha_i + k (q - (alpha_M + 1)) + quo + up  (mod q);
+    // quo < 2^62 with fast_div, < 2^99 otherwise: the sum fits LQ + 1 limbs
+    fold_finish<LQ>(S[1], rd.alpha, RS, rd.negalpha1, yP, k, v);
+    uint64_t add[LQ + 1];
 #pragma unroll
-    for (int l = 0; l < LA; ++l) a[l] = 0;
-    fold_finish<LQ>(S[1], rd.alpha, rd.negalpha1, yP, k, a);
-    uint64_t add[LQ + 2];
+    for (int l = 0; l < LQ + 1; ++l) add[l] = quo[l];
+    add_n<LQ + 1>(v, add);
 #pragma unroll
-    for (int l = 0; l < LQ + 2; ++l) add[l] = l < LMU ? quo[l] : 0;
-    add_n<LQ + 2>(a, add);
-#pragma unroll
-    for (int l = 0; l < LQ + 2; ++l) add[l] = l == 0 ? up : 0;
-    add_n<LQ + 2>(a, add);
-    divmod_q<LQ>(a, rd, quo, rem, LQ + 2);
+    for (int l = 0; l < LQ + 1; ++l) add[l] = l == 0 ? up : 0;
+    add_n<LQ + 1>(v, add);
+    divmod_v<LQ>(v, rd, quo, rem);
 #pragma unroll
     for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = rem[l];
     return;

@@ -133,7 +133,9 @@ struct IcrtSet {
   std::vector<uint64_t> M_limbs;    // [LW]
   // Reconstruction mod q without the full integer: c = sum y_i M_i - k M, so
   // c mod q = (sum y_i (M_i mod q) + k (q - M mod q)) mod q.
-  std::vector<uint64_t> Mq_limbs;   // [R][LQ]: M_i mod q
+  // Row R: 2^32 (M_0 mod q) mod q, the constant of the high 32 bits of the
+  // 64-bit y of modulus 0 (the device splits that y into two halves).
+  std::vector<uint64_t> Mq_limbs;   // [R + 1][LQ]: M_i mod q
   std::vector<uint64_t> negMq_limbs;  // [LQ]: q - (M mod q), in (0, q]
 };
 
@@ -142,8 +144,12 @@ struct IcrtSet {
 //   t c / q = sum y_i alpha_i - k alpha_M + S / q,  S = sum y_i beta_i - k beta_M,
 // so round(t c / q) mod q follows from two short accumulations.
 struct ScaleTables {
-  std::vector<uint64_t> alpha_limbs;  // [R][LQ]: alpha_i mod q
-  std::vector<uint64_t> beta_limbs;   // [R][LQ]: beta_i
+  // Row R belongs to the high half y_0h of the 64-bit y of modulus 0:
+  // y_0h 2^32 t M_0 = q y_0h (2^32 alpha_0 + f) + y_0h beta_0', with
+  // 2^32 beta_0 = q f + beta_0', so row R of alpha is (2^32 alpha_0 + f) mod q
+  // and row R of beta is beta_0'.
+  std::vector<uint64_t> alpha_limbs;  // [R + 1][LQ]: alpha_i mod q
+  std::vector<uint64_t> beta_limbs;   // [R + 1][LQ]: beta_i
   std::vector<uint64_t> negbetaM;     // [LQ]: q - beta_M, in (0, q]
   std::vector<uint64_t> negalpha1;    // [LQ]: q - ((alpha_M + 1) mod q), in (0, q]
 };
@@ -168,6 +174,13 @@ struct RingParams {
   ScaleTables scale;
   std::vector<uint64_t> q_limbs;    // [LQ]
   std::vector<uint64_t> mu_limbs;   // [LW+2-LQ]: floor(2^(64*(LW+1)) / q)
+  // One-word division of the reconstruction epilogues: for V < 2^(64 LQ + 35)
+  // the quotient V / q is below 2^62 when bits(q) >= 64 LQ - 26; then
+  // floor(floor(V / 2^s) qmu / 2^64), s = bits(q) - 1 (bit q_shift of limb
+  // LQ - 1), qmu = floor(2^(s+64) / q), is the quotient or falls short by <= 2.
+  bool fast_div = false;
+  int q_shift = 0;
+  uint64_t qmu = 0;
 
   static int ilog2(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
@@ -220,7 +233,7 @@ struct RingParams {
     s.invMi.assign(R, 0);
     s.M_limbs.assign(LW, 0);
     s.M.to_limbs(s.M_limbs.data(), LW);
-    s.Mq_limbs.assign((size_t)R * LQ, 0);
+    s.Mq_limbs.assign((size_t)(R + 1) * LQ, 0);
     for (int i = 0; i < R; ++i) {
       BigInt Mi = s.M / BigInt(primes[i]);
       Mi.to_limbs(&s.Mi_limbs[(size_t)i * LW], LW);
@@ -230,12 +243,13 @@ struct RingParams {
     }
     s.negMq_limbs.assign(LQ, 0);
     (q - s.M % q).to_limbs(s.negMq_limbs.data(), LQ);
+    ((BigInt::pow2(32) * ((s.M / BigInt(primes[0])) % q)) % q).to_limbs(&s.Mq_limbs[(size_t)R * LQ], LQ);
   }
 
   void build_scale(const IcrtSet& s) {
     const int R = s.R;
-    scale.alpha_limbs.assign((size_t)R * LQ, 0);
-    scale.beta_limbs.assign((size_t)R * LQ, 0);
+    scale.alpha_limbs.assign((size_t)(R + 1) * LQ, 0);
+    scale.beta_limbs.assign((size_t)(R + 1) * LQ, 0);
     scale.negbetaM.assign(LQ, 0);
     scale.negalpha1.assign(LQ, 0);
     if (t == 0) return;
@@ -244,6 +258,13 @@ struct RingParams {
       const BigInt tMi = tb * (s.M / BigInt(primes[i]));
       ((tMi / q) % q).to_limbs(&scale.alpha_limbs[(size_t)i * LQ], LQ);
       (tMi % q).to_limbs(&scale.beta_limbs[(size_t)i * LQ], LQ);
+    }
+    {
+      const BigInt tM0 = tb * (s.M / BigInt(primes[0]));
+      const BigInt a0 = tM0 / q, b0 = tM0 % q;
+      const BigInt hb = BigInt::pow2(32) * b0;
+      ((BigInt::pow2(32) * a0 + hb / q) % q).to_limbs(&scale.alpha_limbs[(size_t)R * LQ], LQ);
+      (hb % q).to_limbs(&scale.beta_limbs[(size_t)R * LQ], LQ);
     }
     const BigInt tM = tb * s.M;
     (q - tM % q).to_limbs(scale.negbetaM.data(), LQ);
@@ -294,6 +315,13 @@ struct RingParams {
     BigInt mu = BigInt::pow2(64 * (LW + 1)) / q;
     mu_limbs.assign(LMU, 0);
     mu.to_limbs(mu_limbs.data(), LMU);
+    const int b = q.bits();
+    q_shift = (b - 1) & 63;
+    fast_div = b >= 64 * LQ - 26 && q > BigInt::pow2(b - 1);
+    if (fast_div) {
+      BigInt m = BigInt::pow2(b - 1 + 64) / q;  // < 2^64 because q > 2^(b-1)
+      qmu = m.to_u64();
+    }
   }
 };
 

@@ -610,6 +610,9 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   rd.Mq[0] = d + o_mq0; rd.negMq[0] = d + o_nmq0;
   rd.Mq[1] = d + o_mq1; rd.negMq[1] = d + o_nmq1;
   rd.alpha = d + o_al; rd.beta = d + o_be; rd.negbetaM = d + o_nbm; rd.negalpha1 = d + o_na1;
+  rd.fast_div = P.fast_div ? 1 : 0;
+  rd.q_shift = P.q_shift;
+  rd.qmu = P.qmu;
   rd.sp[0] = reinterpret_cast<const yhk::SmallPrime*>(d + o_sp0);
   rd.sp[1] = reinterpret_cast<const yhk::SmallPrime*>(d + o_sp1);
   *out = r.release();

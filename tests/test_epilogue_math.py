@@ -41,6 +41,34 @@ def fold_mod_q(primes, ys, k, M, q):
     return acc % q
 
 
+def fold_mod_q_split(primes, ys, k, M, q):
+    """As fold_mod_q with the y of modulus 0 (P) split into 32-bit halves:
+    y_0 = y_0l + 2^32 y_0h, the high half's constant 2^32 (M_0 mod q) mod q."""
+    X = [(M // m) % q for m in primes]
+    y0l, y0h = ys[0] & 0xFFFFFFFF, ys[0] >> 32
+    XR = (X[0] << 32) % q
+    acc = y0l * X[0] + y0h * XR + sum(y * x for y, x in zip(ys[1:], X[1:])) + k * (q - M % q)
+    assert acc < (1 << (64 * ((q.bit_length() + 63) // 64) + 34))
+    return acc % q
+
+
+def fold_scale_split(primes, ys, k, M, q, t):
+    """The scale-and-round fold with y_0 split: the high half uses
+    beta_0' = 2^32 beta_0 mod q and alpha_0' = 2^32 alpha_0 + floor(2^32 beta_0 / q)."""
+    alphas = [(t * (M // m)) // q for m in primes]
+    betas = [(t * (M // m)) % q for m in primes]
+    aM, bM = (t * M) // q, (t * M) % q
+    y0l, y0h = ys[0] & 0xFFFFFFFF, ys[0] >> 32
+    b0p = (betas[0] << 32) % q
+    a0p = ((alphas[0] << 32) + ((betas[0] << 32) // q)) % q
+    S1 = y0l * betas[0] + y0h * b0p + sum(y * b for y, b in zip(ys[1:], betas[1:])) + k * (q - bM)
+    Q1, r = divmod(S1, q)
+    up = 1 if 2 * r > q else 0
+    acc = y0l * (alphas[0] % q) + y0h * a0p + sum(y * (a % q) for y, a in zip(ys[1:], alphas[1:])) \
+        + k * (q - ((aM + 1) % q)) + Q1 + up
+    return acc % q
+
+
 def fold_scale(primes, ys, k, M, q, t, off_by_one=False):
     alphas = [(t * (M // m)) // q for m in primes]
     betas = [(t * (M // m)) % q for m in primes]
@@ -72,8 +100,11 @@ def test_folded_epilogues_match_definitions(n, q, t):
             M_, ys, k = reconstruction(mods, c)
             assert M_ == M
             assert fold_mod_q(mods, ys, k, M, q) == c % q
+            assert fold_mod_q_split(mods, ys, k, M, q) == c % q
             if mods is primes:
-                assert fold_scale(mods, ys, k, M, q, t) == cpuref.round_tq(c, t, q), c
+                want = cpuref.round_tq(c, t, q)
+                assert fold_scale(mods, ys, k, M, q, t) == want, c
+                assert fold_scale_split(mods, ys, k, M, q, t) == want, c
 
 
 def test_folded_scale_planted_mistake_is_caught():
