@@ -113,11 +113,17 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
                  : twist_centered(crt_residue(in.A + j, LQ, N, pw, p, b), p, tab.twist[j]);
   }
   ntt16_fwd_rt(xa, my, tab.fwd, g, logn, wave);
+  // second operand in two groups of eight (group_fence): fewer loads in
+  // flight while the first operand's transform is held in registers
 #pragma unroll
-  for (int c = 0; c < E; ++c) {
-    const int j = g + c * T;
-    xb[c] = gold ? gl::mul(crt_residue_P(in.B + j, LQ, N), tab.twist[j])
-                 : twist_centered(crt_residue(in.B + j, LQ, N, pw, p, b), p, tab.twist[j]);
+  for (int h = 0; h < E; h += 8) {
+#pragma unroll
+    for (int c = h; c < h + 8; ++c) {
+      const int j = g + c * T;
+      xb[c] = gold ? gl::mul(crt_residue_P(in.B + j, LQ, N), tab.twist[j])
+                   : twist_centered(crt_residue(in.B + j, LQ, N, pw, p, b), p, tab.twist[j]);
+    }
+    if (h + 8 < E) group_fence(xb + h);
   }
   xsync(wave);
   ntt16_fwd_rt(xb, my, tab.fwd, g, logn, wave);

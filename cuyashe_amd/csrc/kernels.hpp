@@ -54,14 +54,12 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_KSMAC_WAVES
 #define YH_KSMAC_WAVES 4
 #endif
-// The digit transforms (k_ks_digits, 77 VGPRs) and the reconstruction
-// (k_icrt, 66 VGPRs) run at the occupancy their registers allow; forcing
-// eight waves (64 VGPRs, small spills) measured slower for both.
+// The digit transforms (k_ks_digits, 77 VGPRs) run at six waves; forcing
+// eight (64 VGPRs, small spills) measured slower. The reconstruction
+// (k_icrt) takes the registers its limb count needs (66 VGPRs, seven waves,
+// at 2 limbs; more for wider q, where a wave bound would force spills).
 #ifndef YH_KSDIG_WAVES
 #define YH_KSDIG_WAVES 6
-#endif
-#ifndef YH_ICRT_WAVES
-#define YH_ICRT_WAVES 7
 #endif
 
 // Device view of one ring's tables (all pointers are device memory).
@@ -332,16 +330,30 @@ struct ProdArgs {
 
 // CRT of 16 coefficients into the entry layout of ntt16_fwd (j = g + c*N/16),
 // centred, and for negacyclic transforms twisted by psi^j.
+// The coefficients are loaded and reduced in two groups of eight: the empty
+// asm makes the second group's loads wait for the first group's residues, so
+// at most eight coefficients' limbs are in registers at once (the compiler
+// would otherwise issue all loads up front and spill while the first
+// operand's transform is held in registers).
+__device__ __forceinline__ void group_fence(uint64_t* x) {
+  asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]), "+v"(x[6]),
+               "+v"(x[7])::"memory");
+}
+
 template <int LOGN, bool NEGA>
 __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                          const uint64_t* pw, uint64_t p, uint64_t b, const uint64_t* twist,
                                          int g = threadIdx.x) {
   constexpr int stride = 1 << (LOGN - 4);
 #pragma unroll
-  for (int c = 0; c < E; ++c) {
-    const int j = g + c * stride;
-    const uint64_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, p, b) : 0;
-    x[c] = NEGA ? twist_centered(r, p, twist[j]) : center(r, p);
+  for (int h = 0; h < E; h += 8) {
+#pragma unroll
+    for (int c = h; c < h + 8; ++c) {
+      const int j = g + c * stride;
+      const uint64_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, p, b) : 0;
+      x[c] = NEGA ? twist_centered(r, p, twist[j]) : center(r, p);
+    }
+    if (h + 8 < E) group_fence(x + h);
   }
 }
 
@@ -351,10 +363,14 @@ __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int l
                                            const uint64_t* twist, int g) {
   constexpr int stride = 1 << (LOGN - 4);
 #pragma unroll
-  for (int c = 0; c < E; ++c) {
-    const int j = g + c * stride;
-    const uint64_t r = j < ncoef ? crt_residue_P(a + j, lin, ls) : 0;
-    x[c] = NEGA ? gl::mul(r, twist[j]) : r;
+  for (int h = 0; h < E; h += 8) {
+#pragma unroll
+    for (int c = h; c < h + 8; ++c) {
+      const int j = g + c * stride;
+      const uint64_t r = j < ncoef ? crt_residue_P(a + j, lin, ls) : 0;
+      x[c] = NEGA ? gl::mul(r, twist[j]) : r;
+    }
+    if (h + 8 < E) group_fence(x + h);
   }
 }
 
@@ -1106,7 +1122,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
 }
 
 template <int LQ>
-__global__ void __launch_bounds__(256, YH_ICRT_WAVES) k_icrt(RingDev rd, IcrtArgs ia) {
+__global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int pair = blockIdx.y;
   if (j >= ia.ncoef) return;
