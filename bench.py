@@ -55,6 +55,25 @@ def parse():
 
 TUNE_CACHE = os.path.join(ROOT, "cuyashe_amd", "tune_cache.json")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+SQ_FILE = os.path.join(ROOT, "profiles", "sq_counters_mul_keyswitch_n4096.md")
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs x one wave64 instruction per 4
+# cycles at 2.4 GHz = 614.4 G wave-instructions/s
+VALU_PEAK_G = 256 * 2.4
+
+
+def measured_valu(kernel_symbol):
+    """SQ_INSTS_VALU per launch of the kernel from the committed rocprofv3
+    SQ-counter summary (the count is fixed by the code and the batch), or
+    None."""
+    try:
+        with open(SQ_FILE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.split("|")]
+                if len(cells) > 3 and cells[2] == "SQ_INSTS_VALU" and kernel_symbol in cells[1]:
+                    return float(cells[3])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def measured_traffic(kernel, batch):
@@ -331,6 +350,14 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name, B),
                 "kernel": dom_name, "avg_ms": round(dom_avg, 5), "launches": dom_n}
+    symbols = {"k_prod": "k_prod<12, true>", "k_ks_digits": "k_ks_digits<12, true>", "k_keyswitch": "k_ks_mac<12>"}
+    valu = measured_valu(symbols[dom_name])
+    valu_roofline = None
+    if valu:
+        ach = valu / (dom_avg * 1e-3) / 1e9
+        valu_roofline = {"bound": "valu", "kernel": dom_name, "achieved": round(ach, 1), "peak": VALU_PEAK_G,
+                         "unit": "G wave-instructions/s", "frac": round(ach / VALU_PEAK_G, 4),
+                         "instructions_per_launch": valu}
     ntt_k = [k for k in kernels.values() if k["bytes_per_launch"]]
     ntt_bytes = sum(k["bytes_per_launch"] * k["launches_per_step"] for k in ntt_k)
     ntt_ms = sum(k["avg_ms"] * k["launches_per_step"] for k in ntt_k)
@@ -385,6 +412,7 @@ def main():
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items() if kk != "id"}
                     for k, v in kernels.items()},
         "roofline": roofline,
+        "valu_roofline": valu_roofline,
         "cpu_baseline": cpu,
         "bit_exact_sample": exact,
         "secondary": secondary,

@@ -61,6 +61,11 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_KSDIG_WAVES
 #define YH_KSDIG_WAVES 6
 #endif
+// coefficients per thread of the reconstruction kernel (two measured
+// slower than one at C2)
+#ifndef YH_ICRT_COEFS
+#define YH_ICRT_COEFS 1
+#endif
 
 // Device view of one ring's tables (all pointers are device memory).
 struct RingDev {
@@ -1123,13 +1128,16 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
 
 template <int LQ>
 __global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int pair = blockIdx.y;
-  if (j >= ia.ncoef) return;
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
-  icrt_coef<LQ>(rd, ia.Y + (size_t)pair * y_rows(RS) * ia.N + j, ia.N, ia.set, ia.mode, ia.positive, o + j,
-                ia.out_limbs, ia.out_stride);
+  const uint32_t* Y = ia.Y + (size_t)pair * y_rows(RS) * ia.N;
+#pragma unroll
+  for (int u = 0; u < YH_ICRT_COEFS; ++u) {
+    const int j = (blockIdx.x * YH_ICRT_COEFS + u) * blockDim.x + threadIdx.x;
+    if (j < ia.ncoef)
+      icrt_coef<LQ>(rd, Y + j, ia.N, ia.set, ia.mode, ia.positive, o + j, ia.out_limbs, ia.out_stride);
+  }
 }
 
 // ---------------------------------------------------------------- primitives

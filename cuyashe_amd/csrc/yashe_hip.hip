@@ -255,7 +255,8 @@ int launch_prod(yh_ring* r, yhk::ProdArgs pa, int N, int batch, bool negacyclic)
 
 template <int LQ>
 void launch_icrt_t(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
-  dim3 block(256), grid((ia.ncoef + 255) / 256, batch);
+  constexpr int per = 256 * YH_ICRT_COEFS;
+  dim3 block(256), grid((ia.ncoef + per - 1) / per, batch);
   hipLaunchKernelGGL(yhk::k_icrt<LQ>, grid, block, 0, r->stream, r->rd, ia);
 }
 
