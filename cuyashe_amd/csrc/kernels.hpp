@@ -474,19 +474,24 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
 //
 // Two launches per batch. k_ks_digits: one workgroup per (ciphertext, digit
 // k); digit k of each coefficient (bits [k*w, (k+1)*w) of c in [0, q)) is
-// read straight from the ciphertext limbs, twisted and transformed once,
-// unreduced, and the transform is written to scratch. k_ks_mac: one workgroup
-// per (ciphertext, modulus of the keyswitch set) multiplies the l_w digit
-// transforms by the resident NTT(gamma_k mod m_i) and accumulates, then runs
-// one inverse transform. Sharing the digit transforms between moduli is exact
-// because the keyswitch primes are sized for unreduced digits
-// (l_w n (2^w - 1) (p-1)/2 <= (P-1)/2): the accumulated value is the integer
-// sum_k d_k * (gamma_k mod p_i), which the epilogue reduces mod p_i. The
-// reference (src/ciphertext.cpp:81-130) runs one full CRT product per digit.
+// read straight from the ciphertext limbs, centred (d - 2^(w-1)), twisted and
+// transformed once, unreduced, and the transform is written to scratch.
+// k_ks_mac: one workgroup per (ciphertext, modulus of the keyswitch set)
+// multiplies the l_w digit transforms by the resident NTT(gamma_k mod m_i)
+// and accumulates, then runs one inverse transform. Sharing the digit
+// transforms between moduli is exact because the keyswitch primes are sized
+// for centred unreduced digits (l_w n 2^(w-1) (p-1)/2 <= (P-1)/2): the
+// accumulated value is the integer sum_k (d_k - 2^(w-1)) * (gamma_k mod p_i);
+// the epilogue adds the per-key correction 2^(w-1) sum_k 1 * gamma_k mod p_i
+// (k_ks_corr) and reduces mod p_i. Centring halves the digit bound, which
+// admits primes twice as large and saves one keyswitch modulus on the common
+// rings. The reference (src/ciphertext.cpp:81-130) runs one full CRT product
+// per digit.
 struct KsArgs {
   const uint64_t* C;        // [ct][LQ][n]
   size_t ostride;
   const uint64_t* evk;      // [lw][R_ks][n] transform domain
+  const uint64_t* corr;     // [R_ks][n]: 2^(w_bits-1) sum_k (1 * gamma_k) mod m_i (digit centring)
   uint64_t* D;              // [ct][lw][n] digit transforms (scratch)
   uint32_t* Y;              // [ct][R_ks][n]
   int batch;
@@ -514,13 +519,20 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSDIG_WAVES) k_ks_digits(
   const uint64_t* c0 = ka.C + ct * ka.ostride;
   const int g = threadIdx.x;
   const uint32_t* w32 = reinterpret_cast<const uint32_t*>(c0 + (size_t)(k >> 1) * N) + (k & 1);
+  const uint64_t half = W32 ? (1ull << 31) : (1ull << (rd.w_bits - 1));
   uint64_t x[E];
 #pragma unroll
-  for (int c = 0; c < E; ++c) {
-    const int j = g + c * stride;
-    const uint64_t d = W32 ? (uint64_t)w32[2 * j] : digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
-    // d < 2^32 < P: the digit is its own Goldilocks element
-    x[c] = mul_small(d, rd.tab.twist[j]);
+  for (int h = 0; h < E; h += 8) {
+#pragma unroll
+    for (int c = h; c < h + 8; ++c) {
+      const int j = g + c * stride;
+      const uint64_t d = W32 ? (uint64_t)w32[2 * j] : digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
+      // the centred digit d - 2^(w-1), twisted: |.| <= 2^31 times +-psi^j
+      const bool neg = d < half;
+      const uint64_t w = rd.tab.twist[j];
+      x[c] = mul_small((uint32_t)(neg ? half - d : d - half), neg ? gl::P - w : w);
+    }
+    if (h + 8 < E) group_fence(x + h);
   }
   ntt16_fwd<LOGN>(x, lds, rd.tab.fwd, g);
   uint64_t* o = ka.D + ((size_t)ct * rd.lw + k) * N + g;
@@ -569,11 +581,26 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
   }
   const int g2 = opaque_tid(acc[0]);
   ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g2);
+  // the inverse gives sum_k d'_k * (gamma_k mod m_i) for the centred digits
+  // d'_k = d_k - 2^(w-1); adding corr_i = 2^(w-1) sum_k (1 * gamma_k) mod m_i
+  // restores sum_k d_k * gamma_k mod m_i
   uint32_t* y = ka.Y + (size_t)ct * y_rows(Rks) * N;
+  const uint64_t* cr = ka.corr + (size_t)i * N;
+  if (i == 0) {
+    const uint64_t invM = rd.invMi[1][0];
 #pragma unroll
-  for (int c = 0; c < E; ++c) {
-    const int j = g2 + c * stride;
-    store_y(y, N, i, j, finish_any(rd, 1, i, gl::mont_mul(acc[c], rd.tab.untwist_m[j])));
+    for (int c = 0; c < E; ++c) {
+      const int j = g2 + c * stride;
+      const uint64_t v = gl::add(gl::mont_mul(acc[c], rd.tab.untwist_m[j]), cr[j]);
+      store_y(y, N, 0, j, gl::mul(v, invM));
+    }
+  } else {
+    const SmallPrime sp = rd.sp[1][i];
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      const int j = g2 + c * stride;
+      store_y(y, N, i, j, finish_small(gl::mont_mul(acc[c], rd.tab.untwist_m[j]), sp, (uint32_t)cr[j]));
+    }
   }
 }
 
@@ -629,6 +656,51 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_evk_prep(Rin
   ntt16_fwd<LOGN>(x, lds, rd.tab.fwd);
 #pragma unroll
   for (int c = 0; c < E; ++c) o[c * (N >> 4) + threadIdx.x] = x[c];
+}
+
+// Correction of the centred keyswitch digits, once per key: corr_i =
+// 2^(w-1) sum_k (1 * gamma_k) mod m_i, 1 the all-ones polynomial (its
+// negacyclic product with g is 2 prefix_j(g) - total(g)). One workgroup per
+// keyswitch modulus computes sum_k 1 * (gamma_k mod m_i) exactly as a
+// keyswitch whose every digit transform is NTT(psi^j) (the twisted all-ones
+// vector): |.| <= l_w n (p-1)/2, far inside the Goldilocks bound.
+template <int LOGN>
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ks_corr(RingDev rd, const uint64_t* evk,
+                                                                             uint64_t* corr) {
+  extern __shared__ uint64_t lds[];
+  constexpr int N = 1 << LOGN;
+  constexpr int stride = N >> 4;
+  const int i = blockIdx.x;
+  const int Rks = rd.set_R[1];
+  const int g = threadIdx.x;
+  uint64_t x[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) x[c] = rd.tab.twist[g + c * stride];
+  ntt16_fwd<LOGN>(x, lds, rd.tab.fwd, g);
+#pragma unroll
+  for (int c = 0; c < E; ++c) {
+    uint64_t s = 0;
+    for (int k = 0; k < rd.lw; ++k) s = gl::add(s, evk[((size_t)k * Rks + i) * N + c * stride + g]);
+    x[c] = gl::mul(x[c], s);
+  }
+  __syncthreads();
+  ntt16_inv<LOGN>(x, lds, rd.tab.inv, g);
+  const int wb = rd.w_bits;
+  const uint64_t p = rd.primes[1][i], b = rd.barrett[1][i];
+  const uint64_t half_p = i == 0 ? 0 : (1ull << (wb - 1)) % p;
+#pragma unroll
+  for (int c = 0; c < E; ++c) {
+    const int j = g + c * stride;
+    const uint64_t v = gl::mont_mul(x[c], rd.tab.untwist_m[j]);
+    if (i == 0) {
+      corr[j] = gl::mul_pow2(v, wb - 1);
+    } else {
+      const int64_t sv = gl::lift(v);
+      uint64_t r = mod_p(sv < 0 ? (uint64_t)(-sv) : (uint64_t)sv, p, b);
+      if (sv < 0 && r) r = p - r;
+      corr[(size_t)i * N + j] = mod_p(r * half_p, p, b);
+    }
+  }
 }
 
 // Keyswitch keys of small rings (n < 16): centred residues, no transform.

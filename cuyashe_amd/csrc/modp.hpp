@@ -40,14 +40,14 @@ GL_HD uint32_t shoup(uint32_t a, uint32_t w, uint32_t ws, uint32_t p) {
   return a * w - mulhi32(a, ws) * p;
 }
 
-// y = (lift(x) mod p) (M/p)^-1 mod p in [0, p), x a canonical Goldilocks
-// element.
-GL_HD uint32_t finish_small(uint64_t x, const SmallPrime& k) {
+// y = ((lift(x) + extra) mod p) (M/p)^-1 mod p in [0, p), x a canonical
+// Goldilocks element, extra < p.
+GL_HD uint32_t finish_small(uint64_t x, const SmallPrime& k, uint32_t extra = 0) {
   const uint32_t xh = (uint32_t)(x >> 32), xl = (uint32_t)x;
   const uint32_t t = shoup(xh, k.c32, k.c32s, k.p);       // [0, 2p)
   const uint32_t r = xl - mulhi32(xl, k.bm) * k.p;        // [0, 2p)
   const uint32_t s = x > (gl::P >> 1) ? k.kc : 0u;        // [0, p)
-  const uint32_t y = shoup(t + r + s, k.inv, k.invs, k.p);  // [0, 2p)
+  const uint32_t y = shoup(t + r + s + extra, k.inv, k.invs, k.p);  // sum < 6p < 2^31; y in [0, 2p)
   const uint32_t y2 = y - k.p;
   return y2 < y ? y2 : y;  // y >= p: y - p; else y - p wraps above y
 }

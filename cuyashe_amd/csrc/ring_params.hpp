@@ -14,8 +14,8 @@
 //  * The prime size is the largest that keeps the Goldilocks convolution exact:
 //    for a ring product (residues of both operands centred in (-p/2, p/2]),
 //    n*((p-1)/2)^2 <= (P-1)/2; for the keyswitch, whose digits are transformed
-//    unreduced once and shared by every modulus,
-//    l_w*n*(2^w_bits - 1)*((p-1)/2) <= (P-1)/2.
+//    once, centred but not reduced, and shared by every modulus,
+//    l_w*n*2^(w_bits-1)*((p-1)/2) <= (P-1)/2.
 //  * The prime count makes M >= 4*B, B the largest |coefficient| the operation
 //    can produce, so the ICRT quotient k = round(sum y_i/p_i) can be found
 //    with 64-bit fixed point and no correction step.
@@ -93,14 +93,16 @@ inline uint64_t max_prime_for(int n, int K) {
 }
 
 // Largest prime size for the keyswitch set. The keyswitch transforms every
-// digit d (0 <= d < 2^w_bits) once, unreduced, and shares that transform
-// between all moduli; only the key residue is reduced mod p and centred. The
-// l_w-term multiply-accumulate stays exact in the Goldilocks field when
-// l_w * n * (2^w_bits - 1) * ((p-1)/2) <= (P-1)/2.
+// digit once, centred (d - 2^(w_bits-1), |.| <= 2^(w_bits-1)) but otherwise
+// unreduced, and shares that transform between all moduli; only the key
+// residue is reduced mod p and centred. The l_w-term multiply-accumulate
+// stays exact in the Goldilocks field when
+// l_w * n * 2^(w_bits-1) * ((p-1)/2) <= (P-1)/2; the shift of the digits is
+// a fixed correction per key (added after the inverse transform).
 inline uint64_t max_prime_ks(int n, int lw, int w_bits) {
   const unsigned __int128 lim = (unsigned __int128)((gl::P - 1) / 2);
   const unsigned __int128 per = (unsigned __int128)(lw > 1 ? lw : 1) * (unsigned __int128)n *
-                                (unsigned __int128)((1ull << w_bits) - 1);
+                                (unsigned __int128)(1ull << (w_bits - 1));
   const unsigned __int128 h = lim / per;  // largest (p-1)/2
   const unsigned __int128 cap = (1ull << 28) - 1;
   const unsigned __int128 pm = 2 * h + 1;

@@ -107,7 +107,8 @@ struct yh_event {
 
 struct yh_evk {
   yh_ring* ring = nullptr;
-  uint64_t* d = nullptr;  // [lw][R_ks][n]: transform domain (n >= 16) or centred residues
+  uint64_t* d = nullptr;     // [lw][R_ks][n]: transform domain (n >= 16) or centred residues
+  uint64_t* corr = nullptr;  // [R_ks][n]: correction of the centred digits (n >= 16)
 };
 
 namespace {
@@ -219,6 +220,7 @@ void set_lds_attrs() {
                             bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_mac<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_evk_prep<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)yhk::k_ks_corr<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ntt_cyclic<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
@@ -358,6 +360,7 @@ int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out
   ka.C = c;
   ka.ostride = (size_t)LQ * n;
   ka.evk = k->d;
+  ka.corr = k->corr;
   ka.Y = r->d_y;
   ka.batch = batch;
   if (n >= 16) {
@@ -850,8 +853,16 @@ int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
   if (e != hipSuccess) { delete k; return fail(YH_ERR_NOMEM, hipGetErrorString(e)); }
   dim3 grid(r->rd.set_R[1], r->rd.lw);
   if (r->rd.n >= 16) {
+    e = hipMalloc(&k->corr, (size_t)r->rd.set_R[1] * r->rd.n * sizeof(uint64_t));
+    if (e != hipSuccess) { (void)hipFree(k->d); delete k; return fail(YH_ERR_NOMEM, hipGetErrorString(e)); }
     switch (r->rd.logn) {
-#define X(L) case L: hipLaunchKernelGGL(yhk::k_evk_prep<L>, grid, dim3(r->rd.n / 16), lds_bytes(r->rd.n), r->stream, r->rd, d_gamma, k->d); break;
+#define X(L)                                                                                                     \
+  case L:                                                                                                          \
+    hipLaunchKernelGGL(yhk::k_evk_prep<L>, grid, dim3(r->rd.n / 16), lds_bytes(r->rd.n), r->stream, r->rd, d_gamma, \
+                       k->d);                                                                                      \
+    hipLaunchKernelGGL(yhk::k_ks_corr<L>, dim3(r->rd.set_R[1]), dim3(r->rd.n / 16), lds_bytes(r->rd.n), r->stream, \
+                       r->rd, k->d, k->corr);                                                                      \
+    break;
       YH_LOGN_CASES(X)
 #undef X
       default: break;
@@ -860,7 +871,12 @@ int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
     hipLaunchKernelGGL(yhk::k_evk_residues, grid, dim3(64), 0, r->stream, r->rd, d_gamma, k->d);
   }
   e = hipGetLastError();
-  if (e != hipSuccess) { (void)hipFree(k->d); delete k; return fail(YH_ERR_HIP, hipGetErrorString(e)); }
+  if (e != hipSuccess) {
+    (void)hipFree(k->d);
+    if (k->corr) (void)hipFree(k->corr);
+    delete k;
+    return fail(YH_ERR_HIP, hipGetErrorString(e));
+  }
   *out = k;
   return YH_OK;
 }
@@ -870,6 +886,7 @@ int yh_evk_destroy(yh_evk* k) {
   (void)hipSetDevice(k->ring->ctx->device);
   (void)hipStreamSynchronize(k->ring->stream);
   (void)hipFree(k->d);
+  if (k->corr) (void)hipFree(k->corr);
   delete k;
   return YH_OK;
 }

@@ -61,13 +61,14 @@ def test_ring_plan_bounds(n, q, t, wb):
     # modulus 0 is the Goldilocks prime itself (its convolution is exact mod P
     # for any operands); the small primes follow in descending order, sized
     # for one product of centred residues per convolution in the product set,
-    # and in the keyswitch set for l_w products of an unreduced digit
-    # (< 2^wb, transformed once for every modulus) by a centred key residue
+    # and in the keyswitch set for l_w products of a centred but unreduced
+    # digit (transformed once for every modulus) by a centred key residue
     def exact_prod(p):
         return n * ((p - 1) // 2) ** 2 <= (P - 1) // 2
 
     def exact_ks(p):
-        return max(1, info.lw) * n * ((1 << wb) - 1) * ((p - 1) // 2) <= (P - 1) // 2
+        # centred digits: |d - 2^(wb-1)| <= 2^(wb-1)
+        return max(1, info.lw) * n * (1 << (wb - 1)) * ((p - 1) // 2) <= (P - 1) // 2
 
     def is_prime(p):
         return p > 1 and all(p % d for d in range(2, int(p ** 0.5) + 1))

@@ -342,16 +342,18 @@ def _crt_max_residue(primes, q):
 @pytest.mark.parametrize("n,wb", [(16, 32), (4096, 32), (1024, 16)])
 def test_keyswitch_extremes(gpu_ctx, n, wb):
     # The digit transforms are shared by every keyswitch modulus, so the
-    # multiply-accumulate of unreduced digits must stay inside the Goldilocks
-    # bound: all-(q-1) ciphertexts give digits near 2^wb - 1, and keys whose
-    # residues are the largest centred value make every term of the last
-    # coefficient's negacyclic sum positive.
+    # multiply-accumulate of centred digits (d - 2^(wb-1)) must stay inside
+    # the Goldilocks bound: all-(q-1) ciphertexts give digits near 2^wb - 1
+    # (centred: near +2^(wb-1)), all-zero ones digits 0 (centred: -2^(wb-1)),
+    # and keys whose residues are the largest centred value make every term of
+    # the last coefficient's negacyclic sum the same sign.
     q = Q127
     rng = random.Random(n + wb)
     ring = ya.Ring(gpu_ctx, n, q, t=1024, w_bits=wb)
     lw = ring.lw
     G = _crt_max_residue(ring.ks_primes()[1:], q)
     cases = [([q - 1] * n, [[G] * n] * lw),
+             ([0] * n, [[G] * n] * lw),
              ([q - 1] * n, [[q - 1] * n] * lw),
              ([q - 1] * n, [rand_poly(rng, n, q) for _ in range(lw)]),
              ([0] * (n - 1) + [q - 1], [[G] * n] * lw)]
