@@ -44,6 +44,46 @@ GL_HD uint64_t sub(uint64_t a, uint64_t b) {
   return (a < b) ? d - EPS : d;
 }
 
+// 32-bit add/subtract with carry/borrow in and out (v_add_co/v_sub_co chains
+// on the device; plain arithmetic in a host build).
+GL_HD uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t* cout) {
+#if defined(__clang__)
+  unsigned c;
+  const uint32_t r = __builtin_addc(a, b, cin, &c);
+  *cout = c;
+  return r;
+#else
+  const uint64_t s = (uint64_t)a + b + cin;
+  *cout = (uint32_t)(s >> 32);
+  return (uint32_t)s;
+#endif
+}
+GL_HD uint32_t subc32(uint32_t a, uint32_t b, uint32_t bin, uint32_t* bout) {
+#if defined(__clang__)
+  unsigned c;
+  const uint32_t r = __builtin_subc(a, b, bin, &c);
+  *bout = c;
+  return r;
+#else
+  const uint64_t d = (uint64_t)a - b - bin;
+  *bout = (uint32_t)(d >> 63);
+  return (uint32_t)d;
+#endif
+}
+
+// a - b mod P for canonical a, b, with the borrow taken from the
+// subtraction's own carry chain (one instruction fewer than sub() when
+// neither operand has a known-zero word; sub() is the better form otherwise).
+GL_HD uint64_t sub_full(uint64_t a, uint64_t b) {
+  uint32_t br, x;
+  const uint32_t d0 = subc32((uint32_t)a, (uint32_t)b, 0u, &br);
+  const uint32_t d1 = subc32((uint32_t)(a >> 32), (uint32_t)(b >> 32), br, &br);
+  // On borrow d holds a - b + 2^64; a - b + P is that minus EPS.
+  const uint32_t r0 = subc32(d0, br ? 0xFFFFFFFFu : 0u, 0u, &x);
+  const uint32_t r1 = subc32(d1, 0u, x, &x);
+  return ((uint64_t)r1 << 32) | r0;
+}
+
 GL_HD uint64_t neg(uint64_t a) { return a == 0 ? 0 : P - a; }
 
 // Reduce top 2^128 + hi 2^64 + lo modulo P, canonical result, for top < 2^4
@@ -80,33 +120,6 @@ GL_HD uint64_t mul(uint64_t a, uint64_t b) {
   uint64_t hi, lo;
   mul128(a, b, hi, lo);
   return reduce128(hi, lo);
-}
-
-// 32-bit add/subtract with carry/borrow in and out (v_add_co/v_sub_co chains
-// on the device; plain arithmetic in a host build).
-GL_HD uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t* cout) {
-#if defined(__clang__)
-  unsigned c;
-  const uint32_t r = __builtin_addc(a, b, cin, &c);
-  *cout = c;
-  return r;
-#else
-  const uint64_t s = (uint64_t)a + b + cin;
-  *cout = (uint32_t)(s >> 32);
-  return (uint32_t)s;
-#endif
-}
-GL_HD uint32_t subc32(uint32_t a, uint32_t b, uint32_t bin, uint32_t* bout) {
-#if defined(__clang__)
-  unsigned c;
-  const uint32_t r = __builtin_subc(a, b, bin, &c);
-  *bout = c;
-  return r;
-#else
-  const uint64_t d = (uint64_t)a - b - bin;
-  *bout = (uint32_t)(d >> 63);
-  return (uint32_t)d;
-#endif
 }
 
 // Montgomery reduction with R = 2^64: hi 2^64 + lo (< P 2^64) times 2^-64

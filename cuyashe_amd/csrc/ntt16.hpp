@@ -80,7 +80,10 @@ __device__ __forceinline__ void dif_r(uint64_t* x) {
       for (int j = 0; j < h; ++j) {
         const uint64_t u = x[i + j], v = x[i + j + h];
         x[i + j] = gl::add(u, v);
-        x[i + j + h] = mul_pow2(gl::sub(u, v), EW * (R / ln) * j);
+        // 2^e with e in [96, 192) is -2^(e-96): subtract the other way round
+        // instead of negating
+        const int e = (EW * (R / ln) * j) % 192;
+        x[i + j + h] = e >= 96 ? mul_pow2(gl::sub_full(v, u), e - 96) : mul_pow2(gl::sub_full(u, v), e);
       }
     }
   }
@@ -99,9 +102,16 @@ __device__ __forceinline__ void dit_r(uint64_t* x) {
 #pragma unroll
       for (int j = 0; j < h; ++j) {
         const uint64_t u = x[i + j];
-        const uint64_t v = mul_pow2(x[i + j + h], EWI * (R / ln) * j);
-        x[i + j] = gl::add(u, v);
-        x[i + j + h] = gl::sub(u, v);
+        const int e = (EWI * (R / ln) * j) % 192;
+        if (e >= 96) {  // v = -x 2^(e-96): swap the add and the subtraction
+          const uint64_t w = mul_pow2(x[i + j + h], e - 96);
+          x[i + j] = gl::sub_full(u, w);
+          x[i + j + h] = gl::add(u, w);
+        } else {
+          const uint64_t v = mul_pow2(x[i + j + h], e);
+          x[i + j] = gl::add(u, v);
+          x[i + j + h] = gl::sub_full(u, v);
+        }
       }
     }
   }

@@ -52,7 +52,8 @@ int main() {
     for (int j = 0; j < (i < NE ? NE : 1); ++j) {
       uint64_t b = i < NE ? edges[j] : next() % P;
       if (gl::mul(a, b) != ref_mul(a, b) || gl::add(a, b) != (uint64_t)(((unsigned __int128)a + b) % PP) ||
-          gl::sub(a, b) != (uint64_t)(((unsigned __int128)a + PP - b) % PP)) {
+          gl::sub(a, b) != (uint64_t)(((unsigned __int128)a + PP - b) % PP) ||
+          gl::sub_full(a, b) != (uint64_t)(((unsigned __int128)a + PP - b) % PP)) {
         printf("mul/add/sub mismatch at %llu %llu\n", (unsigned long long)a, (unsigned long long)b);
         return 1;
       }
