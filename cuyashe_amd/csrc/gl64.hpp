@@ -186,11 +186,26 @@ GL_HD uint64_t mul_pow2(uint64_t x, int e) {
   }
   const uint64_t y = ng ? P - x : x;
   if (e > 32) {
+    // h 2^32 + m EPS - r = (h + m) 2^32 - (m + r); with h + m = cA 2^32 + a,
+    // (h + m) 2^32 = (a + cA) 2^32 - cA (2^64 = EPS), so the value is
+    // a1 2^32 - B, a1 = a + cA < 2^32 (h has its low 32 - s bits clear),
+    // B = m + r + cA < 2^33: one 64-bit subtraction, plus P when negative,
+    // and the result is canonical either way.
     const int s = 64 - e;
-    const uint64_t h = (uint64_t)(uint32_t)((uint32_t)y << (32 - s)) << 32;  // <= P - 1
-    const uint64_t m = (uint64_t)(uint32_t)(y >> s);
-    const uint64_t r = (y >> 32) >> s;
-    return sub(add(h, (m << 32) - m), r);
+    const uint32_t h = (uint32_t)y << (32 - s);
+    const uint32_t m = (uint32_t)(y >> s);
+    const uint32_t r = (uint32_t)(y >> 32) >> s;
+    uint32_t cA, cB, c2, br, br2, z;
+    const uint32_t a = addc32(h, m, 0u, &cA);
+    const uint32_t b0 = addc32(m, r, 0u, &cB);
+    const uint32_t a1 = a + cA;
+    const uint32_t bl = addc32(b0, cA, 0u, &c2);
+    const uint32_t bh = cB + c2;
+    const uint32_t v0 = subc32(0u, bl, 0u, &br);
+    const uint32_t v1 = subc32(a1, bh, br, &br);
+    const uint32_t w0 = subc32(v0, br ? 0xFFFFFFFFu : 0u, 0u, &br2);  // + P = - EPS (mod 2^64)
+    const uint32_t w1 = subc32(v1, 0u, br2, &z);
+    return ((uint64_t)w1 << 32) | w0;
   }
   return reduce128(y >> (64 - e), y << e);
 }
