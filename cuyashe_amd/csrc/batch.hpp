@@ -98,8 +98,7 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   const bool wave = logn <= 10;
   const BatchInst in = ba.inst[tk.inst];
   const int i = tk.prime;
-  const uint64_t p = rd.primes[0][i];
-  const uint64_t b = rd.barrett[0][i];
+  const SmallPrime sk = rd.sp[0][i];  // row 0 (modulus P) is unused
   const int LQ = rd.LQ;
   const uint64_t* pw = rd.pow32[0] + (size_t)i * 2 * rd.LW;
   const NttTab tab = rd.tab;
@@ -110,7 +109,7 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   for (int c = 0; c < E; ++c) {
     const int j = g + c * T;
     xa[c] = gold ? gl::mul(crt_residue_P(in.A + j, LQ, N), tab.twist[j])
-                 : twist_centered(crt_residue(in.A + j, LQ, N, pw, p, b), p, tab.twist[j]);
+                 : twist_centered(crt_residue(in.A + j, LQ, N, pw, sk), sk.p, tab.twist[j]);
   }
   ntt16_fwd_rt(xa, my, tab.fwd, g, logn, wave);
   // second operand in two groups of eight (group_fence): fewer loads in
@@ -121,7 +120,7 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
     for (int c = h; c < h + 8; ++c) {
       const int j = g + c * T;
       xb[c] = gold ? gl::mul(crt_residue_P(in.B + j, LQ, N), tab.twist[j])
-                   : twist_centered(crt_residue(in.B + j, LQ, N, pw, p, b), p, tab.twist[j]);
+                   : twist_centered(crt_residue(in.B + j, LQ, N, pw, sk), sk.p, tab.twist[j]);
     }
     if (h + 8 < E) group_fence(xb + h);
   }

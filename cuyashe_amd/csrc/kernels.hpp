@@ -138,30 +138,31 @@ __device__ __forceinline__ uint64_t mod_p(uint64_t x, uint64_t p, uint64_t b) {
 
 // Residue r in [0, p) as a Goldilocks element of its centred lift in
 // (-p/2, p/2]; centring halves the magnitude entering the convolution.
-__device__ __forceinline__ uint64_t center(uint64_t r, uint64_t p) {
+__device__ __forceinline__ uint64_t center(uint32_t r, uint32_t p) {
   return r > (p >> 1) ? gl::P - (p - r) : r;
 }
 
 // centred residue r times w (the negacyclic twist), as a Goldilocks element;
 // a negative residue multiplies |r| by -w (w != 0, so P - w is canonical)
-__device__ __forceinline__ uint64_t twist_centered(uint64_t r, uint64_t p, uint64_t w) {
+__device__ __forceinline__ uint64_t twist_centered(uint32_t r, uint32_t p, uint64_t w) {
   const bool neg = r > (p >> 1);
-  return mul_small((uint32_t)(neg ? p - r : r), neg ? gl::P - w : w);
+  return mul_small(neg ? p - r : r, neg ? gl::P - w : w);
 }
 
 // CRT of one coefficient (LIN 64-bit limbs, limb-major with stride `ls`):
-// sum_k word_k * (2^(32k) mod p), then one Barrett reduction.
-// Matches bn_mod1_low (src/cuda_bn.cu:290-317): the residue x mod p.
-__device__ __forceinline__ uint64_t crt_residue(const uint64_t* a, int lin, size_t ls,
-                                                const uint64_t* pw, uint64_t p, uint64_t b) {
+// sum_k word_k * (2^(32k) mod p) with 32x32-bit products (the table entries
+// are below p < 2^28), then mod_small. Matches bn_mod1_low
+// (src/cuda_bn.cu:290-317): the residue x mod p.
+__device__ __forceinline__ uint32_t crt_residue(const uint64_t* a, int lin, size_t ls, const uint64_t* pw,
+                                                const SmallPrime& k) {
   uint64_t acc = 0;
   for (int l = 0; l < lin; ++l) {
-    uint64_t v = a[(size_t)l * ls];
-    acc += (v & 0xFFFFFFFFull) * pw[2 * l] + (v >> 32) * pw[2 * l + 1];
-    // each product < 2^32 * 2^28; reducing every 4 limbs keeps the sum < 2^64
-    if ((l & 3) == 3) acc = mod_p(acc, p, b);
+    const uint64_t v = a[(size_t)l * ls];
+    acc += (uint64_t)(uint32_t)v * (uint32_t)pw[2 * l] + (uint64_t)(uint32_t)(v >> 32) * (uint32_t)pw[2 * l + 1];
+    // each product < 2^60; reducing every 4 limbs keeps the sum < 2^63
+    if ((l & 3) == 3) acc = mod_small(acc, k);
   }
-  return mod_p(acc, p, b);
+  return mod_small(acc, k);
 }
 
 // ---------------------------------------------------------------- modulus 0 = P
@@ -169,15 +170,11 @@ __device__ __forceinline__ uint64_t crt_residue(const uint64_t* a, int lin, size
 // the convolution mod P exactly for any operand size, so its residues are
 // canonical values in [0, P) (not centred) and its y is 64 bits wide.
 
-// Residue mod P of a coefficient of `lin` 64-bit limbs (stride ls): Horner in
-// 2^64 = EPS (mod P).
+// Residue mod P of a coefficient of `lin` 64-bit limbs (stride ls): Horner
+// steps r 2^64 + limb, each one 128-bit reduction.
 __device__ __forceinline__ uint64_t crt_residue_P(const uint64_t* a, int lin, size_t ls) {
   uint64_t r = 0;
-  for (int l = lin - 1; l >= 0; --l) {
-    uint64_t v = a[(size_t)l * ls];
-    if (v >= gl::P) v -= gl::P;
-    r = gl::add(gl::mul(r, gl::EPS), v);
-  }
+  for (int l = lin - 1; l >= 0; --l) r = gl::reduce128(r, a[(size_t)l * ls]);
   return r;
 }
 
@@ -347,7 +344,7 @@ __device__ __forceinline__ void group_fence(uint64_t* x) {
 
 template <int LOGN, bool NEGA>
 __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
-                                         const uint64_t* pw, uint64_t p, uint64_t b, const uint64_t* twist,
+                                         const uint64_t* pw, const SmallPrime& k, const uint64_t* twist,
                                          int g = threadIdx.x) {
   constexpr int stride = 1 << (LOGN - 4);
 #pragma unroll
@@ -355,8 +352,8 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
 #pragma unroll
     for (int c = h; c < h + 8; ++c) {
       const int j = g + c * stride;
-      const uint64_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, p, b) : 0;
-      x[c] = NEGA ? twist_centered(r, p, twist[j]) : center(r, p);
+      const uint32_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, k) : 0u;
+      x[c] = NEGA ? twist_centered(r, k.p, twist[j]) : center(r, k.p);
     }
     if (h + 8 < E) group_fence(x + h);
   }
@@ -393,8 +390,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   constexpr int N = 1 << LOGN;
   int pair, i;
   if (!xcd_map(blockIdx.x, rd.set_R[pa.set], pa.batch, pair, i)) return;
-  const uint64_t p = rd.primes[pa.set][i];
-  const uint64_t b = rd.barrett[pa.set][i];
+  const SmallPrime sk = rd.sp[pa.set][i];  // row 0 (modulus P) is unused
   const uint64_t* pw = rd.pow32[pa.set] + (size_t)i * 2 * rd.LW;
   uint64_t xa[E], xb[E];
   const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
@@ -402,7 +398,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   const bool gold = i == 0;
   const int g1 = opaque_tid(pair);
   if (gold) load_crt_P<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.tab.twist, g1);
-  else load_crt<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, p, b, pa.tab.twist, g1);
+  else load_crt<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, sk, pa.tab.twist, g1);
   ntt16_fwd<LOGN>(xa, lds, pa.tab.fwd, g1);
   // The second operand re-reads the tables (L1 hits) rather than keeping the
   // first transform's twiddles live in registers (the empty asm hides that
@@ -412,7 +408,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   asm volatile("" : "+s"(twist2), "+s"(fwd2));
   const int g2 = opaque_tid(xa[E - 1]);
   if (gold) load_crt_P<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
-  else load_crt<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, p, b, twist2, g2);
+  else load_crt<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, sk, twist2, g2);
   __syncthreads();  // every thread has left the LDS of the first transform
   ntt16_fwd<LOGN>(xb, lds, fwd2, g2);
 #pragma unroll
@@ -437,8 +433,7 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
   const int pair = blockIdx.y;
   if (threadIdx.x != 0) return;
   const int N = 1 << pa.logN;
-  const uint64_t p = rd.primes[pa.set][i];
-  const uint64_t b = rd.barrett[pa.set][i];
+  const SmallPrime sk = rd.sp[pa.set][i];
   const uint64_t* pw = rd.pow32[pa.set] + (size_t)i * 2 * rd.LW;
   uint64_t ra[64], rb[64], rc[64];
   const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
@@ -451,8 +446,8 @@ __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
       ra[j] = j < pa.ncoef_a ? crt_residue_P(a, pa.lin_a, pa.lstride_a) : 0;
       rb[j] = j < pa.ncoef_b ? crt_residue_P(bb, pa.lin_b, pa.lstride_b) : 0;
     } else {
-      ra[j] = j < pa.ncoef_a ? center(crt_residue(a, pa.lin_a, pa.lstride_a, pw, p, b), p) : 0;
-      rb[j] = j < pa.ncoef_b ? center(crt_residue(bb, pa.lin_b, pa.lstride_b, pw, p, b), p) : 0;
+      ra[j] = j < pa.ncoef_a ? center(crt_residue(a, pa.lin_a, pa.lstride_a, pw, sk), sk.p) : 0;
+      rb[j] = j < pa.ncoef_b ? center(crt_residue(bb, pa.lin_b, pa.lstride_b, pw, sk), sk.p) : 0;
     }
     rc[j] = 0;
   }
@@ -645,14 +640,13 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_evk_prep(Rin
   const int i = blockIdx.x;
   const int k = blockIdx.y;
   const int Rks = rd.set_R[1];
-  const uint64_t p = rd.primes[1][i];
-  const uint64_t b = rd.barrett[1][i];
+  const SmallPrime sk = rd.sp[1][i];
   const uint64_t* pw = rd.pow32[1] + (size_t)i * 2 * rd.LW;
   const uint64_t* g = G + (size_t)k * rd.LQ * N;
   uint64_t* o = out + ((size_t)k * Rks + i) * N;
   uint64_t x[E];
   if (i == 0) load_crt_P<LOGN, true>(x, g, rd.LQ, N, N, rd.tab.twist, threadIdx.x);
-  else load_crt<LOGN, true>(x, g, rd.LQ, N, N, pw, p, b, rd.tab.twist);
+  else load_crt<LOGN, true>(x, g, rd.LQ, N, N, pw, sk, rd.tab.twist);
   ntt16_fwd<LOGN>(x, lds, rd.tab.fwd);
 #pragma unroll
   for (int c = 0; c < E; ++c) o[c * (N >> 4) + threadIdx.x] = x[c];
@@ -708,14 +702,13 @@ __global__ void k_evk_residues(RingDev rd, const uint64_t* G, uint64_t* out) {
   const int i = blockIdx.x;
   const int k = blockIdx.y;
   const int Rks = rd.set_R[1];
-  const uint64_t p = rd.primes[1][i];
-  const uint64_t b = rd.barrett[1][i];
+  const SmallPrime sk = rd.sp[1][i];
   const uint64_t* pw = rd.pow32[1] + (size_t)i * 2 * rd.LW;
   const int N = rd.n;
   const uint64_t* g = G + (size_t)k * rd.LQ * N;
   uint64_t* o = out + ((size_t)k * Rks + i) * N;
   for (int j = threadIdx.x; j < N; j += blockDim.x)
-    o[j] = i == 0 ? crt_residue_P(g + j, rd.LQ, N) : center(crt_residue(g + j, rd.LQ, N, pw, p, b), p);
+    o[j] = i == 0 ? crt_residue_P(g + j, rd.LQ, N) : center(crt_residue(g + j, rd.LQ, N, pw, sk), sk.p);
 }
 
 // ---------------------------------------------------------------- ICRT + epilogues
@@ -1221,11 +1214,10 @@ __global__ void k_crt(RingDev rd, const uint64_t* A, int lin, int ncoef, size_t 
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= spacing) return;
-  const uint64_t p = rd.primes[0][i];
   uint64_t r = 0;
   if (j < ncoef)
     r = i == 0 ? crt_residue_P(A + j, lin, lstride)
-               : crt_residue(A + j, lin, lstride, rd.pow32[0] + (size_t)i * 2 * rd.LW, p, rd.barrett[0][i]);
+               : crt_residue(A + j, lin, lstride, rd.pow32[0] + (size_t)i * 2 * rd.LW, rd.sp[0][i]);
   out[(size_t)i * spacing + j] = r;
 }
 

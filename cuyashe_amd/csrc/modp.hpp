@@ -29,7 +29,7 @@ struct SmallPrime {
   uint32_t kc;     // (p - P mod p) mod p: adds -P (mod p) for x > P/2
   uint32_t inv;    // (M/p)^-1 mod p
   uint32_t invs;   // floor(inv 2^32 / p)
-  uint32_t pad;
+  uint32_t bl;     // floor(2^64 / p) mod 2^32 (floor(2^64 / p) = bm 2^32 + bl)
 };
 static_assert(sizeof(SmallPrime) == 32, "one 32-byte row per prime");
 
@@ -52,6 +52,22 @@ GL_HD uint32_t finish_small(uint64_t x, const SmallPrime& k, uint32_t extra = 0)
   return y2 < y ? y2 : y;  // y >= p: y - p; else y - p wraps above y
 }
 
+// x mod p for x < 2^63 (a sum of 32-bit words times residues mod p), p < 2^28.
+// Barrett with floor(2^64/p) = bm 2^32 + bl, dropping the x0 bl term: the
+// quotient estimate x1 bm + floor((x1 bl + x0 bm) / 2^32) is at most two
+// below floor(x/p), so r = x - q p lies in [0, 3p) and only the low 32-bit
+// words of x and q p are needed; two conditional subtractions finish.
+GL_HD uint32_t mod_small(uint64_t x, const SmallPrime& k) {
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+  const uint64_t m = (uint64_t)x1 * k.bl + (uint64_t)x0 * k.bm;  // < 2^63 + 2^64/p
+  const uint32_t q = x1 * k.bm + (uint32_t)(m >> 32);
+  uint32_t r = x0 - q * k.p;
+  uint32_t r2 = r - k.p;
+  r = r2 < r ? r2 : r;
+  r2 = r - k.p;
+  return r2 < r ? r2 : r;
+}
+
 // Host: the constants of prime p with (M/p)^-1 mod p = inv.
 inline SmallPrime make_small_prime(uint32_t p, uint32_t inv) {
   SmallPrime k{};
@@ -62,6 +78,7 @@ inline SmallPrime make_small_prime(uint32_t p, uint32_t inv) {
   k.kc = (uint32_t)((p - gl::P % p) % p);
   k.inv = inv;
   k.invs = (uint32_t)(((uint64_t)inv << 32) / p);
+  k.bl = (uint32_t)(~0ull / p);  // floor((2^64 - 1) / p) = floor(2^64 / p) for p not a power of two
   return k;
 }
 

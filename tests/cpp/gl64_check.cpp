@@ -135,6 +135,42 @@ int main() {
       }
     }
   }
+  // mod_small: x mod p for x < 2^63 (the CRT sums), every prime size the
+  // modulus sets use and both ends of the input range
+  // the last five primes have a large low word of floor(2^64/p); with them
+  // the quotient estimate can fall two short, and the inputs listed below
+  // (found by search) need both conditional subtractions
+  const uint32_t mprimes[] = {3,         5,         7,         257,       65537,     262139,   524287, 16777213,
+                              94906249,  134217689, 268435399, 91393193,  204562447, 195244051, 19347019, 252698819};
+  const uint64_t two_short[] = {9222041850514571444ull, 9222877272036056722ull, 9222919397028618387ull,
+                                9223056466532287148ull, 9223270872530781452ull, 9222649857621698113ull};
+  const uint64_t top = (1ull << 63) - 1;
+  for (uint64_t x : two_short)
+    for (uint32_t p : mprimes)
+      if (yhk::mod_small(x, yhk::make_small_prime(p, 1)) != x % p) {
+        printf("mod_small(%llu) mod %u wrong\n", (unsigned long long)x, p);
+        return 1;
+      }
+  for (uint32_t p : mprimes) {
+    const yhk::SmallPrime k = yhk::make_small_prime(p, 1);
+    if ((((uint64_t)k.bm << 32) | k.bl) != (uint64_t)(((unsigned __int128)1 << 64) / p)) {
+      printf("make_small_prime(%u): floor(2^64/p) words wrong\n", p);
+      return 1;
+    }
+    const uint64_t qtop = top / p;
+    const uint64_t special[] = {0, 1, p - 1, p, p + 1, 2ull * p - 1, top, top - 1, qtop * p, qtop * p - 1,
+                                (qtop - 1) * p + (p - 1), 0xFFFFFFFFull, 0x100000000ull, (1ull << 62), top - p};
+    const int NS = sizeof(special) / sizeof(special[0]);
+    for (int i = 0; i < NS + 400000; ++i) {
+      const uint64_t x = i < NS ? special[i] : (i & 1 ? next() & top : next() >> (next() % 63 + 1));
+      const uint32_t got = yhk::mod_small(x, k), want = (uint32_t)(x % p);
+      if (got != want) {
+        printf("mod_small(%llu) mod %u = %u, want %u\n", (unsigned long long)x, p, got, want);
+        return 1;
+      }
+      ++checks;
+    }
+  }
   printf("ok %ld\n", checks);
   return 0;
 }

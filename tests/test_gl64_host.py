@@ -3,7 +3,8 @@ and small-prime helpers (cuyashe_amd/csrc/gl64.hpp, modp.hpp), checked
 against 128-bit arithmetic: every twiddle exponent of the shift-only DFTs
 (mul_pow2, e in [-192, 400)) on the field's edge values and random elements,
 mul/add/sub/sub_full/reduce128/reduce_wide, the Montgomery product and
-reduction, and the small-prime finish."""
+reduction, the small-prime finish and the small-prime reduction of CRT sums
+(mod_small)."""
 import os
 import subprocess
 
