@@ -3,7 +3,9 @@
 t=1024, w=2^32 on one MI355X.
 
 A step is one batched multiply+keyswitch over `--batch` resident ciphertext
-pairs (hot path: CRT -> negacyclic Goldilocks NTTs -> pointwise -> INTT ->
+pairs (default 2048, where throughput saturates: every launch then runs many
+rounds of workgroups and the fixed cost per launch is a few per cent; the
+`batch_sweep` field reports 512, 1024 and 4096 beside it) (hot path: CRT -> negacyclic Goldilocks NTTs -> pointwise -> INTT ->
 ICRT with t/q rounding, then digit decomposition -> NTT -> multiply-
 accumulate with the resident keys -> INTT -> ICRT mod q). Inputs sit in HBM
 before the timed region. Prints one JSON line.
@@ -32,7 +34,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--sweep", type=str, default="512,1024,4096",
+                    help="other batch sizes timed after the main measurement (empty: none)")
     ap.add_argument("--plain-ops", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -61,12 +65,16 @@ SQ_FILE = os.path.join(ROOT, "profiles", "sq_counters_mul_keyswitch_n4096.md")
 VALU_PEAK_G = 256 * 2.4
 
 
-def measured_valu(kernel_symbol):
+def measured_valu(kernel_symbol, batch):
     """SQ_INSTS_VALU per launch of the kernel from the committed rocprofv3
     SQ-counter summary (the count is fixed by the code and the batch), or
-    None."""
+    None when the summary was taken at another batch size."""
+    import re
     try:
         with open(SQ_FILE) as f:
+            m = re.search(r"batch (\d+)", f.readline())
+            if not m or int(m.group(1)) != batch:
+                return None
             for line in f:
                 cells = [c.strip() for c in line.split("|")]
                 if len(cells) > 3 and cells[2] == "SQ_INSTS_VALU" and kernel_symbol in cells[1]:
@@ -321,6 +329,25 @@ def main():
                           "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()}}))
         return
 
+    # ---- throughput at other batch sizes (same data tiled; 3 warm-up and 20 timed
+    # steps each), right after the main timed region so the clocks are the same
+    sweep = []
+    for bs in [int(v) for v in args.sweep.split(",") if v.strip()]:
+        reps_s = (bs + uniq - 1) // uniq
+        s1 = ctx.to_device(np.ascontiguousarray(np.tile(A1, (reps_s, 1, 1))[:bs]))
+        s2 = ctx.to_device(np.ascontiguousarray(np.tile(A2, (reps_s, 1, 1))[:bs]))
+        so = ctx.alloc(bs * L * n * 8)
+        for _ in range(3):
+            ring.mul_keyswitch(evk, s1, s2, so, bs)
+        ctx.sync()
+        ts = time.perf_counter()
+        for _ in range(20):
+            ring.mul_keyswitch(evk, s1, s2, so, bs)
+        ctx.sync()
+        el = time.perf_counter() - ts
+        sweep.append({"batch": bs, "ops_per_s": round(bs * 20 / el, 1), "ms_per_step": round(1000.0 * el / 20, 4)})
+        del s1, s2, so
+
     # per-step latency (each step synchronised)
     lat = []
     for _ in range(max(10, args.steps)):
@@ -351,7 +378,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name, B),
                 "kernel": dom_name, "avg_ms": round(dom_avg, 5), "launches": dom_n}
     symbols = {"k_prod": "k_prod<12, true>", "k_ks_digits": "k_ks_digits<12, true>", "k_keyswitch": "k_ks_mac<12>"}
-    valu = measured_valu(symbols[dom_name])
+    valu = measured_valu(symbols[dom_name], B)
     valu_roofline = None
     if valu:
         ach = valu / (dom_avg * 1e-3) / 1e9
@@ -407,6 +434,7 @@ def main():
         "plain": {"value": round(plain_value, 1), "unit": "ops/s",
                   "p50_ms": round(percentile(plain_lat, 50), 4), "p99_ms": round(percentile(plain_lat, 99), 4)},
         "batched_vs_plain": round(value / plain_value, 2),
+        "batch_sweep": sweep,
         "ntt_stages": {"algorithmic_GBps": round(ntt_bytes / (ntt_ms * 1e-3) / 1e9, 1) if ntt_ms else None,
                        "frac_of_hbm": round(ntt_bytes / (ntt_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ntt_ms else None},
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items() if kk != "id"}
