@@ -66,6 +66,15 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_ICRT_COEFS
 #define YH_ICRT_COEFS 1
 #endif
+// minimum waves per SIMD of the reconstruction kernel for q of at most two
+// limbs (wider q takes the registers it needs)
+// 1: the coefficients of a thread in a loop (one in registers at a time)
+#ifndef YH_ICRT_NOUNROLL
+#define YH_ICRT_NOUNROLL 0
+#endif
+#ifndef YH_ICRT_WAVES2
+#define YH_ICRT_WAVES2 1
+#endif
 
 // Device view of one ring's tables (all pointers are device memory).
 struct RingDev {
@@ -147,6 +156,42 @@ __device__ __forceinline__ uint64_t center(uint32_t r, uint32_t p) {
 __device__ __forceinline__ uint64_t twist_centered(uint32_t r, uint32_t p, uint64_t w) {
   const bool neg = r > (p >> 1);
   return mul_small(neg ? p - r : r, neg ? gl::P - w : w);
+}
+
+// a 2^e mod P for a < 2^ABITS (ABITS <= 32) and 0 <= e < 96 known at
+// compile time: a plain shift while the product stays below 2^63 < P.
+template <int ABITS>
+__device__ __forceinline__ uint64_t shift_small(uint32_t a, int e) {
+  return e + ABITS <= 63 ? (uint64_t)a << e : gl::mul_pow2((uint64_t)a, e);
+}
+
+// Input slot c of a merged negacyclic transform (ntt16_fwd<LOGN, true>): the
+// centred residue of r in [0, p) times 2^(78 c), canonical. 2^e with
+// e >= 96 is -2^(e-96); the result is negative when exactly one of the
+// centred value and that sign is, and 0 stays 0.
+__device__ __forceinline__ uint64_t twist_slot(uint32_t r, uint32_t p, int c) {
+  int e = (78 * c) % 192;
+  const bool ng = e >= 96;
+  if (ng) e -= 96;
+  const uint32_t h = p >> 1;
+  const bool neg = r > h;
+  const uint32_t a = neg ? p - r : r;     // < 2^27
+  const bool f = ng ? r - 1u < h : neg;  // ng: negative iff the centred value is > 0
+  const uint64_t v = shift_small<27>(a, e);
+  return f ? gl::P - v : v;
+}
+
+// The same for a keyswitch digit d in [0, 2^w) centred as d - 2^(w-1)
+// (half = 2^(w-1) <= 2^31).
+__device__ __forceinline__ uint64_t twist_slot_digit(uint64_t d, uint64_t half, int c) {
+  int e = (78 * c) % 192;
+  const bool ng = e >= 96;
+  if (ng) e -= 96;
+  const bool neg = d < half;
+  const uint32_t a = (uint32_t)(neg ? half - d : d - half);  // <= 2^31
+  const bool f = ng ? d > half : neg;
+  const uint64_t v = shift_small<32>(a, e);
+  return f ? gl::P - v : v;
 }
 
 // CRT of one coefficient (LIN 64-bit limbs, limb-major with stride `ls`):
@@ -342,7 +387,9 @@ __device__ __forceinline__ void group_fence(uint64_t* x) {
                "+v"(x[7])::"memory");
 }
 
-template <int LOGN, bool NEGA>
+// MERGED: negacyclic input of ntt16_fwd<LOGN, true> (slot shifts only; the
+// twist table is not read).
+template <int LOGN, bool NEGA, bool MERGED = false>
 __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                          const uint64_t* pw, const SmallPrime& k, const uint64_t* twist,
                                          int g = threadIdx.x) {
@@ -353,14 +400,15 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
     for (int c = h; c < h + 8; ++c) {
       const int j = g + c * stride;
       const uint32_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, k) : 0u;
-      x[c] = NEGA ? twist_centered(r, k.p, twist[j]) : center(r, k.p);
+      x[c] = MERGED ? twist_slot(r, k.p, c) : NEGA ? twist_centered(r, k.p, twist[j]) : center(r, k.p);
     }
     if (h + 8 < E) group_fence(x + h);
   }
 }
 
-// Modulus 0 (P): exact residues mod P, twisted by psi^j, not centred.
-template <int LOGN, bool NEGA>
+// Modulus 0 (P): exact residues mod P, twisted by psi^j, not centred
+// (MERGED: by the slot factor 2^(78 c) only).
+template <int LOGN, bool NEGA, bool MERGED = false>
 __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                            const uint64_t* twist, int g) {
   constexpr int stride = 1 << (LOGN - 4);
@@ -370,7 +418,7 @@ __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int l
     for (int c = h; c < h + 8; ++c) {
       const int j = g + c * stride;
       const uint64_t r = j < ncoef ? crt_residue_P(a + j, lin, ls) : 0;
-      x[c] = NEGA ? gl::mul(r, twist[j]) : r;
+      x[c] = MERGED ? gl::mul_pow2(r, (78 * c) % 192) : NEGA ? gl::mul(r, twist[j]) : r;
     }
     if (h + 8 < E) group_fence(x + h);
   }
@@ -396,32 +444,38 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   const uint64_t* A = pa.a_list ? pa.a_list[pair] : pa.A + pair * pa.ostride_a;
   const uint64_t* Bp = pa.b_list ? pa.b_list[pair] : pa.B + pair * pa.ostride_b;
   const bool gold = i == 0;
+  // negacyclic products fold the twist into the first forward pass and the
+  // untwist into the last inverse pass (ntt16.hpp, MERGED)
+  constexpr bool M = NEGA && LOGN >= 5;
   const int g1 = opaque_tid(pair);
-  if (gold) load_crt_P<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.tab.twist, g1);
-  else load_crt<LOGN, NEGA>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, sk, pa.tab.twist, g1);
-  ntt16_fwd<LOGN>(xa, lds, pa.tab.fwd, g1);
+  if (gold) load_crt_P<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.tab.twist, g1);
+  else load_crt<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, sk, pa.tab.twist, g1);
+  ntt16_fwd<LOGN, M>(xa, lds, pa.tab.fwd, g1, pa.tab.fwd0);
   // The second operand re-reads the tables (L1 hits) rather than keeping the
   // first transform's twiddles live in registers (the empty asm hides that
   // the addresses repeat); fewer registers, more waves per SIMD.
   const uint64_t* twist2 = pa.tab.twist;
   const uint64_t* fwd2 = pa.tab.fwd;
-  asm volatile("" : "+s"(twist2), "+s"(fwd2));
+  const uint64_t* fwd02 = pa.tab.fwd0;
+  asm volatile("" : "+s"(twist2), "+s"(fwd2), "+s"(fwd02));
   const int g2 = opaque_tid(xa[E - 1]);
-  if (gold) load_crt_P<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
-  else load_crt<LOGN, NEGA>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, sk, twist2, g2);
+  if (gold) load_crt_P<LOGN, NEGA, M>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
+  else load_crt<LOGN, NEGA, M>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, sk, twist2, g2);
   __syncthreads();  // every thread has left the LDS of the first transform
-  ntt16_fwd<LOGN>(xb, lds, fwd2, g2);
+  ntt16_fwd<LOGN, M>(xb, lds, fwd2, g2, fwd02);
 #pragma unroll
   for (int c = 0; c < E; ++c) xa[c] = gl::mont_mul(xa[c], xb[c]);  // x 2^-64, cancelled by *_pw
   __syncthreads();
   const int g3 = opaque_tid(xa[0]);
-  ntt16_inv<LOGN>(xa, lds, pa.tab.inv, g3);
+  ntt16_inv<LOGN, M>(xa, lds, pa.tab.inv, g3, pa.tab.inv0_pw);
   uint32_t* y = pa.Y + (size_t)pair * y_rows(rd.set_R[pa.set]) * N;
   constexpr int stride = N >> 4;
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     const int j = g3 + c * stride;
-    const uint64_t v = NEGA ? gl::mont_mul(xa[c], pa.tab.untwist_pw[j]) : gl::mont_mul(xa[c], pa.tab.n_inv_pw);
+    const uint64_t v = M      ? xa[c]
+                       : NEGA ? gl::mont_mul(xa[c], pa.tab.untwist_pw[j])
+                              : gl::mont_mul(xa[c], pa.tab.n_inv_pw);
     store_y(y, N, i, j, finish_any(rd, pa.set, i, v));
   }
 }
@@ -523,13 +577,18 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSDIG_WAVES) k_ks_digits(
       const int j = g + c * stride;
       const uint64_t d = W32 ? (uint64_t)w32[2 * j] : digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
       // the centred digit d - 2^(w-1), twisted: |.| <= 2^31 times +-psi^j
-      const bool neg = d < half;
-      const uint64_t w = rd.tab.twist[j];
-      x[c] = mul_small((uint32_t)(neg ? half - d : d - half), neg ? gl::P - w : w);
+      // (merged transforms: times the slot factor 2^(78 c) only)
+      if constexpr (LOGN >= 5) {
+        x[c] = twist_slot_digit(d, half, c);
+      } else {
+        const bool neg = d < half;
+        const uint64_t w = rd.tab.twist[j];
+        x[c] = mul_small((uint32_t)(neg ? half - d : d - half), neg ? gl::P - w : w);
+      }
     }
     if (h + 8 < E) group_fence(x + h);
   }
-  ntt16_fwd<LOGN>(x, lds, rd.tab.fwd, g);
+  ntt16_fwd<LOGN, (LOGN >= 5)>(x, lds, rd.tab.fwd, g, rd.tab.fwd0);
   uint64_t* o = ka.D + ((size_t)ct * rd.lw + k) * N + g;
 #pragma unroll
   for (int c = 0; c < E; ++c) o[c * stride] = x[c];
@@ -575,7 +634,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
     for (int c = 0; c < 8; ++c) acc[h + c] = gl::reduce_wide(top[c], hi[c], lo[c]);
   }
   const int g2 = opaque_tid(acc[0]);
-  ntt16_inv<LOGN>(acc, lds, rd.tab.inv, g2);
+  constexpr bool M = LOGN >= 5;  // untwist folded into the last inverse pass
+  ntt16_inv<LOGN, M>(acc, lds, rd.tab.inv, g2, rd.tab.inv0_m);
   // the inverse gives sum_k d'_k * (gamma_k mod m_i) for the centred digits
   // d'_k = d_k - 2^(w-1); adding corr_i = 2^(w-1) sum_k (1 * gamma_k) mod m_i
   // restores sum_k d_k * gamma_k mod m_i
@@ -586,7 +646,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int j = g2 + c * stride;
-      const uint64_t v = gl::add(gl::mont_mul(acc[c], rd.tab.untwist_m[j]), cr[j]);
+      const uint64_t v = gl::add(M ? acc[c] : gl::mont_mul(acc[c], rd.tab.untwist_m[j]), cr[j]);
       store_y(y, N, 0, j, gl::mul(v, invM));
     }
   } else {
@@ -594,7 +654,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int j = g2 + c * stride;
-      store_y(y, N, i, j, finish_small(gl::mont_mul(acc[c], rd.tab.untwist_m[j]), sp, (uint32_t)cr[j]));
+      store_y(y, N, i, j, finish_small(M ? acc[c] : gl::mont_mul(acc[c], rd.tab.untwist_m[j]), sp, (uint32_t)cr[j]));
     }
   }
 }
@@ -1192,12 +1252,16 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
 }
 
 template <int LQ>
-__global__ void __launch_bounds__(256) k_icrt(RingDev rd, IcrtArgs ia) {
+__global__ void __launch_bounds__(256, LQ <= 2 ? YH_ICRT_WAVES2 : 1) k_icrt(RingDev rd, IcrtArgs ia) {
   const int pair = blockIdx.y;
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
   const uint32_t* Y = ia.Y + (size_t)pair * y_rows(RS) * ia.N;
+#if YH_ICRT_NOUNROLL
+#pragma unroll 1
+#else
 #pragma unroll
+#endif
   for (int u = 0; u < YH_ICRT_COEFS; ++u) {
     const int j = (blockIdx.x * YH_ICRT_COEFS + u) * blockDim.x + threadIdx.x;
     if (j < ia.ncoef)

@@ -37,6 +37,9 @@ struct NttTab {
   const uint64_t* twist;       // psi^j
   const uint64_t* untwist_pw;  // psi^-j N^-1 2^128 (after a mont_mul pointwise product)
   const uint64_t* untwist_m;   // psi^-j N^-1 2^64 (Montgomery form)
+  const uint64_t* fwd0;        // first forward pass with the twist folded in: w_N^(brev4(c) g) psi^g
+  const uint64_t* inv0_pw;     // last inverse pass with the untwist folded in, 2^128
+  const uint64_t* inv0_m;      // the same, 2^64 (Montgomery form)
   uint64_t n_inv_pw;           // N^-1 2^128 (cyclic products)
 };
 
@@ -135,11 +138,19 @@ __device__ __forceinline__ void ld16(const uint64_t* lds, uint64_t* x, int base,
 // g is the thread's index within the transform (threadIdx.x unless several
 // transforms share a workgroup); __syncthreads spans the whole workgroup, so
 // every thread of it calls the same transform.
-template <int LOGN>
+//
+// MERGED (negacyclic, LOGN >= 5): the caller has multiplied input slot c by
+// 2^(78 c) only, and the first pass multiplies every output slot c by
+// tw0[c N/16 + g] = w_N^(brev4(c) g) psi^g (Montgomery form): psi^g is common
+// to the thread's 16 inputs (elements g + c N/16, psi^(g + c N/16) = psi^g
+// 2^(78 c)) and passes through the 16-point DFT, so the result is the twisted
+// transform, bit for bit.
+template <int LOGN, bool MERGED = false>
 __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ tw,
-                                          int g = threadIdx.x) {
+                                          int g = threadIdx.x, const uint64_t* __restrict__ tw0 = nullptr) {
   constexpr int NFULL = LOGN / 4;
   constexpr int SHORT = LOGN % 4;
+  static_assert(!MERGED || LOGN >= 5, "the merged first pass needs N >= 32");
 #pragma unroll
   for (int p = 0; p < NFULL; ++p) {
     const int b0 = LOGN - 4 - 4 * p;
@@ -149,7 +160,11 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
       __syncthreads();
     }
     dif_r<16>(x);
-    if (b0 > 0) {
+    if (MERGED && p == 0) {
+      const uint64_t* t = tw0 + g;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[c << b0]);
+    } else if (b0 > 0) {
       const int s = 1 << b0;
       const uint64_t* t = tw + pass_offset(LOGN, p) + (g & (s - 1));
 #pragma unroll
@@ -171,11 +186,17 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
 // Inverse (unscaled, no untwist). Entry: positions 16g + c as left by
 // ntt16_fwd. Exit: x holds elements g + c*N/16. Callers synchronise before
 // calling if another transform used the LDS just before.
-template <int LOGN>
+//
+// MERGED (negacyclic, LOGN >= 5): the last pass multiplies every input slot c
+// by twi0[c N/16 + g] = w_N^(-brev4(c) g) psi^-g N^-1 (times the caller's
+// Montgomery constant) and output slot c (element g + c N/16) by 2^(-78 c):
+// the exit values are untwisted and scaled by N^-1.
+template <int LOGN, bool MERGED = false>
 __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ twi,
-                                          int g = threadIdx.x) {
+                                          int g = threadIdx.x, const uint64_t* __restrict__ twi0 = nullptr) {
   constexpr int NFULL = LOGN / 4;
   constexpr int SHORT = LOGN % 4;
+  static_assert(!MERGED || LOGN >= 5, "the merged last pass needs N >= 32");
   if (SHORT) {
     constexpr int R = 1 << SHORT;
 #pragma unroll
@@ -191,7 +212,11 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
       ld16(lds, x, base, b0);
       __syncthreads();
     }
-    if (b0 > 0) {
+    if (MERGED && p == 0) {
+      const uint64_t* t = twi0 + g;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[c << b0]);
+    } else if (b0 > 0) {
       const int s = 1 << b0;
       const uint64_t* t = twi + pass_offset(LOGN, p) + (g & (s - 1));
 #pragma unroll
@@ -202,6 +227,10 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
       st16(lds, x, base, b0);
       __syncthreads();
     }
+  }
+  if (MERGED) {
+#pragma unroll
+    for (int c = 1; c < 16; ++c) x[c] = gl::mul_pow2(x[c], (192 - (78 * c) % 192) % 192);
   }
 }
 

@@ -334,8 +334,16 @@ struct RingParams {
 // fwd/inv in Montgomery form (applied with gl::mont_mul); untwist_pw and
 // n_inv_pw carry 2^128 (Montgomery form and the 2^-64 of a mont_mul
 // pointwise product), untwist_m 2^64.
+// Negacyclic transforms with the twist folded into the first forward pass
+// and the untwist into the last inverse pass (s = N/16, g < s, c < 16, the
+// pass touches elements j = g + c s): psi^j = psi^g 2^(78 c) because
+// psi^s = 7^((P-1)/32) = 2^78, so the per-thread factor psi^g joins the
+// pass's twiddles and the per-slot factor is a shift:
+//   fwd0[c s + g] = w_N^(brev4(c) g) psi^g                  (Montgomery form)
+//   inv0_m[c s + g] = w_N^(-brev4(c) g) psi^-g N^-1         (Montgomery form)
+//   inv0_pw = inv0_m carrying 2^128 (after a mont_mul pointwise product).
 struct Ntt16Tables {
-  std::vector<uint64_t> fwd, inv, twist, untwist_pw, untwist_m;
+  std::vector<uint64_t> fwd, inv, twist, untwist_pw, untwist_m, fwd0, inv0_pw, inv0_m;
   uint64_t n_inv_pw = 0;
 };
 
@@ -376,6 +384,21 @@ inline Ntt16Tables ntt16_tables(int N) {
     t.untwist_pw[j] = gl::to_mont(t.untwist_m[j]);
     a = gl::mul(a, psi);
     ai = gl::mul(ai, psii);
+  }
+  const int s = N / 16;
+  const uint64_t wi = gl::inv(w);
+  t.fwd0.resize(N);
+  t.inv0_m.resize(N);
+  t.inv0_pw.resize(N);
+  for (int c = 0; c < 16; ++c) {
+    const int br = ((c & 1) << 3) | ((c & 2) << 1) | ((c & 4) >> 1) | ((c & 8) >> 3);
+    for (int g = 0; g < s; ++g) {
+      const uint64_t f = gl::mul(gl::pow(w, (uint64_t)br * g), t.twist[g]);
+      const uint64_t v = gl::mul(gl::mul(gl::pow(wi, (uint64_t)br * g), gl::pow(psii, (uint64_t)g)), n_inv);
+      t.fwd0[(size_t)c * s + g] = gl::to_mont(f);
+      t.inv0_m[(size_t)c * s + g] = gl::to_mont(v);
+      t.inv0_pw[(size_t)c * s + g] = gl::to_mont(gl::to_mont(v));
+    }
   }
   return t;
 }

@@ -173,6 +173,9 @@ int upload_ntt16(int N, uint64_t** buf, yhk::NttTab* tab) {
   size_t o_t = h.size(); h.insert(h.end(), t.twist.begin(), t.twist.end());
   size_t o_u = h.size(); h.insert(h.end(), t.untwist_pw.begin(), t.untwist_pw.end());
   size_t o_m = h.size(); h.insert(h.end(), t.untwist_m.begin(), t.untwist_m.end());
+  size_t o_f0 = h.size(); h.insert(h.end(), t.fwd0.begin(), t.fwd0.end());
+  size_t o_ip = h.size(); h.insert(h.end(), t.inv0_pw.begin(), t.inv0_pw.end());
+  size_t o_im = h.size(); h.insert(h.end(), t.inv0_m.begin(), t.inv0_m.end());
   YH_HIP(hipMalloc(buf, h.size() * sizeof(uint64_t)));
   YH_HIP(hipMemcpy(*buf, h.data(), h.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   tab->fwd = *buf + o_f;
@@ -180,6 +183,9 @@ int upload_ntt16(int N, uint64_t** buf, yhk::NttTab* tab) {
   tab->twist = *buf + o_t;
   tab->untwist_pw = *buf + o_u;
   tab->untwist_m = *buf + o_m;
+  tab->fwd0 = *buf + o_f0;
+  tab->inv0_pw = *buf + o_ip;
+  tab->inv0_m = *buf + o_im;
   tab->n_inv_pw = t.n_inv_pw;
   return YH_OK;
 }
