@@ -66,6 +66,10 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_ICRT_COEFS
 #define YH_ICRT_COEFS 1
 #endif
+// threads per workgroup of the reconstruction kernel
+#ifndef YH_ICRT_BLOCK
+#define YH_ICRT_BLOCK 256
+#endif
 // minimum waves per SIMD of the reconstruction kernel for q of at most two
 // limbs (wider q takes the registers it needs)
 // 1: the coefficients of a thread in a loop (one in registers at a time)
@@ -446,7 +450,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   const bool gold = i == 0;
   // negacyclic products fold the twist into the first forward pass and the
   // untwist into the last inverse pass (ntt16.hpp, MERGED)
-  constexpr bool M = NEGA && LOGN >= 5;
+  constexpr bool M = NEGA;
   const int g1 = opaque_tid(pair);
   if (gold) load_crt_P<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.tab.twist, g1);
   else load_crt<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, sk, pa.tab.twist, g1);
@@ -578,17 +582,11 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSDIG_WAVES) k_ks_digits(
       const uint64_t d = W32 ? (uint64_t)w32[2 * j] : digit_of(c0 + j, N, rd.LQ, k, rd.w_bits);
       // the centred digit d - 2^(w-1), twisted: |.| <= 2^31 times +-psi^j
       // (merged transforms: times the slot factor 2^(78 c) only)
-      if constexpr (LOGN >= 5) {
-        x[c] = twist_slot_digit(d, half, c);
-      } else {
-        const bool neg = d < half;
-        const uint64_t w = rd.tab.twist[j];
-        x[c] = mul_small((uint32_t)(neg ? half - d : d - half), neg ? gl::P - w : w);
-      }
+      x[c] = twist_slot_digit(d, half, c);
     }
     if (h + 8 < E) group_fence(x + h);
   }
-  ntt16_fwd<LOGN, (LOGN >= 5)>(x, lds, rd.tab.fwd, g, rd.tab.fwd0);
+  ntt16_fwd<LOGN, true>(x, lds, rd.tab.fwd, g, rd.tab.fwd0);
   uint64_t* o = ka.D + ((size_t)ct * rd.lw + k) * N + g;
 #pragma unroll
   for (int c = 0; c < E; ++c) o[c * stride] = x[c];
@@ -634,7 +632,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
     for (int c = 0; c < 8; ++c) acc[h + c] = gl::reduce_wide(top[c], hi[c], lo[c]);
   }
   const int g2 = opaque_tid(acc[0]);
-  constexpr bool M = LOGN >= 5;  // untwist folded into the last inverse pass
+  constexpr bool M = true;  // untwist folded into the last inverse pass
   ntt16_inv<LOGN, M>(acc, lds, rd.tab.inv, g2, rd.tab.inv0_m);
   // the inverse gives sum_k d'_k * (gamma_k mod m_i) for the centred digits
   // d'_k = d_k - 2^(w-1); adding corr_i = 2^(w-1) sum_k (1 * gamma_k) mod m_i
@@ -1252,7 +1250,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
 }
 
 template <int LQ>
-__global__ void __launch_bounds__(256, LQ <= 2 ? YH_ICRT_WAVES2 : 1) k_icrt(RingDev rd, IcrtArgs ia) {
+__global__ void __launch_bounds__(YH_ICRT_BLOCK, LQ <= 2 ? YH_ICRT_WAVES2 : 1) k_icrt(RingDev rd, IcrtArgs ia) {
   const int pair = blockIdx.y;
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;

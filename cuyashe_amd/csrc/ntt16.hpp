@@ -139,18 +139,18 @@ __device__ __forceinline__ void ld16(const uint64_t* lds, uint64_t* x, int base,
 // transforms share a workgroup); __syncthreads spans the whole workgroup, so
 // every thread of it calls the same transform.
 //
-// MERGED (negacyclic, LOGN >= 5): the caller has multiplied input slot c by
+// MERGED (negacyclic): the caller has multiplied input slot c by
 // 2^(78 c) only, and the first pass multiplies every output slot c by
 // tw0[c N/16 + g] = w_N^(brev4(c) g) psi^g (Montgomery form): psi^g is common
 // to the thread's 16 inputs (elements g + c N/16, psi^(g + c N/16) = psi^g
 // 2^(78 c)) and passes through the 16-point DFT, so the result is the twisted
-// transform, bit for bit.
+// transform, bit for bit. (N = 16: one thread, psi^g = 1 and the table holds
+// 1; psi itself is 2^78.)
 template <int LOGN, bool MERGED = false>
 __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ tw,
                                           int g = threadIdx.x, const uint64_t* __restrict__ tw0 = nullptr) {
   constexpr int NFULL = LOGN / 4;
   constexpr int SHORT = LOGN % 4;
-  static_assert(!MERGED || LOGN >= 5, "the merged first pass needs N >= 32");
 #pragma unroll
   for (int p = 0; p < NFULL; ++p) {
     const int b0 = LOGN - 4 - 4 * p;
@@ -187,7 +187,7 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
 // ntt16_fwd. Exit: x holds elements g + c*N/16. Callers synchronise before
 // calling if another transform used the LDS just before.
 //
-// MERGED (negacyclic, LOGN >= 5): the last pass multiplies every input slot c
+// MERGED (negacyclic): the last pass multiplies every input slot c
 // by twi0[c N/16 + g] = w_N^(-brev4(c) g) psi^-g N^-1 (times the caller's
 // Montgomery constant) and output slot c (element g + c N/16) by 2^(-78 c):
 // the exit values are untwisted and scaled by N^-1.
@@ -196,7 +196,6 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
                                           int g = threadIdx.x, const uint64_t* __restrict__ twi0 = nullptr) {
   constexpr int NFULL = LOGN / 4;
   constexpr int SHORT = LOGN % 4;
-  static_assert(!MERGED || LOGN >= 5, "the merged last pass needs N >= 32");
   if (SHORT) {
     constexpr int R = 1 << SHORT;
 #pragma unroll
