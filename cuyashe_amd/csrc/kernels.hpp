@@ -954,6 +954,17 @@ __device__ __forceinline__ bool twice_gt_q(const uint64_t* r, const uint64_t* q)
   return !geq_n<LQ + 1>(qe, d);
 }
 
+// k_icrt's pipelined path (two coefficients per thread, rows held in
+// registers): q of at most two limbs, at most ICRT_PIPE_ROWS scratch rows,
+// mod-q and scale-and-round epilogues. The host sizes the grid with it.
+#ifndef YH_ICRT_PIPE
+#define YH_ICRT_PIPE 1
+#endif
+constexpr int ICRT_PIPE_ROWS = 12;
+__host__ __device__ inline bool icrt_pipelined(int LQ, int RS, int mode) {
+  return YH_ICRT_PIPE && LQ <= 2 && RS + 1 <= ICRT_PIPE_ROWS && (mode == EPI_MODQ || mode == EPI_SCALE);
+}
+
 struct IcrtArgs {
   const uint32_t* Y;     // [pair][R_S][N]
   int N;                 // row length of Y
@@ -1128,6 +1139,66 @@ __device__ __forceinline__ void icrt_mod_q(const RingDev& rd, const uint32_t* y,
   divmod_v<LQ>(v, rd, quo, rem);
 }
 
+// Scale-and-round epilogue from the column sums of icrt_pass (NF = 2 over
+// beta, alpha): writes round(t c / q) mod q (EPI_SCALE in icrt_coef).
+template <int LQ>
+__device__ __forceinline__ void scale_finish(const RingDev& rd, int set, const uint64_t (&S)[2][2 * LQ], uint64_t yP,
+                                             uint32_t k, uint64_t* o, size_t out_stride) {
+  constexpr int LMU = 2 * LQ + 3 - LQ;
+  const int RS = rd.set_R[set];
+  uint64_t v[LQ + 1], quo[LMU], rem[LQ];
+  fold_finish<LQ>(S[0], rd.beta, RS, rd.negbetaM, yP, k, v);
+  divmod_v<LQ>(v, rd, quo, rem);  // S' = quo q + rem
+  const uint64_t up = twice_gt_q<LQ>(rem, rd.q) ? 1 : 0;
+  // result = sum y_i alpha_i + k (q - (alpha_M + 1)) + quo + up  (mod q);
+  // quo < 2^62 with fast_div, < 2^99 otherwise: the sum fits LQ + 1 limbs
+  fold_finish<LQ>(S[1], rd.alpha, RS, rd.negalpha1, yP, k, v);
+  uint64_t add[LQ + 1];
+#pragma unroll
+  for (int l = 0; l < LQ + 1; ++l) add[l] = quo[l];
+  add_n<LQ + 1>(v, add);
+#pragma unroll
+  for (int l = 0; l < LQ + 1; ++l) add[l] = l == 0 ? up : 0;
+  add_n<LQ + 1>(v, add);
+  divmod_v<LQ>(v, rd, quo, rem);
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = rem[l];
+}
+
+// icrt_pass over y rows already in registers (yv[r] = row r, rows 0..RS,
+// RS + 1 <= MR): the loop bound is the compile-time MR, so the rows stay in
+// registers (k_icrt's pipelined path).
+template <int LQ, int NF, int MR>
+__device__ __forceinline__ uint32_t icrt_pass_regs(const RingDev& rd, const uint32_t (&yv)[MR], int set,
+                                                   const uint64_t* const* X, uint64_t (&S)[NF][2 * LQ],
+                                                   uint64_t& yP) {
+  constexpr int NX = 2 * LQ;
+  const int RS = rd.set_R[set];
+  const uint64_t* bar = rd.barrett[set];
+  yP = (uint64_t)yv[0] | ((uint64_t)yv[1] << 32);
+  uint64_t flo = yP + gl::umulhi(yP, gl::EPS), fhi = 0;
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int w = 0; w < NX; ++w) S[f][w] = 0;
+#pragma unroll
+  for (int i = 1; i + 1 < MR; ++i) {
+    if (i < RS) {
+      const uint64_t yi = yv[i + 1];
+      const uint64_t fr = yi * bar[i];
+      flo += fr;
+      fhi += (flo < fr);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const uint64_t* Xi = X[f] + (size_t)i * LQ;
+#pragma unroll
+        for (int w = 0; w < NX; ++w) S[f][w] += yi * word_of(Xi, w);
+      }
+    }
+  }
+  return (uint32_t)(fhi + (flo >> 63));
+}
+
 // ICRT of one coefficient plus its epilogue. y points at the operand's first
 // scratch row (rows ystride apart, layout of store_y), o at limb 0 of the
 // output coefficient. EPI_RAW writes the reconstructed integer itself (in
@@ -1184,25 +1255,8 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
     // is non-negative: floor(S / q) = floor(S' / q) - k.
     uint64_t yP, S[2][2 * LQ];
     const uint64_t* X[2] = {rd.beta, rd.alpha};
-    const int RS = rd.set_R[set];
     const uint32_t k = icrt_pass<LQ, 2>(rd, y, ystride, set, X, S, yP);
-    uint64_t v[LQ + 1], quo[LMU], rem[LQ];
-    fold_finish<LQ>(S[0], rd.beta, RS, rd.negbetaM, yP, k, v);
-    divmod_v<LQ>(v, rd, quo, rem);  // S' = quo q + rem
-    const uint64_t up = twice_gt_q<LQ>(rem, rd.q) ? 1 : 0;
-    // result = sum y_i alpha_i + k (q - (alpha_M + 1)) + quo + up  (mod q);
-    // quo < 2^62 with fast_div, < 2^99 otherwise: the sum fits LQ + 1 limbs
-    fold_finish<LQ>(S[1], rd.alpha, RS, rd.negalpha1, yP, k, v);
-    uint64_t add[LQ + 1];
-#pragma unroll
-    for (int l = 0; l < LQ + 1; ++l) add[l] = quo[l];
-    add_n<LQ + 1>(v, add);
-#pragma unroll
-    for (int l = 0; l < LQ + 1; ++l) add[l] = l == 0 ? up : 0;
-    add_n<LQ + 1>(v, add);
-    divmod_v<LQ>(v, rd, quo, rem);
-#pragma unroll
-    for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = rem[l];
+    scale_finish<LQ>(rd, set, S, yP, k, o, out_stride);
     return;
   }
   // EPI_RAW: the integer itself.
@@ -1255,6 +1309,42 @@ __global__ void __launch_bounds__(YH_ICRT_BLOCK, LQ <= 2 ? YH_ICRT_WAVES2 : 1) k
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
   const uint32_t* Y = ia.Y + (size_t)pair * y_rows(RS) * ia.N;
+  if (icrt_pipelined(LQ, RS, ia.mode)) {
+    // two coefficients per thread: the second one's rows are requested
+    // before the first one is computed, so its loads overlap that work
+    constexpr int MR = ICRT_PIPE_ROWS;
+    const int nrow = y_rows(RS);
+    uint32_t cur[MR], nxt[MR];
+    const int j0 = blockIdx.x * 2 * blockDim.x + threadIdx.x, j1 = j0 + blockDim.x;
+#pragma unroll
+    for (int r = 0; r < MR; ++r) cur[r] = (r < nrow && j0 < ia.ncoef) ? Y[(size_t)r * ia.N + j0] : 0u;
+#pragma unroll
+    for (int r = 0; r < MR; ++r) nxt[r] = (r < nrow && j1 < ia.ncoef) ? Y[(size_t)r * ia.N + j1] : 0u;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = u ? j1 : j0;
+      const uint32_t(&yv)[MR] = u ? nxt : cur;
+      if (j < ia.ncoef) {
+        uint64_t yP;
+        if (ia.mode == EPI_SCALE) {
+          uint64_t S[2][2 * LQ];
+          const uint64_t* X[2] = {rd.beta, rd.alpha};
+          const uint32_t k = icrt_pass_regs<LQ, 2, MR>(rd, yv, ia.set, X, S, yP);
+          scale_finish<LQ>(rd, ia.set, S, yP, k, o + j, ia.out_stride);
+        } else {
+          constexpr int LMU = 2 * LQ + 3 - LQ;
+          uint64_t S[1][2 * LQ], v[LQ + 1], quo[LMU], rem[LQ];
+          const uint64_t* X[1] = {rd.Mq[ia.set]};
+          const uint32_t k = icrt_pass_regs<LQ, 1, MR>(rd, yv, ia.set, X, S, yP);
+          fold_finish<LQ>(S[0], rd.Mq[ia.set], RS, rd.negMq[ia.set], yP, k, v);
+          divmod_v<LQ>(v, rd, quo, rem);
+#pragma unroll
+          for (int l = 0; l < LQ; ++l) o[j + (size_t)l * ia.out_stride] = rem[l];
+        }
+      }
+    }
+    return;
+  }
 #if YH_ICRT_NOUNROLL
 #pragma unroll 1
 #else

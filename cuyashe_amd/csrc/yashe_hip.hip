@@ -263,7 +263,7 @@ int launch_prod(yh_ring* r, yhk::ProdArgs pa, int N, int batch, bool negacyclic)
 
 template <int LQ>
 void launch_icrt_t(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
-  constexpr int per = YH_ICRT_BLOCK * YH_ICRT_COEFS;
+  const int per = YH_ICRT_BLOCK * (yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode) ? 2 : YH_ICRT_COEFS);
   dim3 block(YH_ICRT_BLOCK), grid((ia.ncoef + per - 1) / per, batch);
   hipLaunchKernelGGL(yhk::k_icrt<LQ>, grid, block, 0, r->stream, r->rd, ia);
 }
