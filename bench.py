@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=2048)
     ap.add_argument("--sweep", type=str, default="512,1024,4096",
                     help="other batch sizes timed after the main measurement (empty: none)")
+    ap.add_argument("--no-two-streams", dest="two_streams", action="store_false",
+                    help="skip the two-stream measurement")
     ap.add_argument("--plain-ops", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -348,6 +350,40 @@ def main():
         sweep.append({"batch": bs, "ops_per_s": round(bs * 20 / el, 1), "ms_per_step": round(1000.0 * el / 20, 4)})
         del s1, s2, so
 
+    # ---- two independent batches in flight: steps alternate between two
+    # rings of the same parameters on streams of their own (default
+    # priority), so one batch's latency-bound reconstruction overlaps the
+    # other's transforms. Reported beside the single-stream value (whose
+    # per-kernel event times stay those of unshared launches).
+    two = None
+    if args.two_streams:
+        rings2 = [ya.Ring(ctx, N_RING, Q, t=T, w_bits=W_BITS) for _ in range(2)]
+        evks2 = [r.evk(ctx.to_device(r.pack(gammas))) for r in rings2]
+        outs2 = [ctx.alloc(B * L * n * 8) for _ in rings2]
+        for r in rings2:
+            r.use_own_stream()
+        for _ in range(2):
+            for k2, r in enumerate(rings2):
+                r.mul_keyswitch(evks2[k2], d1, d2, outs2[k2], B)
+        for r in rings2:
+            r.sync()
+        ts = time.perf_counter()
+        for st in range(args.steps):
+            k2 = st % 2
+            rings2[k2].mul_keyswitch(evks2[k2], d1, d2, outs2[k2], B)
+        for r in rings2:
+            r.sync()
+        el = time.perf_counter() - ts
+        ref = out.download((B, L, n))
+        same = all(bool((o.download((B, L, n)) == ref).all()) for o in outs2)
+        two = {"value": round(B * args.steps / el, 1), "unit": "ops/s", "ms_per_step": round(1000.0 * el / args.steps, 4),
+               "steps": args.steps, "identical_to_single_stream": same}
+        for e in evks2:
+            e.close()
+        for r in rings2:
+            r.close()
+        del outs2
+
     # per-step latency (each step synchronised)
     lat = []
     for _ in range(max(10, args.steps)):
@@ -438,6 +474,7 @@ def main():
                   "p50_ms": round(percentile(plain_lat, 50), 4), "p99_ms": round(percentile(plain_lat, 99), 4)},
         "batched_vs_plain": round(value / plain_value, 2),
         "batch_sweep": sweep,
+        "two_streams": two,
         "ntt_stages": {"algorithmic_GBps": round(ntt_bytes / (ntt_ms * 1e-3) / 1e9, 1) if ntt_ms else None,
                        "frac_of_hbm": round(ntt_bytes / (ntt_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if ntt_ms else None},
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items() if kk != "id"}
