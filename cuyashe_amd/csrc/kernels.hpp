@@ -42,10 +42,9 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_MIN_WAVES
 #define YH_MIN_WAVES 2
 #endif
-// The fused product kernel runs at four waves per SIMD (128 VGPRs; at
-// n = 4096 the compiler spills 68 bytes per lane, and four waves still
-// measure ~6 % faster than three without spills at 140 VGPRs); the keyswitch
-// kernels fit four or more without spills.
+// The fused product kernel runs at four waves per SIMD (119 VGPRs at
+// n = 4096, no spills; five waves spill); the keyswitch kernels fit four or
+// more without spills.
 #ifndef YH_PROD_WAVES
 #define YH_PROD_WAVES 4
 #endif
@@ -54,30 +53,21 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_KSMAC_WAVES
 #define YH_KSMAC_WAVES 4
 #endif
-// The digit transforms (k_ks_digits, 77 VGPRs) run at six waves; forcing
+// The digit transforms (k_ks_digits, 79 VGPRs) run at six waves; forcing
 // eight (64 VGPRs, small spills) measured slower. The reconstruction
-// (k_icrt) takes the registers its limb count needs (66 VGPRs, seven waves,
-// at 2 limbs; more for wider q, where a wave bound would force spills).
+// (k_icrt) takes the registers its limb count needs (112 VGPRs with the
+// two-coefficient pipelined path at two limbs; a wave bound makes it spill).
 #ifndef YH_KSDIG_WAVES
 #define YH_KSDIG_WAVES 6
 #endif
-// coefficients per thread of the reconstruction kernel (two measured
-// slower than one at C2)
+// coefficients per thread of the reconstruction kernel outside its
+// pipelined path (two measured slower than one at n = 4096, two limbs)
 #ifndef YH_ICRT_COEFS
 #define YH_ICRT_COEFS 1
 #endif
 // threads per workgroup of the reconstruction kernel
 #ifndef YH_ICRT_BLOCK
 #define YH_ICRT_BLOCK 256
-#endif
-// minimum waves per SIMD of the reconstruction kernel for q of at most two
-// limbs (wider q takes the registers it needs)
-// 1: the coefficients of a thread in a loop (one in registers at a time)
-#ifndef YH_ICRT_NOUNROLL
-#define YH_ICRT_NOUNROLL 0
-#endif
-#ifndef YH_ICRT_WAVES2
-#define YH_ICRT_WAVES2 1
 #endif
 
 // Device view of one ring's tables (all pointers are device memory).
@@ -1304,7 +1294,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
 }
 
 template <int LQ>
-__global__ void __launch_bounds__(YH_ICRT_BLOCK, LQ <= 2 ? YH_ICRT_WAVES2 : 1) k_icrt(RingDev rd, IcrtArgs ia) {
+__global__ void __launch_bounds__(YH_ICRT_BLOCK) k_icrt(RingDev rd, IcrtArgs ia) {
   const int pair = blockIdx.y;
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
@@ -1345,11 +1335,7 @@ __global__ void __launch_bounds__(YH_ICRT_BLOCK, LQ <= 2 ? YH_ICRT_WAVES2 : 1) k
     }
     return;
   }
-#if YH_ICRT_NOUNROLL
-#pragma unroll 1
-#else
 #pragma unroll
-#endif
   for (int u = 0; u < YH_ICRT_COEFS; ++u) {
     const int j = (blockIdx.x * YH_ICRT_COEFS + u) * blockDim.x + threadIdx.x;
     if (j < ia.ncoef)

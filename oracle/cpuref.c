@@ -15,8 +15,8 @@
  *                 s_add/s_sub (:355-383) as exact arithmetic mod P
  *   pointwise     polynomialNTTMul (src/cuda_functions.cu:460-474)
  *   inverse scale NTTScale (src/cuda_functions.cu:397-402) - here a multiply by
- *                 N^-1 mod P; the reference's integer /N equals it whenever the
- *                 reference is exact (deviation recorded in DESIGN.md).
+ *                 N^-1 mod P; the reference's integer /N equals it whenever
+ *                 N c < P, i.e. whenever the reference itself is exact.
  *   ICRT          sum_i M_i * ((r_i * (M_i^-1 mod p_i)) mod p_i) mod M
  *                 (cuPreICRT / cuICRT, src/cuda_bn.cu:854-968; Integer::icrt,
  *                 src/integer.cpp:84-117)
