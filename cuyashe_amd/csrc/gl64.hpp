@@ -124,23 +124,22 @@ GL_HD uint64_t mul(uint64_t a, uint64_t b) {
 
 // Montgomery reduction with R = 2^64: hi 2^64 + lo (< P 2^64) times 2^-64
 // mod P, canonical. P^-1 = 1 + 2^32 (mod 2^64), so the quotient digit is
-// m = lo (1 + 2^32) mod 2^64 with no multiply, and floor(m P / 2^64) =
-// m - (m >> 32) - [m_lo > m_hi], where [m_lo > m_hi] is the carry e of
-// lo + (lo << 32). The result hi - floor(m P / 2^64) lies in (-P, P).
+// m = lo (1 + 2^32) mod 2^64 = (m1 : l0) with m1 = l1 + l0 mod 2^32 and no
+// multiply, and floor(m P / 2^64) = m - m1 - e, e the carry of l1 + l0. The
+// result hi - floor(m P / 2^64) = (hi + m1 + e) - m lies in (-P, P); the sum
+// hi + m1 + e stays below 2^64 (hi < P and m1 + e < 2^32), so it is one
+// 64-bit addition, one 64-bit subtraction and +P when negative.
 GL_HD uint64_t mont_reduce(uint64_t hi, uint64_t lo) {
   const uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32);
-  uint32_t e, br1, br2, x;
+  uint32_t e, c0, c1, br, br2, x;
   const uint32_t m1 = addc32(l1, l0, 0u, &e);  // m = (m1 : l0)
-  const uint32_t t0 = subc32(l0, 0u, e, &br1);
-  const uint32_t b0 = subc32(t0, m1, 0u, &br2);
-  uint32_t b1 = subc32(m1, 0u, br1, &x);
-  b1 = subc32(b1, 0u, br2, &x);
-  uint32_t c1, c2, c3;
-  const uint32_t r0 = subc32((uint32_t)hi, b0, 0u, &c1);
-  const uint32_t r1 = subc32((uint32_t)(hi >> 32), b1, c1, &c2);
+  const uint32_t a0 = addc32((uint32_t)hi, m1, e, &c0);
+  const uint32_t a1 = addc32((uint32_t)(hi >> 32), 0u, c0, &c1);  // no carry out
+  const uint32_t r0 = subc32(a0, l0, 0u, &br);
+  const uint32_t r1 = subc32(a1, m1, br, &br2);
   // negative: add P = 2^64 - EPS, i.e. subtract EPS modulo 2^64
-  const uint32_t s0 = subc32(r0, c2 ? 0xFFFFFFFFu : 0u, 0u, &c3);
-  const uint32_t s1 = subc32(r1, 0u, c3, &x);
+  const uint32_t s0 = subc32(r0, br2 ? 0xFFFFFFFFu : 0u, 0u, &x);
+  const uint32_t s1 = subc32(r1, 0u, x, &x);
   return ((uint64_t)s1 << 32) | s0;
 }
 
