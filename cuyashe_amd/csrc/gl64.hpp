@@ -95,11 +95,13 @@ GL_HD uint64_t reduce_wide(uint32_t top, uint64_t hi, uint64_t lo) {
   const uint64_t hl = hi & EPS;
   uint64_t t0 = lo - hh;
   if (lo < hh) t0 -= EPS;              // borrow: add P = 2^64 - EPS
-  const uint64_t t1 = (hl << 32) - hl;  // hl * EPS < 2^64
-  uint64_t t2 = t0 + t1;
-  if (t2 < t0) t2 += EPS;              // carry: 2^64 = EPS
-  if (t2 >= P) t2 -= P;
-  return t2;
+  const uint64_t t1 = (hl << 32) - hl;  // hl * EPS <= 2^64 - 2^33 + 1
+  const uint64_t t2 = t0 + t1;
+  // On carry the value is t2 + 2^64 = t2 + EPS (mod P), and then t2 + EPS <=
+  // P - 2 (t0 < 2^64, t1 <= 2^64 - 2^33 + 1); without carry t2 >= P needs
+  // t2 - P = t2 + EPS (mod 2^64). One conditional addition covers both.
+  const bool fix = (t2 < t0) | (t2 >= P);
+  return fix ? t2 + EPS : t2;
 }
 
 // Reduce hi * 2^64 + lo modulo P, canonical result.
