@@ -13,6 +13,7 @@ from oracle import cpuref
 pytestmark = pytest.mark.gpu
 
 P = cpuref.P
+Q61 = (1 << 61) - 1  # one 64-bit limb
 Q127 = (1 << 127) - 1
 Q255 = (1 << 255) - 19
 Q_POLY = 77287149995192912462927307869  # src/test.cu:45
@@ -166,7 +167,7 @@ def test_plain_simple_multiply(gpu_ctx, kats):
     ring.close()
 
 
-@pytest.mark.parametrize("n,q", [(8, Q_POLY), (64, Q127), (1024, Q127), (4096, Q127), (2048, Q255)])
+@pytest.mark.parametrize("n,q", [(8, Q_POLY), (64, Q127), (1024, Q127), (4096, Q127), (2048, Q255), (1024, Q61)])
 def test_plain_random(gpu_ctx, n, q):
     rng = random.Random(n)
     ring = ya.Ring(gpu_ctx, n, q)
@@ -204,7 +205,7 @@ def test_ring_mul_golden(gpu_ctx, kats):
 
 @pytest.mark.parametrize("n,q", [(2, Q_POLY), (4, Q_POLY), (8, Q127), (16, Q127), (32, Q127), (64, Q127),
                                  (1024, Q127), (4096, Q127), (8192, Q255), (4096, (1 << 511) - 187),
-                                 (16384, Q127), (256, 655615111)])
+                                 (16384, Q127), (256, 655615111), (4096, Q61)])
 def test_ring_mul_random(gpu_ctx, n, q):
     rng = random.Random(n * 7 + q % 1000)
     ring = ya.Ring(gpu_ctx, n, q)
@@ -288,7 +289,8 @@ def test_ring_addsub(gpu_ctx):
 
 # ------------------------------------------------------------------ YASHE operations
 
-@pytest.mark.parametrize("n,q,t", [(4, Q_POLY, 35951), (64, Q127, 1024), (4096, Q127, 1024), (1024, Q255, 1 << 16)])
+@pytest.mark.parametrize("n,q,t", [(4, Q_POLY, 35951), (64, Q127, 1024), (4096, Q127, 1024), (1024, Q255, 1 << 16),
+                                   (64, Q61, 17), (4096, Q61, 1024)])
 def test_yashe_mul(gpu_ctx, n, q, t):
     rng = random.Random(n + t)
     ring = ya.Ring(gpu_ctx, n, q, t=t, w_bits=32)
@@ -305,7 +307,7 @@ def test_yashe_mul(gpu_ctx, n, q, t):
 
 
 @pytest.mark.parametrize("n,q,wb", [(4, Q_POLY, 32), (64, Q127, 32), (4096, Q127, 32), (256, Q255, 16),
-                                    (1024, Q127, 24)])
+                                    (1024, Q127, 24), (4096, Q61, 32), (256, Q61, 16)])
 def test_keyswitch(gpu_ctx, n, q, wb):
     rng = random.Random(n * 3 + wb)
     ring = ya.Ring(gpu_ctx, n, q, t=1024, w_bits=wb)
