@@ -53,10 +53,11 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_KSMAC_WAVES
 #define YH_KSMAC_WAVES 4
 #endif
-// The digit transforms (k_ks_digits, 79 VGPRs) run at six waves; forcing
-// eight (64 VGPRs, small spills) measured slower. The reconstruction
-// (k_icrt) takes the registers its limb count needs (112 VGPRs with the
-// two-coefficient pipelined path at two limbs; a wave bound makes it spill).
+// The digit transforms (k_ks_digits, 79 VGPRs) may run six waves; forcing
+// eight (64 VGPRs, small spills) measured slower. At n = 4096 the LDS of a
+// transform (34 KB) admits four workgroups per CU, so every transform kernel
+// there runs four waves per SIMD. The reconstruction (k_icrt) has no LDS;
+// its wave bound is icrt_min_waves below.
 #ifndef YH_KSDIG_WAVES
 #define YH_KSDIG_WAVES 6
 #endif
