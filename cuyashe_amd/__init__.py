@@ -12,4 +12,5 @@ from ._lib import (  # noqa: F401
     KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH, KERNEL_KS_DIGITS,
     WIRE_CIPHERTEXT, WIRE_KEYSWITCH_KEY, WIRE_PUBLIC_KEY, WIRE_SECRET_KEY, WIRE_PLAINTEXT,
     wire_encode, wire_decode, wire_peek, wire_blob_size,
+    YH_ADD, YH_SUB, YH_MUL,
 )

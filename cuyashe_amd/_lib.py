@@ -21,6 +21,7 @@ YH_FORWARD = 1
 YH_INVERSE = 0
 YH_ADD = 0
 YH_SUB = 1
+YH_MUL = 2
 
 OP_RING_MUL, OP_YASHE_MUL, OP_KEYSWITCH, OP_MUL_KS, OP_ADD = range(5)
 KERNEL_PROD, KERNEL_KEYSWITCH, KERNEL_ICRT, KERNEL_BATCH, KERNEL_KS_DIGITS = range(5)
@@ -93,6 +94,8 @@ _SIGS = {
     "yh_ntt": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "yh_pointwise_mulmod": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
     "yh_residue_addsub": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "yh_residue_scalar": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _u64p, ctypes.c_int]),
+    "yh_ring_scalar": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]),
     "yh_poly_mul_plain": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int)]),
     "yh_poly_mul_plain_length": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -174,6 +177,15 @@ def check(rc: int, what: str = ""):
 def int_to_limbs(v: int, L: int) -> List[int]:
     mask = (1 << 64) - 1
     return [(v >> (64 * l)) & mask for l in range(L)]
+
+
+def _scalar_limbs(v: int):
+    """(ctypes limb array, limb count) of a non-negative integer."""
+    v = int(v)
+    if v < 0:
+        raise YashError("scalars are non-negative integers")
+    L = max(1, (v.bit_length() + 63) // 64)
+    return (ctypes.c_uint64 * L)(*int_to_limbs(v, L)), L
 
 
 def ints_to_array(values: Sequence[int], L: int) -> np.ndarray:
@@ -373,8 +385,10 @@ class Context:
         return Graph(h.value)
 
     def close(self):
+        """Destroys the context; refused (YashError, the context stays
+        usable) while rings, batch plans or graphs created on it are alive."""
         if self.h:
-            lib().yh_ctx_destroy(self.h)
+            check(lib().yh_ctx_destroy(self.h), "yh_ctx_destroy")
             self.h = None
 
 
@@ -505,6 +519,18 @@ class Ring:
         check(lib().yh_residue_addsub(self.h, c.ptr, a.ptr, b.ptr, spacing, YH_SUB if sub else YH_ADD),
               "yh_residue_addsub")
 
+    def residue_scalar(self, c: DeviceBuffer, a: DeviceBuffer, spacing: int, op: int, scalar: int):
+        """op (YH_ADD / YH_SUB on coefficient 0, YH_MUL on all) with a scalar
+        on residues [R][spacing], mod each CRT modulus."""
+        s, L = _scalar_limbs(scalar)
+        check(lib().yh_residue_scalar(self.h, c.ptr, a.ptr, spacing, op, s, L), "yh_residue_scalar")
+
+    def scalar(self, a: DeviceBuffer, op: int, scalar: int, out: DeviceBuffer, batch: int = 1):
+        """Ring elements: op (YH_ADD / YH_SUB on coefficient 0, YH_MUL on all)
+        with scalar mod q."""
+        s, L = _scalar_limbs(scalar)
+        check(lib().yh_ring_scalar(self.h, a.ptr, s, L, out.ptr, batch, op), "yh_ring_scalar")
+
     def poly_mul_plain(self, a: DeviceBuffer, deg_a: int, b: DeviceBuffer, deg_b: int, out: DeviceBuffer,
                        out_limbs: int, min_N: int = 1) -> int:
         N = ctypes.c_int(0)
@@ -567,7 +593,7 @@ class Ring:
 
     def close(self):
         if self.h:
-            lib().yh_ring_destroy(self.h)
+            check(lib().yh_ring_destroy(self.h), "yh_ring_destroy")
             self.h = None
 
 
@@ -605,12 +631,14 @@ class Batch:
 
     def close(self):
         if self.h:
-            lib().yh_batch_destroy(self.h)
+            check(lib().yh_batch_destroy(self.h), "yh_batch_destroy")
             self.h = None
 
     def __del__(self):
+        # never after the context is gone (interpreter shutdown)
         try:
-            self.close()
+            if self.h and self.ctx.h:
+                self.close()
         except Exception:
             pass
 
@@ -640,5 +668,5 @@ class Evk:
 
     def close(self):
         if self.h:
-            lib().yh_evk_destroy(self.h)
+            check(lib().yh_evk_destroy(self.h), "yh_evk_destroy")
             self.h = None

@@ -1482,6 +1482,98 @@ __global__ void k_addsub_q(RingDev rd, const uint64_t* A, const uint64_t* B, uin
   for (int l = 0; l < LQ; ++l) c[(size_t)l * n] = x[l];
 }
 
+// ---------------------------------------------------------------- scalar operations
+// The reference's polynomialOPInteger (src/cuda_functions.cu:477-548): with
+// op ADD / SUB the scalar is added to / subtracted from coefficient 0 only,
+// with MUL every coefficient is multiplied by it (add_mode_t values,
+// src/settings.h:36).
+enum ScalarOp { SC_ADD = 0, SC_SUB = 1, SC_MUL = 2 };
+
+// Scalar s mod q as LQ limbs (k_scalar_q) or its residues mod each CRT
+// modulus (k_residue_scalar), passed by value.
+struct ScalarArg {
+  uint64_t v[64];
+};
+
+// Ring elements [batch][LQ][n] in [0, q): c = a (+,-) s on coefficient 0
+// (the others copied), or c = a * s mod q on every coefficient; s in [0, q).
+// The product of two LQ-limb values is reduced with the ring's Barrett
+// reciprocal (divmod_q over its 2 LQ limbs). c may alias a.
+template <int LQ>
+__global__ void k_scalar_q(RingDev rd, const uint64_t* A, uint64_t* C, int n, size_t count, int op, ScalarArg sa) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const size_t pr = t / n, j = t % n;
+  const uint64_t* a = A + pr * LQ * n + j;
+  uint64_t* c = C + pr * LQ * n + j;
+  if (op == SC_MUL) {
+    constexpr int LA = 2 * LQ + 2, LMU = LQ + 3;
+    uint64_t x[LA], quo[LMU], rem[LQ];
+#pragma unroll
+    for (int l = 0; l < LA; ++l) x[l] = 0;
+#pragma unroll
+    for (int i = 0; i < LQ; ++i) {
+      const uint64_t ai = a[(size_t)i * n];
+      uint64_t carry = 0;
+#pragma unroll
+      for (int k = 0; k < LQ; ++k) {
+        const uint64_t lo = ai * sa.v[k];
+        uint64_t hi = gl::umulhi(ai, sa.v[k]);
+        const uint64_t s1 = x[i + k] + lo;
+        hi += (s1 < lo);
+        const uint64_t s2 = s1 + carry;
+        hi += (s2 < s1);
+        x[i + k] = s2;
+        carry = hi;
+      }
+      x[i + LQ] = carry;
+    }
+    divmod_q<LQ>(x, rd, quo, rem, 2 * LQ);
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) c[(size_t)l * n] = rem[l];
+    return;
+  }
+  uint64_t x[LQ + 1];
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) x[l] = a[(size_t)l * n];
+  if (j == 0) {
+    uint64_t y[LQ + 1], qe[LQ + 1];
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) { y[l] = sa.v[l]; qe[l] = rd.q[l]; }
+    x[LQ] = y[LQ] = qe[LQ] = 0;
+    if (op == SC_ADD) {
+      add_n<LQ + 1>(x, y);
+      if (geq_n<LQ + 1>(x, qe)) sub_n<LQ + 1>(x, qe);
+    } else {
+      if (!geq_n<LQ + 1>(x, y)) add_n<LQ + 1>(x, qe);
+      sub_n<LQ + 1>(x, y);
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) c[(size_t)l * n] = x[l];
+}
+
+// Residues [R][spacing] (modulus i in row i, values in [0, m_i)): the same
+// operations mod each CRT modulus, with the scalar's residue s_i = sa.v[i].
+// c may alias a.
+__global__ void k_residue_scalar(RingDev rd, uint64_t* C, const uint64_t* A, int spacing, int op, ScalarArg sa) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= spacing) return;
+  const size_t o = (size_t)i * spacing + j;
+  const uint64_t a = A[o], s = sa.v[i];
+  uint64_t r = a;
+  if (i == 0) {  // modulus P itself
+    if (op == SC_MUL) r = gl::mul(a, s);
+    else if (j == 0) r = op == SC_ADD ? gl::add(a, s) : gl::sub(a, s);
+  } else {
+    const uint64_t p = rd.primes[0][i];
+    if (op == SC_MUL) r = mod_p(a * s, p, rd.barrett[0][i]);  // a, s < p < 2^28
+    else if (j == 0) r = op == SC_ADD ? (a + s >= p ? a + s - p : a + s) : (a >= s ? a - s : a + p - s);
+  }
+  C[o] = r;
+}
+
 // Reduction mod (x^n + 1, q) of a length-2n coefficient vector
 // (Polynomial::reduce, src/cuda_functions.cu:1061-1078, and the % q of
 // src/polynomial.h:593-610): out_j = (c_j - c_{j+n}) mod q in [0, q).

@@ -75,12 +75,21 @@ struct yh_ctx {
   int cus = 0;           // compute units of the device
   std::string arch;      // gcnArchName
   bool capturing = false;
+  // objects whose device memory the launches being captured reference; the
+  // graph takes a reference on each when the capture ends
+  std::vector<yh_ring*> cap_rings;
+  std::vector<const yh_evk*> cap_evks;
+  std::vector<yh_batch*> cap_batches;
+  int live = 0;  // rings, batch plans and graphs created on this context and not yet destroyed
 };
 
 struct yh_graph {
   yh_ctx* ctx = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<yh_ring*> rings;
+  std::vector<const yh_evk*> evks;
+  std::vector<yh_batch*> batches;
 };
 
 struct yh_ring {
@@ -98,6 +107,12 @@ struct yh_ring {
   std::map<int, yh_ring*> sub;  // rings x^m + 1 over the same q, m < n (inverse recursion)
   hipStream_t stream = nullptr;  // the context stream, or the ring's own (yh_ring_use_own_stream)
   bool own_stream = false;
+  // captured graphs holding pointers to this ring's tables and scratch
+  // (while any exists the scratch cannot be reallocated), and batch plans
+  // holding copies of its table pointers; while either exists the ring
+  // cannot be destroyed
+  int graph_users = 0;
+  int plan_users = 0;
 };
 
 struct yh_event {
@@ -109,6 +124,7 @@ struct yh_evk {
   yh_ring* ring = nullptr;
   uint64_t* d = nullptr;     // [lw][R_ks][n]: transform domain (n >= 16) or centred residues
   uint64_t* corr = nullptr;  // [R_ks][n]: correction of the centred digits (n >= 16)
+  mutable int users = 0;     // captured graphs that read this key
 };
 
 namespace {
@@ -150,18 +166,63 @@ int set_dev(yh_ctx* ctx) {
   return YH_OK;
 }
 
-// Grows a scratch buffer used by work on `stream` (waits for that work first).
-int ensure(yh_ctx* ctx, hipStream_t stream, void** buf, size_t* have, size_t need) {
+// Selects the ring's device and, while the context stream is being captured,
+// records that the graph will reference the ring's tables and scratch. Every
+// ring entry point that launches work calls it first.
+int ring_enter(yh_ring* r) {
+  if (int s = set_dev(r->ctx)) return s;
+  yh_ctx* c = r->ctx;
+  if (c->capturing && r->stream == c->stream &&
+      std::find(c->cap_rings.begin(), c->cap_rings.end(), r) == c->cap_rings.end())
+    c->cap_rings.push_back(r);
+  return YH_OK;
+}
+
+void note_evk(yh_ring* r, const yh_evk* k) {
+  yh_ctx* c = r->ctx;
+  if (c->capturing && r->stream == c->stream &&
+      std::find(c->cap_evks.begin(), c->cap_evks.end(), k) == c->cap_evks.end())
+    c->cap_evks.push_back(k);
+}
+
+// Grows one of the ring's scratch buffers (waits for the ring's work first).
+// Growth frees the old buffer, so it is refused while a captured graph or a
+// batch plan holds the ring (their launches keep the old addresses).
+int ensure(yh_ring* r, void** buf, size_t* have, size_t need) {
   if (*have >= need) return YH_OK;
-  if (ctx->capturing) return fail(YH_ERR_ARG, "scratch would grow during graph capture; run the sequence once first");
+  if (r->ctx->capturing) return fail(YH_ERR_ARG, "scratch would grow during graph capture; run the sequence once first");
+  if (r->graph_users > 0)
+    return fail(YH_ERR_ARG,
+                "ring scratch would be reallocated while a captured graph references it; run the largest batch "
+                "before capturing, or destroy the graph first");
   if (*buf) {
-    YH_HIP(hipStreamSynchronize(stream));
+    YH_HIP(hipStreamSynchronize(r->stream));
     YH_HIP(hipFree(*buf));
     *buf = nullptr;
     *have = 0;
   }
   YH_HIP(hipMalloc(buf, need));
   *have = need;
+  return YH_OK;
+}
+
+// Device scratch of one fused operation: reconstruction rows, the
+// mul+keyswitch intermediate and the keyswitch digit transforms. The
+// allocating entry points take them from the ring (grown on demand), the
+// allocation-free one from the library's static array.
+struct Scratch {
+  uint32_t* y = nullptr;
+  uint64_t* tmp = nullptr;
+  uint64_t* dig = nullptr;
+};
+
+size_t y_bytes_for(const yh_ring* r, int set, int batch) {
+  return (size_t)batch * yhk::y_rows(r->rd.set_R[set]) * r->rd.n * sizeof(uint32_t);
+}
+
+int ring_y(yh_ring* r, size_t need, Scratch* s) {
+  if (int st = ensure(r, (void**)&r->d_y, &r->y_bytes, need)) return st;
+  s->y = r->d_y;
   return YH_OK;
 }
 
@@ -310,10 +371,10 @@ int addsub_impl(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* c, i
   return YH_OK;
 }
 
-yhk::IcrtArgs icrt_args(yh_ring* r, int N, int ncoef, int set, int mode, uint64_t* out, int out_limbs,
+yhk::IcrtArgs icrt_args(const uint32_t* Y, int N, int ncoef, int set, int mode, uint64_t* out, int out_limbs,
                         size_t out_stride, size_t out_pstride, int positive) {
   yhk::IcrtArgs ia{};
-  ia.Y = r->d_y;
+  ia.Y = Y;
   ia.N = N;
   ia.ncoef = ncoef;
   ia.set = set;
@@ -326,19 +387,20 @@ yhk::IcrtArgs icrt_args(yh_ring* r, int N, int ncoef, int set, int mode, uint64_
   return ia;
 }
 
-int ring_product(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* out, int batch, int mode,
-                 const uint64_t* const* a_list = nullptr, const uint64_t* const* b_list = nullptr,
+// Ring product with one of the reconstruction epilogues; sc.y holds at least
+// y_bytes_for(r, 0, batch) bytes.
+int ring_product(yh_ring* r, const Scratch& sc, const uint64_t* a, const uint64_t* b, uint64_t* out, int batch,
+                 int mode, const uint64_t* const* a_list = nullptr, const uint64_t* const* b_list = nullptr,
                  uint64_t* const* out_list = nullptr) {
-  const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * yhk::y_rows(R) * n * sizeof(uint32_t));
-  if (st) return st;
+  const int n = r->rd.n, LQ = r->rd.LQ;
+  int st;
   yhk::ProdArgs pa{};
   pa.A = a; pa.B = b;
   pa.lin_a = pa.lin_b = LQ;
   pa.ncoef_a = pa.ncoef_b = n;
   pa.lstride_a = pa.lstride_b = n;
   pa.ostride_a = pa.ostride_b = (size_t)LQ * n;
-  pa.Y = r->d_y;
+  pa.Y = sc.y;
   pa.logN = r->rd.logn;
   pa.tab = r->rd.tab;
   pa.set = 0;
@@ -346,7 +408,7 @@ int ring_product(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* out
   pa.b_list = b_list;
   st = launch_prod(r, pa, n, batch, true);
   if (st) return st;
-  yhk::IcrtArgs ia = icrt_args(r, n, n, 0, mode, out, LQ, n, (size_t)LQ * n, 0);
+  yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 0, mode, out, LQ, n, (size_t)LQ * n, 0);
   ia.out_list = out_list;
   return launch_icrt(r, ia, batch);
 }
@@ -357,22 +419,25 @@ void launch_reduce_t(yh_ring* r, const uint64_t* c, int lin, uint64_t* out, int 
   hipLaunchKernelGGL(yhk::k_reduce<LQ>, grid, block, 0, r->stream, r->rd, c, lin, out);
 }
 
-int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out, int batch,
+size_t dig_bytes_for(const yh_ring* r, int batch) {
+  return r->rd.n >= 16 ? (size_t)batch * r->rd.lw * r->rd.n * sizeof(uint64_t) : 0;
+}
+
+// Keyswitch of `batch` ciphertexts; sc.y holds y_bytes_for(r, 1, batch) bytes
+// and sc.dig dig_bytes_for(r, batch).
+int keyswitch_impl(yh_ring* r, const Scratch& sc, const yh_evk* k, const uint64_t* c, uint64_t* out, int batch,
                    uint64_t* const* out_list = nullptr) {
   const int n = r->rd.n, LQ = r->rd.LQ, Rks = r->rd.set_R[1];
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * yhk::y_rows(Rks) * n * sizeof(uint32_t));
-  if (st) return st;
+  note_evk(r, k);
   yhk::KsArgs ka{};
   ka.C = c;
   ka.ostride = (size_t)LQ * n;
   ka.evk = k->d;
   ka.corr = k->corr;
-  ka.Y = r->d_y;
+  ka.Y = sc.y;
   ka.batch = batch;
   if (n >= 16) {
-    st = ensure(r->ctx, r->stream, (void**)&r->d_dig, &r->dig_bytes, (size_t)batch * r->rd.lw * n * sizeof(uint64_t));
-    if (st) return st;
-    ka.D = r->d_dig;
+    ka.D = sc.dig;
     const bool w32 = r->rd.w_bits == 32;
     {
       LaunchTimer timer(r->ctx, YH_KERNEL_KS_DIGITS, r->stream);
@@ -406,9 +471,35 @@ int keyswitch_impl(yh_ring* r, const yh_evk* k, const uint64_t* c, uint64_t* out
     hipLaunchKernelGGL(yhk::k_keyswitch_small, dim3(Rks, batch), dim3(64), 0, r->stream, r->rd, ka, k->d);
   }
   YH_HIP(hipGetLastError());
-  yhk::IcrtArgs ia = icrt_args(r, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0);
+  yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0);
   ia.out_list = out_list;
   return launch_icrt(r, ia, batch);
+}
+
+// Scratch of the allocating keyswitch: reconstruction rows and digit
+// transforms from the ring.
+int ks_scratch(yh_ring* r, int batch, Scratch* sc) {
+  if (int st = ring_y(r, y_bytes_for(r, 1, batch), sc)) return st;
+  if (r->rd.n >= 16) {
+    if (int st = ensure(r, (void**)&r->d_dig, &r->dig_bytes, dig_bytes_for(r, batch))) return st;
+    sc->dig = r->d_dig;
+  }
+  return YH_OK;
+}
+
+// y rows for both the product and the keyswitch of a mul+keyswitch, and the
+// intermediate ring element.
+int mulks_scratch(yh_ring* r, int batch, Scratch* sc) {
+  const size_t yb = std::max(y_bytes_for(r, 0, batch), y_bytes_for(r, 1, batch));
+  if (int st = ring_y(r, yb, sc)) return st;
+  if (int st = ensure(r, (void**)&r->d_tmp, &r->tmp_bytes, (size_t)batch * r->rd.LQ * r->rd.n * sizeof(uint64_t)))
+    return st;
+  sc->tmp = r->d_tmp;
+  if (r->rd.n >= 16) {
+    if (int st = ensure(r, (void**)&r->d_dig, &r->dig_bytes, dig_bytes_for(r, batch))) return st;
+    sc->dig = r->d_dig;
+  }
+  return YH_OK;
 }
 
 }  // namespace
@@ -454,6 +545,9 @@ int yh_ctx_create(int device, yh_ctx** out) {
 
 int yh_ctx_destroy(yh_ctx* ctx) {
   if (!ctx) return YH_OK;
+  if (ctx->live > 0)
+    return fail(YH_ERR_ARG, "context still has " + std::to_string(ctx->live) +
+                                " ring(s), batch plan(s) or graph(s); destroy them first");
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   drain_timing(ctx);
@@ -625,6 +719,7 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
   rd.qmu = P.qmu;
   rd.sp[0] = reinterpret_cast<const yhk::SmallPrime*>(d + o_sp0);
   rd.sp[1] = reinterpret_cast<const yhk::SmallPrime*>(d + o_sp1);
+  ctx->live++;
   *out = r.release();
   return YH_OK;
   YH_TRY_END
@@ -632,9 +727,13 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
 
 int yh_ring_destroy(yh_ring* r) {
   if (!r) return YH_OK;
+  if (r->graph_users + r->plan_users > 0)
+    return fail(YH_ERR_ARG, "ring is referenced by " + std::to_string(r->graph_users) + " captured graph(s) and " +
+                                std::to_string(r->plan_users) + " batch plan(s); destroy them first");
   (void)hipSetDevice(r->ctx->device);
   (void)hipStreamSynchronize(r->stream);
   for (auto& kv : r->sub) yh_ring_destroy(kv.second);
+  r->ctx->live--;
   if (r->own_stream) (void)hipStreamDestroy(r->stream);
   (void)hipFree(r->d_tables);
   if (r->d_ntt) (void)hipFree(r->d_ntt);
@@ -706,7 +805,7 @@ int yh_crt(yh_ring* r, const uint64_t* d_coefs, int limbs, int ncoef, int spacin
   if (!r || !d_res || (ncoef > 0 && !d_coefs)) return fail(YH_ERR_ARG, "null argument");
   if (limbs < 1 || limbs > r->rd.LW) return fail(YH_ERR_ARG, "limbs out of range");
   if (ncoef < 0 || spacing < ncoef || spacing <= 0) return fail(YH_ERR_ARG, "bad spacing");
-  if (int s = set_dev(r->ctx)) return s;
+  if (int s = ring_enter(r)) return s;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
   hipLaunchKernelGGL(yhk::k_crt, grid, block, 0, r->stream, r->rd, d_coefs, limbs, ncoef, (size_t)ncoef,
                      d_res, spacing);
@@ -717,13 +816,13 @@ int yh_crt(yh_ring* r, const uint64_t* d_coefs, int limbs, int ncoef, int spacin
 int yh_icrt(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, uint64_t* d_coefs, int out_limbs) {
   if (!r || !d_res || !d_coefs) return fail(YH_ERR_ARG, "null argument");
   if (ncoef <= 0 || ncoef > spacing || out_limbs < 1) return fail(YH_ERR_ARG, "bad sizes");
-  if (int s = set_dev(r->ctx)) return s;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)yhk::y_rows(r->rd.R) * spacing * sizeof(uint32_t));
-  if (st) return st;
+  if (int s = ring_enter(r)) return s;
+  Scratch sc;
+  if (int st = ring_y(r, (size_t)yhk::y_rows(r->rd.R) * spacing * sizeof(uint32_t), &sc)) return st;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  hipLaunchKernelGGL(yhk::k_residue_to_y, grid, block, 0, r->stream, r->rd, d_res, spacing, r->d_y, 0);
+  hipLaunchKernelGGL(yhk::k_residue_to_y, grid, block, 0, r->stream, r->rd, d_res, spacing, sc.y, 0);
   YH_HIP(hipGetLastError());
-  return launch_icrt(r, icrt_args(r, spacing, ncoef, 0, yhk::EPI_RAW, d_coefs, out_limbs, ncoef, 0, 1), 1);
+  return launch_icrt(r, icrt_args(sc.y, spacing, ncoef, 0, yhk::EPI_RAW, d_coefs, out_limbs, ncoef, 0, 1), 1);
 }
 
 int yh_ntt(yh_ctx* ctx, uint64_t* d_data, int N, int count, int direction) {
@@ -757,7 +856,7 @@ int yh_pointwise_mulmod(yh_ctx* ctx, uint64_t* d_a, const uint64_t* d_b, size_t 
 
 int yh_residue_addsub(yh_ring* r, uint64_t* d_c, const uint64_t* d_a, const uint64_t* d_b, int spacing, int op) {
   if (!r || !d_c || !d_a || !d_b || spacing <= 0) return fail(YH_ERR_ARG, "bad argument");
-  if (int s = set_dev(r->ctx)) return s;
+  if (int s = ring_enter(r)) return s;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
   hipLaunchKernelGGL(yhk::k_residue_addsub, grid, block, 0, r->stream, r->rd, d_c, d_a, d_b, spacing,
                      op == YH_SUB ? 1 : 0);
@@ -774,9 +873,10 @@ int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t
   if (min_N < 1 || (min_N & (min_N - 1))) return fail(YH_ERR_ARG, "min_N must be a power of two");
   const long N = yh_poly_mul_plain_length(deg_a, deg_b, min_N);
   if (N > 16384) return fail(YH_ERR_UNSUPPORTED, "plain product longer than 16384");
-  if (int s = set_dev(r->ctx)) return s;
+  if (int s = ring_enter(r)) return s;
   const int R = r->rd.R, LQ = r->rd.LQ;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)yhk::y_rows(R) * N * sizeof(uint32_t));
+  Scratch sc;
+  int st = ring_y(r, (size_t)yhk::y_rows(R) * N * sizeof(uint32_t), &sc);
   if (st) return st;
   DevTables tw{};
   if (N >= 16) {
@@ -788,14 +888,14 @@ int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t
   pa.ncoef_a = deg_a + 1; pa.ncoef_b = deg_b + 1;
   pa.lstride_a = deg_a + 1; pa.lstride_b = deg_b + 1;
   pa.ostride_a = pa.ostride_b = 0;
-  pa.Y = r->d_y;
+  pa.Y = sc.y;
   pa.logN = ilog2i(N);
   pa.tab = tw.tab;
   pa.set = 0;
   st = launch_prod(r, pa, (int)N, 1, false);
   if (st) return st;
   if (N_out) *N_out = (int)N;
-  return launch_icrt(r, icrt_args(r, (int)N, (int)N, 0, yhk::EPI_RAW, d_out, out_limbs, N, 0, 1), 1);
+  return launch_icrt(r, icrt_args(sc.y, (int)N, (int)N, 0, yhk::EPI_RAW, d_out, out_limbs, N, 0, 1), 1);
 }
 
 int yh_poly_mul_plain_length(int deg_a, int deg_b, int min_N) {
@@ -812,20 +912,22 @@ int yh_poly_mul_plain_length(int deg_a, int deg_b, int min_N) {
 
 int yh_ring_mul(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch) {
   if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
-  if (int s = set_dev(r->ctx)) return s;
-  return ring_product(r, d_a, d_b, d_out, batch, yhk::EPI_MODQ);
+  if (int s = ring_enter(r)) return s;
+  Scratch sc;
+  if (int st = ring_y(r, y_bytes_for(r, 0, batch), &sc)) return st;
+  return ring_product(r, sc, d_a, d_b, d_out, batch, yhk::EPI_MODQ);
 }
 
 int yh_ring_addsub(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch, int op) {
   if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
-  if (int s = set_dev(r->ctx)) return s;
+  if (int s = ring_enter(r)) return s;
   return addsub_impl(r, d_a, d_b, d_out, batch, op == YH_SUB ? 1 : 0, yhk::AddsubLists{nullptr, nullptr, nullptr});
 }
 
 int yh_ring_reduce(yh_ring* r, const uint64_t* d_c, int in_limbs, uint64_t* d_out, int batch) {
   if (!r || !d_c || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (in_limbs < 1 || in_limbs > r->rd.LW) return fail(YH_ERR_ARG, "in_limbs out of range");
-  if (int s = set_dev(r->ctx)) return s;
+  if (int s = ring_enter(r)) return s;
   switch (r->rd.LQ) {
     case 1: launch_reduce_t<1>(r, d_c, in_limbs, d_out, batch); break;
     case 2: launch_reduce_t<2>(r, d_c, in_limbs, d_out, batch); break;
@@ -841,16 +943,65 @@ int yh_ring_reduce(yh_ring* r, const uint64_t* d_c, int in_limbs, uint64_t* d_ou
   return YH_OK;
 }
 
+// The scalar (host limbs, any width) reduced mod m, as limbs into v[0..L).
+static int scalar_mod(const uint64_t* s, int s_limbs, const yh::BigInt& m, uint64_t* v, int L) {
+  if (!s || s_limbs < 1 || s_limbs > 1024) return fail(YH_ERR_ARG, "scalar must have 1 to 1024 limbs");
+  yh::BigInt x = yh::BigInt::from_limbs(s, s_limbs) % m;
+  x.to_limbs(v, L);
+  return YH_OK;
+}
+
+int yh_ring_scalar(yh_ring* r, const uint64_t* d_a, const uint64_t* scalar, int scalar_limbs, uint64_t* d_out,
+                   int batch, int op) {
+  if (!r || !d_a || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (op != YH_ADD && op != YH_SUB && op != YH_MUL) return fail(YH_ERR_ARG, "op must be YH_ADD, YH_SUB or YH_MUL");
+  YH_TRY_BEGIN
+  yhk::ScalarArg sa{};
+  if (int st = scalar_mod(scalar, scalar_limbs, r->P->q, sa.v, r->rd.LQ)) return st;
+  if (int s = ring_enter(r)) return s;
+  const size_t count = (size_t)batch * r->rd.n;
+  const dim3 block(256), grid((unsigned)((count + 255) / 256));
+  switch (r->rd.LQ) {
+#define X(L) case L: hipLaunchKernelGGL(yhk::k_scalar_q<L>, grid, block, 0, r->stream, r->rd, d_a, d_out, r->rd.n, count, op, sa); break;
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+#undef X
+    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
+  }
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+  YH_TRY_END
+}
+
+int yh_residue_scalar(yh_ring* r, uint64_t* d_c, const uint64_t* d_a, int spacing, int op, const uint64_t* scalar,
+                      int scalar_limbs) {
+  if (!r || !d_c || !d_a || spacing <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (op != YH_ADD && op != YH_SUB && op != YH_MUL) return fail(YH_ERR_ARG, "op must be YH_ADD, YH_SUB or YH_MUL");
+  if (r->rd.R > 64) return fail(YH_ERR_UNSUPPORTED, "more than 64 CRT moduli");
+  YH_TRY_BEGIN
+  yhk::ScalarArg sa{};
+  for (int i = 0; i < r->rd.R; ++i)
+    if (int st = scalar_mod(scalar, scalar_limbs, yh::BigInt(r->P->primes[i]), &sa.v[i], 1)) return st;
+  if (int s = ring_enter(r)) return s;
+  const dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
+  hipLaunchKernelGGL(yhk::k_residue_scalar, grid, block, 0, r->stream, r->rd, d_c, d_a, spacing, op, sa);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+  YH_TRY_END
+}
+
 int yh_yashe_mul(yh_ring* r, const uint64_t* d_c1, const uint64_t* d_c2, uint64_t* d_out, int batch) {
   if (!r || !d_c1 || !d_c2 || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
-  if (int s = set_dev(r->ctx)) return s;
-  return ring_product(r, d_c1, d_c2, d_out, batch, yhk::EPI_SCALE);
+  if (int s = ring_enter(r)) return s;
+  Scratch sc;
+  if (int st = ring_y(r, y_bytes_for(r, 0, batch), &sc)) return st;
+  return ring_product(r, sc, d_c1, d_c2, d_out, batch, yhk::EPI_SCALE);
 }
 
 int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
   if (!r || !d_gamma || !out) return fail(YH_ERR_ARG, "null argument");
   if (r->rd.w_bits == 0) return fail(YH_ERR_ARG, "ring has no keyswitch base");
+  if (r->ctx->capturing) return fail(YH_ERR_ARG, "yh_evk_create allocates; create keys before capturing");
   if (int s = set_dev(r->ctx)) return s;
   auto* k = new yh_evk();
   k->ring = r;
@@ -889,6 +1040,9 @@ int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
 
 int yh_evk_destroy(yh_evk* k) {
   if (!k) return YH_OK;
+  if (k->users > 0)
+    return fail(YH_ERR_ARG, "keyswitch key is referenced by " + std::to_string(k->users) +
+                                " captured graph(s); destroy them first");
   (void)hipSetDevice(k->ring->ctx->device);
   (void)hipStreamSynchronize(k->ring->stream);
   (void)hipFree(k->d);
@@ -901,8 +1055,10 @@ int yh_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c, uint64_t* d_o
   if (!r || !k || !d_c || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (k->ring != r) return fail(YH_ERR_ARG, "key belongs to another ring");
   if (d_c == d_out) return fail(YH_ERR_ARG, "keyswitch cannot run in place");
-  if (int s = set_dev(r->ctx)) return s;
-  return keyswitch_impl(r, k, d_c, d_out, batch);
+  if (int s = ring_enter(r)) return s;
+  Scratch sc;
+  if (int st = ks_scratch(r, batch, &sc)) return st;
+  return keyswitch_impl(r, sc, k, d_c, d_out, batch);
 }
 
 int yh_yashe_mul_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, const uint64_t* d_c2,
@@ -910,19 +1066,18 @@ int yh_yashe_mul_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, co
   if (!r || !k || !d_c1 || !d_c2 || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (k->ring != r) return fail(YH_ERR_ARG, "key belongs to another ring");
   if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
-  if (int s = set_dev(r->ctx)) return s;
-  size_t need = (size_t)batch * r->rd.LQ * r->rd.n * sizeof(uint64_t);
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_tmp, &r->tmp_bytes, need);
-  if (st) return st;
-  st = ring_product(r, d_c1, d_c2, r->d_tmp, batch, yhk::EPI_SCALE);
-  if (st) return st;
-  return keyswitch_impl(r, k, r->d_tmp, d_out, batch);
+  if (int s = ring_enter(r)) return s;
+  Scratch sc;
+  if (int st = mulks_scratch(r, batch, &sc)) return st;
+  if (int st = ring_product(r, sc, d_c1, d_c2, sc.tmp, batch, yhk::EPI_SCALE)) return st;
+  return keyswitch_impl(r, sc, k, sc.tmp, d_out, batch);
 }
 
 static int decrypt_impl(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch,
                         const uint64_t* const* c_list, uint64_t* const* m_list) {
-  const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * yhk::y_rows(R) * n * sizeof(uint32_t));
+  const int n = r->rd.n, LQ = r->rd.LQ;
+  Scratch sc;
+  int st = ring_y(r, y_bytes_for(r, 0, batch), &sc);
   if (st) return st;
   yhk::ProdArgs pa{};
   pa.A = d_f; pa.B = d_c;
@@ -931,14 +1086,14 @@ static int decrypt_impl(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, ui
   pa.lstride_a = pa.lstride_b = n;
   pa.ostride_a = 0;
   pa.ostride_b = (size_t)LQ * n;
-  pa.Y = r->d_y;
+  pa.Y = sc.y;
   pa.logN = r->rd.logn;
   pa.tab = r->rd.tab;
   pa.set = 0;
   pa.b_list = c_list;
   st = launch_prod(r, pa, n, batch, true);
   if (st) return st;
-  yhk::IcrtArgs ia = icrt_args(r, n, n, 0, yhk::EPI_DECRYPT, d_m, 1, n, n, 0);
+  yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 0, yhk::EPI_DECRYPT, d_m, 1, n, n, 0);
   ia.out_list = m_list;
   return launch_icrt(r, ia, batch);
 }
@@ -946,7 +1101,7 @@ static int decrypt_impl(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, ui
 int yh_yashe_decrypt(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch) {
   if (!r || !d_f || !d_c || !d_m || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
-  if (int s = set_dev(r->ctx)) return s;
+  if (int s = ring_enter(r)) return s;
   return decrypt_impl(r, d_f, d_c, d_m, batch, nullptr, nullptr);
 }
 
@@ -954,7 +1109,7 @@ int yh_yashe_decrypt(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint6
 int yh_ring_addsub_list(yh_ring* r, const uint64_t* const* d_a, const uint64_t* const* d_b, uint64_t* const* d_out,
                         int batch, int op) {
   if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
-  if (int s = set_dev(r->ctx)) return s;
+  if (int s = ring_enter(r)) return s;
   return addsub_impl(r, nullptr, nullptr, nullptr, batch, op == YH_SUB ? 1 : 0, yhk::AddsubLists{d_a, d_b, d_out});
 }
 
@@ -963,20 +1118,18 @@ int yh_yashe_mul_keyswitch_list(yh_ring* r, const yh_evk* k, const uint64_t* con
   if (!r || !k || !d_c1 || !d_c2 || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (k->ring != r) return fail(YH_ERR_ARG, "key belongs to another ring");
   if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
-  if (int s = set_dev(r->ctx)) return s;
-  size_t need = (size_t)batch * r->rd.LQ * r->rd.n * sizeof(uint64_t);
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_tmp, &r->tmp_bytes, need);
-  if (st) return st;
-  st = ring_product(r, nullptr, nullptr, r->d_tmp, batch, yhk::EPI_SCALE, d_c1, d_c2, nullptr);
-  if (st) return st;
-  return keyswitch_impl(r, k, r->d_tmp, nullptr, batch, d_out);
+  if (int s = ring_enter(r)) return s;
+  Scratch sc;
+  if (int st = mulks_scratch(r, batch, &sc)) return st;
+  if (int st = ring_product(r, sc, nullptr, nullptr, sc.tmp, batch, yhk::EPI_SCALE, d_c1, d_c2, nullptr)) return st;
+  return keyswitch_impl(r, sc, k, sc.tmp, nullptr, batch, d_out);
 }
 
 int yh_yashe_decrypt_list(yh_ring* r, const uint64_t* d_f, const uint64_t* const* d_c, uint64_t* const* d_m,
                           int batch) {
   if (!r || !d_f || !d_c || !d_m || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
-  if (int s = set_dev(r->ctx)) return s;
+  if (int s = ring_enter(r)) return s;
   return decrypt_impl(r, d_f, nullptr, nullptr, batch, d_c, d_m);
 }
 
@@ -1050,9 +1203,10 @@ int yh_event_destroy(yh_event* e) {
 
 int yh_ring_mul_bcast(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch) {
   if (!r || !d_a || !d_b || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
-  if (int s = set_dev(r->ctx)) return s;
-  const int n = r->rd.n, LQ = r->rd.LQ, R = r->rd.R;
-  int st = ensure(r->ctx, r->stream, (void**)&r->d_y, &r->y_bytes, (size_t)batch * yhk::y_rows(R) * n * sizeof(uint32_t));
+  if (int s = ring_enter(r)) return s;
+  const int n = r->rd.n, LQ = r->rd.LQ;
+  Scratch sc;
+  int st = ring_y(r, y_bytes_for(r, 0, batch), &sc);
   if (st) return st;
   yhk::ProdArgs pa{};
   pa.A = d_a; pa.B = d_b;
@@ -1061,13 +1215,13 @@ int yh_ring_mul_bcast(yh_ring* r, const uint64_t* d_a, const uint64_t* d_b, uint
   pa.lstride_a = pa.lstride_b = n;
   pa.ostride_a = 0;
   pa.ostride_b = (size_t)LQ * n;
-  pa.Y = r->d_y;
+  pa.Y = sc.y;
   pa.logN = r->rd.logn;
   pa.tab = r->rd.tab;
   pa.set = 0;
   st = launch_prod(r, pa, n, batch, true);
   if (st) return st;
-  return launch_icrt(r, icrt_args(r, n, n, 0, yhk::EPI_MODQ, d_out, LQ, n, (size_t)LQ * n, 0), batch);
+  return launch_icrt(r, icrt_args(sc.y, n, n, 0, yhk::EPI_MODQ, d_out, LQ, n, (size_t)LQ * n, 0), batch);
 }
 
 static int sub_ring(yh_ring* r, int m, yh_ring** out) {
@@ -1165,6 +1319,22 @@ int yh_ring_inverse(yh_ring* r, const uint64_t* d_f, uint64_t* d_out) {
   YH_TRY_END
 }
 
+struct yh_batch {
+  yh_ctx* ctx = nullptr;
+  yh_batch_config cfg{};
+  int ninst = 0, nbundles = 0, grid = 0, W = 0, maxlq = 0;
+  void* d_mem = nullptr;
+  yhk::BatchArgs args{};
+  std::vector<yh_ring*> rings;  // distinct rings of the items (each holds a reference while the plan lives)
+  int users = 0;                // captured graphs that launch this plan
+  ~yh_batch() {
+    if (d_mem) (void)hipFree(d_mem);
+    for (yh_ring* r : rings) r->plan_users--;
+  }
+};
+
+static void batch_users(yh_batch* b, int d) { b->users += d; }
+
 int yh_graph_begin(yh_ctx* ctx) {
   if (!ctx) return fail(YH_ERR_ARG, "null ctx");
   if (ctx->capturing) return fail(YH_ERR_ARG, "already capturing");
@@ -1172,6 +1342,9 @@ int yh_graph_begin(yh_ctx* ctx) {
   if (int s = drain_timing(ctx)) return s;
   YH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   ctx->capturing = true;
+  ctx->cap_rings.clear();
+  ctx->cap_evks.clear();
+  ctx->cap_batches.clear();
   return YH_OK;
 }
 
@@ -1181,8 +1354,16 @@ int yh_graph_end(yh_ctx* ctx, yh_graph** out) {
   ctx->capturing = false;
   auto g = std::make_unique<yh_graph>();
   g->ctx = ctx;
+  g->rings.swap(ctx->cap_rings);
+  g->evks.swap(ctx->cap_evks);
+  g->batches.swap(ctx->cap_batches);
   YH_HIP(hipStreamEndCapture(ctx->stream, &g->graph));
   YH_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  // the graph's launches hold these objects' device addresses from now on
+  for (yh_ring* r : g->rings) r->graph_users++;
+  for (const yh_evk* k : g->evks) k->users++;
+  for (yh_batch* b : g->batches) batch_users(b, +1);
+  ctx->live++;
   *out = g.release();
   return YH_OK;
 }
@@ -1200,21 +1381,15 @@ int yh_graph_destroy(yh_graph* g) {
   (void)hipStreamSynchronize(g->ctx->stream);
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (yh_ring* r : g->rings) r->graph_users--;
+  for (const yh_evk* k : g->evks) k->users--;
+  for (yh_batch* b : g->batches) batch_users(b, -1);
+  g->ctx->live--;
   delete g;
   return YH_OK;
 }
 
 
-struct yh_batch {
-  yh_ctx* ctx = nullptr;
-  yh_batch_config cfg{};
-  int ninst = 0, nbundles = 0, grid = 0, W = 0, maxlq = 0;
-  void* d_mem = nullptr;
-  yhk::BatchArgs args{};
-  ~yh_batch() {
-    if (d_mem) (void)hipFree(d_mem);
-  }
-};
 
 extern "C++" {
 namespace {
@@ -1441,6 +1616,10 @@ int yh_batch_create(yh_ctx* ctx, const yh_batch_item* items, int count, const yh
   a.done = (int*)(base + o_done);
   a.nqueues = nq;
   b->grid = std::min(ctx->cus * cfg.wgs_per_cu, std::max(b->nbundles, 1));
+  // the plan holds copies of each ring's table pointers
+  b->rings = rings;
+  for (yh_ring* r : rings) r->plan_users++;
+  ctx->live++;
   *out = b.release();
   return YH_OK;
   YH_TRY_END
@@ -1450,6 +1629,9 @@ int yh_batch_run(yh_batch* b) {
   if (!b) return fail(YH_ERR_ARG, "null batch");
   if (int s = set_dev(b->ctx)) return s;
   hipStream_t st = b->ctx->stream;
+  if (b->ctx->capturing &&
+      std::find(b->ctx->cap_batches.begin(), b->ctx->cap_batches.end(), b) == b->ctx->cap_batches.end())
+    b->ctx->cap_batches.push_back(b);
   hipLaunchKernelGGL(yhk::k_batch_reset, dim3(std::max(1, std::min(64, (b->ninst + 255) / 256))), dim3(256), 0, st,
                      b->args.head, b->args.nqueues, b->args.done, b->ninst);
   YH_HIP(hipGetLastError());
@@ -1468,8 +1650,12 @@ int yh_batch_run(yh_batch* b) {
 
 int yh_batch_destroy(yh_batch* b) {
   if (!b) return YH_OK;
+  if (b->users > 0)
+    return fail(YH_ERR_ARG, "batch plan is launched by " + std::to_string(b->users) +
+                                " captured graph(s); destroy them first");
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
+  b->ctx->live--;
   delete b;
   return YH_OK;
 }

@@ -4,12 +4,15 @@
 // src/cuda_distribution.cu:11-35; its "Gaussian" is uniform mod 7 and a
 // layout bug fills only part of the buffer). These are host-side samplers
 // with the distributions the configurations state: ternary uniform
-// {-1, 0, 1} (NARROW), a rounded Gaussian of standard deviation sigma cut at
-// +-bound (DISCRETE_GAUSSIAN) and uniform mod q (UNIFORMLY), all driven by a
-// seeded xoshiro256** stream so runs are reproducible.
+// {-1, 0, 1} (NARROW), the discrete Gaussian D_{Z, sigma} restricted to
+// [-bound, bound] (DISCRETE_GAUSSIAN: P(x) proportional to
+// exp(-x^2 / (2 sigma^2)), drawn by inverse transform of its cumulative
+// table) and uniform mod q (UNIFORMLY), all driven by a seeded xoshiro256**
+// stream so runs are reproducible.
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -65,17 +68,7 @@ class Distribution {
       switch (kind_) {
         case NARROW: v = (int64_t)rng_.below(3) - 1; break;
         case BINARY: v = (int64_t)rng_.below(2); break;
-        case DISCRETE_GAUSSIAN: {
-          int64_t x;
-          do {
-            double u1 = rng_.uniform01(), u2 = rng_.uniform01();
-            if (u1 < 1e-300) u1 = 1e-300;
-            double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-            x = (int64_t)std::llround(z * sigma_);
-          } while (x > bound_ || x < -bound_);
-          v = x;
-          break;
-        }
+        case DISCRETE_GAUSSIAN: v = gaussian(); break;
         default: v = (int64_t)rng_.below(1ull << 62); break;
       }
     }
@@ -99,8 +92,24 @@ class Distribution {
   }
 
  private:
+  int64_t gaussian() {
+    if (cdf_.empty()) {
+      double acc = 0;
+      for (int x = -bound_; x <= bound_; ++x) {
+        acc += std::exp(-(double)x * x / (2.0 * sigma_ * sigma_));
+        cdf_.push_back(acc);
+      }
+      for (auto& c : cdf_) c /= acc;
+    }
+    const double u = rng_.uniform01();
+    size_t i = (size_t)(std::upper_bound(cdf_.begin(), cdf_.end(), u) - cdf_.begin());
+    if (i >= cdf_.size()) i = cdf_.size() - 1;
+    return (int64_t)i - bound_;
+  }
+
   kind_t kind_;
   double sigma_;
   int bound_;
   Xoshiro256 rng_;
+  std::vector<double> cdf_;
 };

@@ -197,7 +197,43 @@ Polynomial Polynomial::operator+(const ZZ& b) const {
   c += b;
   return c;
 }
+// Scalar operators. On a ring element (a modulus q > 1 from the polynomial
+// or global_mod, every coefficient in [0, q), degree below the ring degree)
+// they run on the device through yh_ring_scalar and give the result in
+// Z_q[x]: the reference runs them as polynomialOPInteger on CRT residues
+// (src/polynomial.h:459-573, src/cuda_functions.cu:477-548) and its tests
+// compare them with NTL's ZZ_p arithmetic after reduce()
+// (src/test.cu:630-790). Other polynomials (no modulus, negative or wider
+// coefficients) keep the exact integer operation on the host, the
+// reference's host-resident branch of the same operators.
+bool Polynomial::device_scalar(const ZZ& b, int op) {
+  const ZZ q = mod_.is_zero() ? global_mod : mod_;
+  if (q <= ZZ(1)) return false;
+  const int d = deg();
+  const int n = ring_degree() ? ring_degree() : next_pow2(std::max(d + 1, 2));
+  if (n < 2 || n > 16384 || (n & (n - 1)) || d >= n) return false;
+  for (int i = 0; i <= d; ++i)
+    if (coefs_[i].is_negative() || coefs_[i] >= q) return false;
+  yh_ring* r = runtime().ring(n, q);
+  yh_ring_info info;
+  cuyashe::check(yh_ring_get_info(r, &info), "yh_ring_get_info");
+  const int L = info.q_limbs;
+  std::vector<ZZ> c(coefs_.begin(), coefs_.begin() + std::max(d + 1, 0));
+  c.resize(n);
+  auto host = cuyashe::pack(c, L, n);
+  cuyashe::DeviceBuffer in(host.size() * 8);
+  in.upload(host.data(), host.size() * 8);
+  const ZZ s = b % q;  // non-negative (remainder takes the divisor's sign)
+  std::vector<uint64_t> sl(std::max(1, s.num_limbs()), 0);
+  s.to_limbs(sl.data(), (int)sl.size());
+  auto out = std::make_shared<cuyashe::DeviceBuffer>(host.size() * 8);
+  cuyashe::check(yh_ring_scalar(r, in.get(), sl.data(), (int)sl.size(), out->get(), 1, op), "yh_ring_scalar");
+  adopt_device(out, L, n);
+  return true;
+}
+
 Polynomial& Polynomial::operator+=(const ZZ& b) {
+  if (device_scalar(b, YH_ADD)) return *this;
   set_coeff(0, get_coeff(0) + b);
   return *this;
 }
@@ -207,6 +243,7 @@ Polynomial Polynomial::operator-(const ZZ& b) const {
   return c;
 }
 Polynomial& Polynomial::operator-=(const ZZ& b) {
+  if (device_scalar(b, YH_SUB)) return *this;
   set_coeff(0, get_coeff(0) - b);
   return *this;
 }
@@ -216,6 +253,7 @@ Polynomial Polynomial::operator*(const ZZ& b) const {
   return c;
 }
 Polynomial& Polynomial::operator*=(const ZZ& b) {
+  if (device_scalar(b, YH_MUL)) return *this;
   for (auto& v : coefs_) v *= b;
   touched();
   return *this;

@@ -209,6 +209,9 @@ class Polynomial {
   std::shared_ptr<cuyashe::DeviceBuffer> res_;
   int res_spacing_ = 0;
   void touched() { device_valid_ = false; res_.reset(); }
+  // scalar operator on the device when this is a ring element (see the
+  // operators); false when it is not and the host path applies
+  bool device_scalar(const ZZ& b, int op);
   friend class Ciphertext;
   friend class Yashe;
 };

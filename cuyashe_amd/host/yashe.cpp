@@ -136,12 +136,21 @@ Ciphertext Yashe::encrypt(const Polynomial& m) {
   auto d_h = upload_ring(h, s);
   auto d_s = upload_ring(sp, s);
   auto hs = ring_mul(d_h, d_s, s);
-  std::vector<ZZ> add(n);
-  for (int j = 0; j < n; ++j) add[j] = (ev[j] + delta.get_value() * m.get_coeff(j)) % q;
-  Polynomial ap = host_poly(add);
-  auto d_add = upload_ring(ap, s);
+  Polynomial ep = host_poly(ev);
+  auto d_e = upload_ring(ep, s);
+  // delta * m on the device (yh_ring_scalar; the reference's Integer
+  // product, src/integer.cpp:167-222)
+  std::vector<ZZ> mc(n);
+  for (int j = 0; j < n; ++j) mc[j] = m.get_coeff(j) % q;
+  Polynomial mp = host_poly(mc);
+  auto d_m = upload_ring(mp, s);
+  const ZZ dv = delta.get_value() % q;
+  std::vector<uint64_t> dl(std::max(1, dv.num_limbs()), 0);
+  dv.to_limbs(dl.data(), (int)dl.size());
+  cuyashe::check(yh_ring_scalar(s.r, d_m->get(), dl.data(), (int)dl.size(), d_m->get(), 1, YH_MUL), "yh_ring_scalar");
   auto out = std::make_shared<DeviceBuffer>((size_t)s.L * n * 8);
-  cuyashe::check(yh_ring_addsub(s.r, hs->get(), d_add->get(), out->get(), 1, YH_ADD), "yh_ring_addsub");
+  cuyashe::check(yh_ring_addsub(s.r, hs->get(), d_e->get(), out->get(), 1, YH_ADD), "yh_ring_addsub");
+  cuyashe::check(yh_ring_addsub(s.r, out->get(), d_m->get(), out->get(), 1, YH_ADD), "yh_ring_addsub");
   Ciphertext c(q);
   c.adopt_device(out, s.L, n);
   return c;

@@ -20,7 +20,15 @@ class Integer {
   ZZ get_value() const { return value_; }
   void set_value(const ZZ& a) { value_ = a; }
   ZZ operator*(const ZZ& a) const { return value_ * a; }
-  Polynomial operator*(const Polynomial& p) const { return p * value_; }
+  // Integer::operator*(Polynomial&) (src/integer.cpp:167-222): the scalar
+  // product followed by % the polynomial's modulus; on a ring element the
+  // product runs on the device (yh_ring_scalar) and is already reduced.
+  Polynomial operator*(const Polynomial& p) const {
+    Polynomial r = p * value_;
+    const ZZ m = p.get_mod();
+    if (!r.get_crt_computed() && m > ZZ(1)) r %= m;
+    return r;
+  }
   Polynomial operator+(const Polynomial& p) const { return p + value_; }
 
  private:

@@ -9,11 +9,17 @@ Gaussian polynomials, messages) are drawn on the host and uploaded once.
              h = t g f^-1,                          src/yashe.cpp:37-69
              gamma_k = f w^k + e_k + h s_k, k < lw   src/yashe.cpp:70-93
     encrypt: c = h s + e + floor(q/t) m             src/yashe.cpp:104-142
+             (floor(q/t) m by the device scalar product, yh_ring_scalar)
     decrypt: m = [round(t/q [f c]_q)]_t (f f after a mul without
              keyswitch)                             src/yashe.cpp:167-197
 
 The samplers are not bit-compatible with the reference's (it draws from
 rand()/NTL's RNG); what matters for parity is the arithmetic on the samples.
+Errors follow the discrete Gaussian D_{Z, sigma} with sigma = 8 cut at
++-48 (6 sigma): P(x) proportional to exp(-x^2 / (2 sigma^2)) on the integers
+of [-48, 48], drawn by inverse transform of that table (the reference's
+settings, src/yashe.h:29-33; its own sampler draws uniform values mod 7,
+src/distribution.cpp:4-63).
 """
 from __future__ import annotations
 
@@ -22,10 +28,30 @@ from typing import Optional
 import numpy as np
 
 from ._lib import (Context, DeviceBuffer, Ring, YashError, ints_to_array, WIRE_CIPHERTEXT, WIRE_KEYSWITCH_KEY,
-                   WIRE_PUBLIC_KEY, WIRE_SECRET_KEY, wire_blob_size, wire_peek)
+                   WIRE_PUBLIC_KEY, WIRE_SECRET_KEY, YH_MUL, wire_blob_size, wire_peek)
 
-GAUSS_SIGMA = 8.0   # src/distribution.h: discrete Gaussian with sigma 8
+GAUSS_SIGMA = 8.0   # src/yashe.h:29-33: discrete Gaussian with sigma 8
 GAUSS_BOUND = 48    # truncated at 6 sigma
+KEYGEN_ATTEMPTS = 17  # draws of f before key generation gives up (as the C++ Yashe)
+
+
+def discrete_gaussian_cdf(sigma: float = GAUSS_SIGMA, bound: int = GAUSS_BOUND) -> np.ndarray:
+    """Cumulative distribution of D_{Z, sigma} restricted to [-bound, bound]
+    (entry i for the value i - bound); the last entry is 1."""
+    x = np.arange(-bound, bound + 1, dtype=np.float64)
+    w = np.exp(-(x * x) / (2.0 * sigma * sigma))
+    c = np.cumsum(w)
+    return c / c[-1]
+
+
+_GAUSS_CDF = discrete_gaussian_cdf()
+
+
+def sample_discrete_gaussian(rng: np.random.Generator, shape) -> np.ndarray:
+    """Integers distributed as D_{Z, 8} on [-48, 48] (inverse transform)."""
+    u = rng.random(size=shape)
+    idx = np.searchsorted(_GAUSS_CDF, u, side="right")
+    return np.minimum(idx, 2 * GAUSS_BOUND).astype(np.int64) - GAUSS_BOUND
 
 
 class Yashe:
@@ -40,7 +66,6 @@ class Yashe:
         self.rng = np.random.default_rng(seed)
         self._ternary = ints_to_array([self.q - 1, 0, 1], self.L)            # index v + 1
         self._gauss = ints_to_array([v % self.q for v in range(-GAUSS_BOUND, GAUSS_BOUND + 1)], self.L)
-        self._dm = ints_to_array([self.delta * m for m in range(self.t)], self.L) if self.t <= (1 << 20) else None
         self.elt_bytes = self.L * self.n * 8
         self.f = self.ff = self.h = self.evk = None
 
@@ -49,8 +74,7 @@ class Yashe:
         return self.rng.integers(-1, 2, size=(batch, self.n), dtype=np.int64)
 
     def gaussian(self, batch: int) -> np.ndarray:
-        e = np.rint(self.rng.normal(0.0, GAUSS_SIGMA, size=(batch, self.n))).astype(np.int64)
-        return np.clip(e, -GAUSS_BOUND, GAUSS_BOUND)
+        return sample_discrete_gaussian(self.rng, (batch, self.n))
 
     def _lookup(self, table: np.ndarray, idx: np.ndarray) -> np.ndarray:
         """[batch][L][n] limbs of table[idx]."""
@@ -63,18 +87,20 @@ class Yashe:
             return self._lookup(self._gauss, v + GAUSS_BOUND)
         raise ValueError(kind)
 
-    def pack_scaled_message(self, m: np.ndarray) -> np.ndarray:
-        """floor(q/t) m mod q as limbs; m in [0, t)."""
-        m = np.asarray(m, dtype=np.int64)
-        if self._dm is not None:
-            return self._lookup(self._dm, m)
-        return np.stack([ints_to_array([self.delta * int(x) for x in row], self.L) for row in m])
+    def pack_message(self, m: np.ndarray) -> np.ndarray:
+        """Messages m in [0, t) as ring elements [batch][L][n] (t < 2^64)."""
+        m = np.atleast_2d(np.asarray(m, dtype=np.int64))
+        if m.min(initial=0) < 0 or m.max(initial=0) >= self.t:
+            raise YashError("messages must lie in [0, t)")
+        out = np.zeros((m.shape[0], self.L, self.n), dtype=np.uint64)
+        out[:, 0, :] = m.astype(np.uint64)
+        return out
 
     # ------------------------------------------------------------ keys
     def generate_keys(self):
         n, q, t = self.n, self.q, self.t
         ring = self.ring
-        while True:
+        for attempt in range(KEYGEN_ATTEMPTS):
             fp = self.ternary(1)[0]
             f_vals = [(t * int(x) + (1 if j == 0 else 0)) % q for j, x in enumerate(fp)]
             d_f = self.ctx.to_device(ints_to_array(f_vals, self.L)[None])
@@ -82,8 +108,11 @@ class Yashe:
             try:
                 ring.inverse(d_f, finv)
                 break
-            except YashError:  # f not invertible: draw again (src/yashe.cpp:45-55)
-                continue
+            except YashError as e:
+                # f not invertible: draw again (src/yashe.cpp:45-55); any other
+                # failure (allocation, capture) is not the sample's fault
+                if "not invertible" not in str(e) or attempt == KEYGEN_ATTEMPTS - 1:
+                    raise
         g = self.ternary(1)[0]
         d_tg = self.ctx.to_device(ints_to_array([(t * int(x)) % q for x in g], self.L)[None])
         self.h = self.ctx.alloc(self.elt_bytes)
@@ -162,7 +191,8 @@ class Yashe:
         ring.ring_mul_bcast(self.h, S, out, batch)
         E = self.ctx.to_device(self.pack_small(self.gaussian(batch), "gauss"))
         ring.ring_addsub(out, E, out, batch)
-        DM = self.ctx.to_device(self.pack_scaled_message(m))
+        DM = self.ctx.to_device(self.pack_message(m))
+        ring.scalar(DM, YH_MUL, self.delta, DM, batch)
         ring.ring_addsub(out, DM, out, batch)
         return out
 

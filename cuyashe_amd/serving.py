@@ -34,8 +34,8 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import OP_ADD, OP_MUL_KS, Context, DeviceBuffer, Event, check, lib
-from .scheme import GAUSS_BOUND, Yashe
+from ._lib import OP_ADD, OP_MUL_KS, YH_MUL, Context, DeviceBuffer, Event, check, lib
+from .scheme import Yashe, sample_discrete_gaussian
 from .workloads import Q127, Q255, Q511
 
 KINDS = ("add", "mulks", "encdec")
@@ -134,6 +134,16 @@ class Server:
     def _table(self) -> DeviceBuffer:
         return self._tables.pop() if self._tables else self.ctx.alloc(4 * self.max_batch * 8)
 
+    def close(self):
+        """Releases the rings' keys and rings (after the loop has drained)."""
+        self._pool.shutdown(wait=True)
+        for rs in self.rings.values():
+            rs.ring.sync()
+            if rs.Y.evk is not None:
+                rs.Y.evk.close()
+            rs.ring.close()
+        self.rings = {}
+
     def _event(self) -> Event:
         return self._events.pop() if self._events else Event(self.ctx)
 
@@ -157,10 +167,9 @@ class Server:
         for r in reqs:
             g = np.random.default_rng((self.seed, r.rid))
             S.append(g.integers(-1, 2, size=rs.n))
-            e = np.rint(g.normal(0.0, 8.0, size=rs.n)).astype(np.int64)
-            E.append(np.clip(e, -GAUSS_BOUND, GAUSS_BOUND))
+            E.append(sample_discrete_gaussian(g, rs.n))
         return (Y.pack_small(np.stack(S), "ternary"), Y.pack_small(np.stack(E), "gauss"),
-                Y.pack_scaled_message(np.stack([r.msg for r in reqs])))
+                Y.pack_message(np.stack([r.msg for r in reqs])))
 
     def _launch(self, rs: _RingState, kind: str, reqs: List[Request], reserved: int) -> _Group:
         k = len(reqs)
@@ -181,6 +190,7 @@ class Server:
             bufs["E"].upload(E)
             bufs["DM"].upload(DM)
             stage += list(bufs.items())
+            ring.scalar(bufs["DM"], YH_MUL, rs.Y.delta, bufs["DM"], k)  # floor(q/t) m mod q
             ring.ring_mul_bcast(rs.Y.h, bufs["S"], bufs["HS"], k)
             host[:k] = [bufs["HS"].ptr + m * rs.elt for m in range(k)]
             host[mb:mb + k] = [bufs["E"].ptr + m * rs.elt for m in range(k)]

@@ -40,8 +40,9 @@ extern "C" {
 
 #define YH_FORWARD 1 /* ntt_mode_t FORWARD, src/settings.h:38 */
 #define YH_INVERSE 0 /* ntt_mode_t INVERSE */
-#define YH_ADD 0     /* add_mode_t ADD, src/settings.h:37 */
+#define YH_ADD 0     /* add_mode_t ADD, src/settings.h:36 */
 #define YH_SUB 1     /* add_mode_t SUB */
+#define YH_MUL 2     /* add_mode_t MUL (scalar operations only) */
 
 typedef struct yh_ctx yh_ctx;
 typedef struct yh_ring yh_ring;
@@ -139,6 +140,15 @@ int yh_pointwise_mulmod(yh_ctx* ctx, uint64_t* d_a, const uint64_t* d_b, size_t 
  * the mod p_i that the reference omits. c may alias a. */
 int yh_residue_addsub(yh_ring* ring, uint64_t* d_c, const uint64_t* d_a, const uint64_t* d_b, int spacing,
                       int op);
+/* Scalar operation on residues [R][spacing] (values in [0, p_i)): op YH_ADD /
+ * YH_SUB adds / subtracts the scalar's residue to coefficient 0 of every
+ * residue row, YH_MUL multiplies every residue by it, all mod p_i. The
+ * scalar is a host integer of scalar_limbs 64-bit limbs (little-endian, any
+ * value; its residues are taken on the host). d_c may alias d_a; other
+ * coefficients are copied. Replaces CUDAFunctions::callPolynomialOPInteger
+ * (src/cuda_functions.cu:477-548), which returns a freshly allocated buffer. */
+int yh_residue_scalar(yh_ring* ring, uint64_t* d_c, const uint64_t* d_a, int spacing, int op, const uint64_t* scalar,
+                      int scalar_limbs);
 
 /* ------------------------------------------------------------ fused operations
  * All take `batch` independent operands laid out back to back. */
@@ -163,6 +173,15 @@ int yh_ring_mul(yh_ring* ring, const uint64_t* d_a, const uint64_t* d_b, uint64_
 /* c = a +- b mod q (op YH_ADD / YH_SUB) on [batch][q_limbs][n] in [0, q).
  * Replaces common_addition (src/common.h:24-80) followed by % q. */
 int yh_ring_addsub(yh_ring* ring, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out, int batch, int op);
+/* Scalar operation on ring elements [batch][q_limbs][n] in [0, q): op YH_ADD
+ * / YH_SUB gives (a_0 +- s) mod q on coefficient 0 (the other coefficients
+ * copied), YH_MUL gives a_j * s mod q on every coefficient, s = scalar mod q
+ * (scalar: host limbs, any value). d_out may alias d_a. Replaces the scalar
+ * operators of Polynomial (src/polynomial.h:459-573, the +, +=, -, -=, * and *= operators taking a ZZ)
+ * and Integer::operator* (src/integer.cpp:167-222) on ring elements, which
+ * the reference runs as polynomialOPInteger on residues followed by % q. */
+int yh_ring_scalar(yh_ring* ring, const uint64_t* d_a, const uint64_t* scalar, int scalar_limbs, uint64_t* d_out,
+                   int batch, int op);
 /* Reduction of [batch][in_limbs][2n] coefficients (any value < 2^(64 in_limbs))
  * mod (x^n + 1) and q: out_j = (c_j - c_{j+n}) mod q in [0, q).
  * Replaces Polynomial::reduce (src/cuda_functions.cu:1040-1078). */

@@ -190,6 +190,41 @@ def poly_mul_plain(a: Sequence[int], b: Sequence[int], N: int = None, qbits: int
     return from_limbs(out)
 
 
+# ------------------------------------------------------------------ scalar operations
+
+ADD, SUB, MUL = 0, 1, 2  # add_mode_t (src/settings.h:36)
+
+
+def residue_scalar(res: Sequence[Sequence[int]], primes: Sequence[int], op: int, s: int) -> List[List[int]]:
+    """polynomialOPInteger (src/cuda_functions.cu:477-548) on residues
+    [R][spacing]: ADD / SUB the scalar's residue on coefficient 0 of each
+    row, MUL every residue, all mod p_i (the scalar's residues are those of
+    Integer::crt, src/integer.cpp:62-82)."""
+    out = []
+    for row, p in zip(res, primes):
+        sp = int(s) % p
+        r = [int(v) for v in row]
+        if op == MUL:
+            r = [v * sp % p for v in r]
+        elif op == ADD:
+            r[0] = (r[0] + sp) % p
+        else:
+            r[0] = (r[0] - sp) % p
+        out.append(r)
+    return out
+
+
+def ring_scalar(a: Sequence[int], op: int, s: int, q: int) -> List[int]:
+    """The scalar operators on a ring element (src/polynomial.h:459-573,
+    Integer::operator* src/integer.cpp:167-222) followed by % q, the
+    arithmetic src/test.cu:630-790 checks against NTL's ZZ_p."""
+    r = [int(v) for v in a]
+    if op == MUL:
+        return [v * int(s) % q for v in r]
+    r[0] = (r[0] + int(s)) % q if op == ADD else (r[0] - int(s)) % q
+    return r
+
+
 # ------------------------------------------------------------------ ring operations
 
 def fold(c: Sequence[int], n: int) -> List[int]:

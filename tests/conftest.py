@@ -34,4 +34,9 @@ def gpu_ctx():
     import cuyashe_amd as ya
     ctx = ya.Context(0)
     yield ctx
-    ctx.close()
+    try:
+        ctx.close()
+    except ya.YashError as e:
+        # a failed test left rings or plans alive: keep the context (the
+        # process ends next) rather than free what they still reference
+        sys.__stderr__.write(f"[gpu_ctx] context kept alive: {e}\n")

@@ -188,6 +188,53 @@ int main() {
       }
       return true;
     });
+    run("randomPolynomialOperations (scalar *= and += on the device vs Z_q arithmetic)", [&] {
+      // src/test.cu:630-790 with random_ab = 3: c = a + b, then random
+      // c *= z or c += z (z = rand() % q), reduce(), compared with the same
+      // operations mod q; scalar operators on a reduced ring element run on
+      // the device and leave the device copy valid
+      srand(36251);
+      Polynomial a, b;
+      Polynomial::random(&a, degree - 1);
+      Polynomial::random(&b, degree - 1);
+      Polynomial c = a + b;
+      std::vector<ZZ> want(degree);
+      for (int i = 0; i < degree; ++i) want[i] = (a.get_coeff(i) + b.get_coeff(i)) % q;
+      int on_device = 0;
+      for (int count = 0; count < 40; ++count) {
+        const int op = rand() % 3;
+        ZZ z = ZZ((int64_t)rand()) * ZZ((int64_t)rand()) * ZZ((int64_t)rand()) % q;
+        if (op == 0) {
+          c *= z;
+          for (auto& v : want) v = (v * z) % q;
+        } else if (op == 1) {
+          c += z;
+          want[0] = (want[0] + z) % q;
+        } else {
+          c -= z;
+          want[0] = (want[0] - z) % q;
+        }
+        on_device += c.get_crt_computed() ? 1 : 0;
+        c.reduce();
+        c.normalize();
+        Polynomial w;
+        w.set_coeffs(want);
+        w.normalize();
+        if (!(c == w)) return false;
+      }
+      // every operation after the first reduce() ran on the device
+      if (on_device < 39) return false;
+      // Integer::operator* (src/integer.cpp:167-222): product then % mod
+      Integer t((uint64_t)35951);
+      Polynomial tc = t * c;
+      for (int i = 0; i < degree; ++i)
+        if (tc.get_coeff(i) != (c.get_coeff(i) * ZZ((int64_t)35951)) % q) return false;
+      // a polynomial without a valid ring image keeps exact integer arithmetic
+      Polynomial big;
+      big.set_coeff(0, q * ZZ((int64_t)3));
+      big *= ZZ((int64_t)5);
+      return big.get_coeff(0) == q * ZZ((int64_t)15) && !big.get_crt_computed();
+    });
     run("modularInversion (a * InvMod(a, phi) == 1)", [&] {
       srand(36251);
       Polynomial a;

@@ -124,10 +124,20 @@ def test_graph_circuit_depth3(gpu_ctx):
     eager = B["Z"].download((16, Y.L, n)).copy()
     p0 = get(B["Pm"], Y.ring, 1)[0]
     assert np.array_equal(Y.decrypt(B["Z"], 16), _plain_circuit(x, t))
-    # one level-1 node bit-exact against the oracle
+    # one node of every level bit-exact against the oracle (mul + keyswitch
+    # of the level's GPU inputs), and the additions feeding them
+    xs = get(X, Y.ring, 2)
     a = get(B["A"], Y.ring, 128)
+    assert a[0] == [(u + v) % q for u, v in zip(xs[0], xs[1])]
     gam = get(Y.gamma, Y.ring, Y.lw)
     assert p0 == cpuref.keyswitch(cpuref.yashe_mul(a[0], a[64], n, q, t), gam, n, q, 32)
+    b = get(B["B"], Y.ring, 64)
+    assert b[0] == [(u + v) % q for u, v in zip(p0, xs[0])]
+    r0 = get(B["Rm"], Y.ring, 1)[0]
+    assert r0 == cpuref.keyswitch(cpuref.yashe_mul(b[0], b[32], n, q, t), gam, n, q, 32)
+    s_ = get(B["S"], Y.ring, 32)
+    u0 = get(B["U"], Y.ring, 1)[0]
+    assert u0 == cpuref.keyswitch(cpuref.yashe_mul(s_[0], s_[16], n, q, t), gam, n, q, 32)
 
     gpu_ctx.graph_begin()
     _circuit(Y, X, **B)

@@ -15,7 +15,9 @@ RINGS = [(2048, workloads.Q127), (4096, workloads.Q255), (2048, workloads.Q511)]
 
 @pytest.fixture(scope="module")
 def server(gpu_ctx):
-    return Server(gpu_ctx, RINGS, t=1024, pool=16, slots=48, max_batch=32, seed=5, keep_outputs=True)
+    srv = Server(gpu_ctx, RINGS, t=1024, pool=16, slots=48, max_batch=32, seed=5, keep_outputs=True)
+    yield srv
+    srv.close()
 
 
 def test_serving_mix_all_decrypts(server):
@@ -54,3 +56,31 @@ def test_serving_admission_budget(gpu_ctx):
     one.budget = fp - 1
     with pytest.raises(ValueError):
         one.run([Request(0, big, RINGS[0], a=0, b=1, msg=reqs[0].msg)])
+    one.close()
+
+
+def test_serving_all_c5_rings(gpu_ctx):
+    # every ring of the serving configuration: n in {2048, 4096, 8192, 16384}
+    # and q of 127, 255 and 511 bits; each result decrypted and checked, and
+    # one multiplication per ring equal alone and in the mix
+    rings = workloads_c5()
+    srv = Server(gpu_ctx, rings, t=1024, pool=4, slots=8, max_batch=8, seed=11, keep_outputs=True)
+    reqs = poisson_requests(180, 20000.0, rings, 1024, 4, seed=12)
+    st = srv.run(reqs)
+    assert st["all_decrypts_ok"], [r.rid for r in reqs if not r.ok][:10]
+    assert {r.ring for r in reqs} == set(rings)
+    for key in rings:
+        mul = [r for r in reqs if r.ring == key and r.kind == "mulks"][:1]
+        for r in mul:
+            alone = copy.copy(r)
+            alone.arrival, alone.done, alone.ok, alone.output = 0.0, None, None, None
+            srv.run([alone])
+            assert alone.ok and (alone.output == r.output).all(), key
+    srv.close()
+
+
+def workloads_c5():
+    from cuyashe_amd.serving import c5_rings
+    rings = c5_rings()
+    assert len(rings) == 12
+    return rings

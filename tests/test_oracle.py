@@ -249,7 +249,53 @@ def test_planted_mistakes_are_caught(kats, monkeypatch):
     # (3) a transposed operand in the digit decomposition
     d = cpuref.worddecomp([(1 << 40) + 5], 32, 2)
     assert d == [[5], [256]] and d[::-1] != d
-    # (4) rounding up on 2*rem == ... (q odd: ties impossible) vs floor only
+    # (4) floor only, without rounding up when 2 * rem > q
     q, t = (1 << 61) - 1, 1024
-    x = q // 2 + 7
-    assert cpuref.round_tq(x, t, q) != (t * x // q) % q or (2 * (t * x % q) <= q)
+    x = next(v for v in range(q // 2, q // 2 + 5000) if 2 * (t * v % q) > q)
+    assert cpuref.round_tq(x, t, q) == (t * x // q + 1) % q
+    assert cpuref.round_tq(x, t, q) != (t * x // q) % q
+    # (5) a scalar add applied to every coefficient instead of coefficient 0
+    primes = cpuref.gen_primes(4, 64)
+    res = cpuref.crt([5, 6, 7, 8], primes)
+    wrong = [[(v + 3) % p for v in row] for row, p in zip(res, primes)]
+    assert cpuref.residue_scalar(res, primes, cpuref.ADD, 3) != wrong
+
+
+# ------------------------------------------------------------------ scalar operations
+
+def test_residue_scalar_through_crt_icrt():
+    # polynomialOPInteger on residues is the integer operation mod M: the
+    # (pinned) ICRT of the modified residues gives a_0 +- s, a_j s mod M
+    rng = random.Random(11)
+    q = (1 << 127) - 1
+    primes = cpuref.gen_primes(16, q.bit_length())
+    M = 1
+    for p in primes:
+        M *= p
+    a = [rng.randrange(q) for _ in range(16)]
+    res = cpuref.crt(a, primes)
+    for s in (0, 1, rng.randrange(q), rng.randrange(1 << 400)):
+        back = cpuref.icrt(cpuref.residue_scalar(res, primes, cpuref.MUL, s), primes)
+        assert back == [v * s % M for v in a]
+        back = cpuref.icrt(cpuref.residue_scalar(res, primes, cpuref.ADD, s), primes)
+        assert back == [(a[0] + s) % M] + a[1:]
+        back = cpuref.icrt(cpuref.residue_scalar(res, primes, cpuref.SUB, s), primes)
+        assert back == [(a[0] - s) % M] + a[1:]
+
+
+def test_ring_scalar_through_the_product():
+    # a scalar product is the ring product with the constant polynomial s
+    # (the pinned plain product, fold and mod q); adding s to coefficient 0
+    # distributes over a ring product: (a + s) b = a b + s b
+    rng = random.Random(12)
+    n, q = 16, (1 << 127) - 1
+    a = [rng.randrange(q) for _ in range(n)]
+    b = [rng.randrange(q) for _ in range(n)]
+    for s in (0, 1, q - 1, rng.randrange(q)):
+        const = [s % q] + [0] * (n - 1)
+        assert cpuref.ring_scalar(a, cpuref.MUL, s, q) == cpuref.ring_mul(a, const, n, q)
+        lhs = cpuref.ring_mul(cpuref.ring_scalar(a, cpuref.ADD, s, q), b, n, q)
+        ab = cpuref.ring_mul(a, b, n, q)
+        assert lhs == [(u + s * v) % q for u, v in zip(ab, b)]
+        lhs = cpuref.ring_mul(cpuref.ring_scalar(a, cpuref.SUB, s, q), b, n, q)
+        assert lhs == [(u - s * v) % q for u, v in zip(ab, b)]
