@@ -107,6 +107,7 @@ _SIGS = {
     "yh_evk_destroy": (ctypes.c_int, [_vp]),
     "yh_keyswitch": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_yashe_mul_keyswitch": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "yh_yashe_mul_keyswitch_static": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_yashe_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int]),
     "yh_mem_footprint": (ctypes.c_size_t, [_vp, ctypes.c_int]),
     "yh_wire_size": (ctypes.c_size_t, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -587,6 +588,12 @@ class Ring:
 
     def mul_keyswitch(self, evk: "Evk", c1: DeviceBuffer, c2: DeviceBuffer, out: DeviceBuffer, batch: int = 1):
         check(lib().yh_yashe_mul_keyswitch(self.h, evk.h, c1.ptr, c2.ptr, out.ptr, batch), "yh_yashe_mul_keyswitch")
+
+    def mul_keyswitch_static(self, evk: "Evk", c1: DeviceBuffer, c2: DeviceBuffer, out: DeviceBuffer,
+                             batch: int = 1):
+        """mul_keyswitch with the library's static device scratch (no allocation)."""
+        check(lib().yh_yashe_mul_keyswitch_static(self.h, evk.h, c1.ptr, c2.ptr, out.ptr, batch),
+              "yh_yashe_mul_keyswitch_static")
 
     def decrypt(self, f: DeviceBuffer, c: DeviceBuffer, m_out: DeviceBuffer, batch: int = 1):
         check(lib().yh_yashe_decrypt(self.h, f.ptr, c.ptr, m_out.ptr, batch), "yh_yashe_decrypt")

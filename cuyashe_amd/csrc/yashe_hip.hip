@@ -56,6 +56,12 @@ int upload_ntt16(int N, uint64_t** buf, yhk::NttTab* tab);
 
 }  // namespace
 
+// Device scratch of yh_yashe_mul_keyswitch_static: one copy per GPU, held
+// until the process ends. Host code never uses this name as an address (on
+// the host it is only a stand-in); it uses the per-device address returned
+// by hipGetSymbolAddress (static_scratch_ptr below).
+static __device__ __align__(256) uint8_t scratch[YH_STATIC_SCRATCH_BYTES];
+
 struct TimedLaunch {
   int kernel;
   hipEvent_t start, stop;
@@ -80,6 +86,7 @@ struct yh_ctx {
   std::vector<yh_ring*> cap_rings;
   std::vector<const yh_evk*> cap_evks;
   std::vector<yh_batch*> cap_batches;
+  uint8_t* static_scratch = nullptr;  // this device's copy of the allocation-free entry point's array
   int live = 0;  // rings, batch plans and graphs created on this context and not yet destroyed
 };
 
@@ -129,6 +136,26 @@ struct yh_evk {
 
 namespace {
 
+// Set while the allocation-free entry point runs: no event is created then.
+thread_local bool t_no_alloc = false;
+
+// Per-device address of `scratch`, looked up once per device (with that
+// device current) when the first context on it is created.
+std::mutex g_scratch_mu;
+std::map<int, uint8_t*> g_scratch_ptr;
+
+int static_scratch_ptr(int device, uint8_t** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) { *out = it->second; return YH_OK; }
+  void* p = nullptr;
+  YH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(YH_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+  g_scratch_ptr[device] = (uint8_t*)p;
+  *out = (uint8_t*)p;
+  return YH_OK;
+}
+
 hipEvent_t take_event(yh_ctx* ctx) {
   if (!ctx->event_pool.empty()) {
     hipEvent_t e = ctx->event_pool.back();
@@ -147,7 +174,7 @@ struct LaunchTimer {
   hipStream_t stream;
   hipEvent_t start = nullptr;
   LaunchTimer(yh_ctx* c, int k, hipStream_t s) : ctx(c), kernel(k), stream(s) {
-    if (ctx->timing && !ctx->capturing && (ctx->timing_mask >> kernel & 1u)) {
+    if (ctx->timing && !ctx->capturing && !t_no_alloc && (ctx->timing_mask >> kernel & 1u)) {
       start = take_event(ctx);
       if (start) (void)hipEventRecord(start, stream);
     }
@@ -539,6 +566,11 @@ int yh_ctx_create(int device, yh_ctx** out) {
   }
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return fail(YH_ERR_HIP, hipGetErrorString(e)); }
+  if (int st = static_scratch_ptr(device, &c->static_scratch)) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return st;
+  }
   *out = c;
   return YH_OK;
 }
@@ -1071,6 +1103,35 @@ int yh_yashe_mul_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, co
   if (int st = mulks_scratch(r, batch, &sc)) return st;
   if (int st = ring_product(r, sc, d_c1, d_c2, sc.tmp, batch, yhk::EPI_SCALE)) return st;
   return keyswitch_impl(r, sc, k, sc.tmp, d_out, batch);
+}
+
+int yh_yashe_mul_keyswitch_static(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, const uint64_t* d_c2,
+                                  uint64_t* d_out, int batch) {
+  if (!r || !k || !d_c1 || !d_c2 || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
+  if (k->ring != r) return fail(YH_ERR_ARG, "key belongs to another ring");
+  if (r->rd.t == 0) return fail(YH_ERR_ARG, "ring has no plaintext modulus t");
+  if (r->rd.n < 16) return fail(YH_ERR_UNSUPPORTED, "the allocation-free entry point takes n >= 16");
+  // layout of the static array: reconstruction rows | intermediate | digit transforms
+  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t yb = al(std::max(y_bytes_for(r, 0, batch), y_bytes_for(r, 1, batch)));
+  const size_t tb = al((size_t)batch * r->rd.LQ * r->rd.n * sizeof(uint64_t));
+  const size_t db = al(dig_bytes_for(r, batch));
+  if (yb + tb + db > (size_t)YH_STATIC_SCRATCH_BYTES)
+    return fail(YH_ERR_ARG, "batch needs " + std::to_string(yb + tb + db) + " bytes of scratch, more than the " +
+                                std::to_string((size_t)YH_STATIC_SCRATCH_BYTES) +
+                                " of the static array; split the batch");
+  if (int s = ring_enter(r)) return s;
+  uint8_t* base = r->ctx->static_scratch;
+  if (!base) return fail(YH_ERR_INTERNAL, "static scratch address not set up");
+  Scratch sc;
+  sc.y = (uint32_t*)base;
+  sc.tmp = (uint64_t*)(base + yb);
+  sc.dig = (uint64_t*)(base + yb + tb);
+  t_no_alloc = true;
+  int st = ring_product(r, sc, d_c1, d_c2, sc.tmp, batch, yhk::EPI_SCALE);
+  if (!st) st = keyswitch_impl(r, sc, k, sc.tmp, d_out, batch);
+  t_no_alloc = false;
+  return st;
 }
 
 static int decrypt_impl(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, uint64_t* d_m, int batch,

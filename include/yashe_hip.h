@@ -205,6 +205,20 @@ int yh_keyswitch(yh_ring* ring, const yh_evk* evk, const uint64_t* d_c, uint64_t
  * reference, src/ciphertext.cpp:44-47, made eager). */
 int yh_yashe_mul_keyswitch(yh_ring* ring, const yh_evk* evk, const uint64_t* d_c1, const uint64_t* d_c2,
                            uint64_t* d_out, int batch);
+/* yh_yashe_mul_keyswitch without any device allocation: its scratch
+ * (reconstruction rows, the intermediate product and the digit transforms,
+ * batch * (4 (max(R, R_ks) + 1) n + 8 q_limbs n + 8 l_w n) bytes, each part
+ * rounded up to 256) lives in one static device array of
+ * YH_STATIC_SCRATCH_BYTES per GPU, held until the process ends. A larger
+ * batch is refused with YH_ERR_ARG. Every call on one GPU shares that array:
+ * calls that overlap on one GPU (different contexts, or rings with their own
+ * streams) race and must be serialised by the caller. Results equal
+ * yh_yashe_mul_keyswitch. Requires n >= 16. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). */
+#define YH_STATIC_SCRATCH_BYTES (64u << 20)
+int yh_yashe_mul_keyswitch_static(yh_ring* ring, const yh_evk* evk, const uint64_t* d_c1, const uint64_t* d_c2,
+                                  uint64_t* d_out, int batch);
 /* Decryption core: m_j = round(t * [f*c]_q,j / q) mod t for every coefficient
  * ([.]_q in [0, q)). f: [q_limbs][n] (one key), c: [batch][q_limbs][n];
  * m: [batch][n] u64. Replaces Yashe::decrypt (src/yashe.cpp:143-201), which
