@@ -31,12 +31,33 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_hip(verbose: bool = True) -> str:
+# Translation units of the library: the C ABI's host logic and one per kernel
+# family (compiled in parallel, then linked; each object carries its own
+# gfx950 code object).
+HIP_SRCS = ["yashe_hip.hip", "k_prod.hip", "k_keyswitch.hip", "k_recon.hip", "k_misc.hip", "k_batch.hip"]
+OBJDIR = os.path.join(ROOT, "_build", "hip")
+
+
+def build_hip(verbose: bool = True, jobs: int = None) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
-    src = os.path.join(CSRC, "yashe_hip.hip")
+    os.makedirs(OBJDIR, exist_ok=True)
+    jobs = jobs or max(1, min(len(HIP_SRCS), os.cpu_count() or 1))
+    objs, procs, pending = [], [], list(HIP_SRCS)
+    while pending or procs:
+        while pending and len(procs) < jobs:
+            src = pending.pop(0)
+            obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+            objs.append(obj)
+            cmd = [HIPCC, *HIP_FLAGS, "-c", "-I", os.path.join(ROOT, "include"), "-o", obj, os.path.join(CSRC, src)]
+            print("+", " ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd)))
+        src, p = procs.pop(0)
+        if p.wait() != 0:
+            for _, q in procs:
+                q.wait()
+            raise subprocess.CalledProcessError(p.returncode, src)
     tmp = LIB_HIP + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, src]
-    _run(cmd)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB_HIP)
     return LIB_HIP
 

@@ -69,7 +69,7 @@ constexpr int BATCH_MAX_LOGN = 13;
 // elements plus padding = 17 W words whatever n is.
 __host__ __device__ constexpr size_t batch_lds_bytes(int W) { return (size_t)W * 17 * sizeof(uint64_t); }
 
-__global__ void k_batch_reset(int* head, int nq, int* done, int ninst) {
+static __global__ void k_batch_reset(int* head, int nq, int* done, int ninst) {
   for (int i = threadIdx.x; i < nq; i += blockDim.x) head[i] = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x) done[i] = 0;
 }

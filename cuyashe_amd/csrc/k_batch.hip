@@ -1,0 +1,30 @@
+// The persistent kernel of uneven batches (k_batch_prod) and its counter
+// reset.
+#include "launch.hpp"
+
+namespace yhl {
+
+hipError_t batch_reset(dim3 grid, dim3 block, hipStream_t s, int* head, int nq, int* done, int ninst) {
+  hipLaunchKernelGGL(yhk::k_batch_reset, grid, block, 0, s, head, nq, done, ninst);
+  return hipGetLastError();
+}
+
+hipError_t batch_prod(int maxlq, int waves, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                      const yhk::BatchArgs& args) {
+  if (maxlq > 4)
+    hipLaunchKernelGGL((yhk::k_batch_prod<8, 2>), grid, block, lds, s, args);
+  else if (waves >= 4)
+    hipLaunchKernelGGL((yhk::k_batch_prod<4, 4>), grid, block, lds, s, args);
+  else
+    hipLaunchKernelGGL((yhk::k_batch_prod<4, 3>), grid, block, lds, s, args);
+  return hipGetLastError();
+}
+
+void batch_attrs() {
+  for (const void* f : {(const void*)yhk::k_batch_prod<4, 3>, (const void*)yhk::k_batch_prod<4, 4>,
+                        (const void*)yhk::k_batch_prod<8, 2>})
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)yhk::batch_lds_bytes(yhk::BATCH_MAX_THREADS));
+}
+
+}  // namespace yhl

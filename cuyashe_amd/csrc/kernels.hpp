@@ -477,7 +477,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
 
 // Small transforms (N < 64): one thread per (pair, prime) computes the same
 // Goldilocks convolution of centred residues directly.
-__global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
+static __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
   const int i = blockIdx.x;
   const int pair = blockIdx.y;
   if (threadIdx.x != 0) return;
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
 }
 
 // Keyswitch for small rings (n < 64): direct negacyclic convolution per thread.
-__global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* gamma_res /*[lw][Rks][n] centred residues*/) {
+static __global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* gamma_res /*[lw][Rks][n] centred residues*/) {
   const int i = blockIdx.x;
   const int ct = blockIdx.y;
   if (threadIdx.x != 0) return;
@@ -747,7 +747,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ks_corr(Ring
 }
 
 // Keyswitch keys of small rings (n < 16): centred residues, no transform.
-__global__ void k_evk_residues(RingDev rd, const uint64_t* G, uint64_t* out) {
+static __global__ void k_evk_residues(RingDev rd, const uint64_t* G, uint64_t* out) {
   const int i = blockIdx.x;
   const int k = blockIdx.y;
   const int Rks = rd.set_R[1];
@@ -1354,7 +1354,7 @@ k_icrt(RingDev rd, IcrtArgs ia) {
 
 // CRT primitive: residues r[i][j] = coef_j mod p_i in [0, p_i), layout
 // [R][spacing] like the reference's d_polyCRT (src/polynomial.h:1435).
-__global__ void k_crt(RingDev rd, const uint64_t* A, int lin, int ncoef, size_t lstride, uint64_t* out,
+static __global__ void k_crt(RingDev rd, const uint64_t* A, int lin, int ncoef, size_t lstride, uint64_t* out,
                       int spacing) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
@@ -1367,7 +1367,7 @@ __global__ void k_crt(RingDev rd, const uint64_t* A, int lin, int ncoef, size_t 
 }
 
 // y_i = r_i * (M/p_i)^-1 mod p_i (the cuPreICRT inner step, src/cuda_bn.cu:875-879).
-__global__ void k_residue_to_y(RingDev rd, const uint64_t* res, int spacing, uint32_t* Y, int set) {
+static __global__ void k_residue_to_y(RingDev rd, const uint64_t* res, int spacing, uint32_t* Y, int set) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= spacing) return;
@@ -1384,7 +1384,7 @@ __global__ void k_residue_to_y(RingDev rd, const uint64_t* res, int spacing, uin
 
 // Residue-domain add/sub mod p_i (the reference's polynomialAddSub,
 // src/cuda_functions.cu:99-126, with the mod p_i it omits).
-__global__ void k_residue_addsub(RingDev rd, uint64_t* C, const uint64_t* A, const uint64_t* B, int spacing, int sub) {
+static __global__ void k_residue_addsub(RingDev rd, uint64_t* C, const uint64_t* A, const uint64_t* B, int spacing, int sub) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= spacing) return;
@@ -1399,7 +1399,7 @@ __global__ void k_residue_addsub(RingDev rd, uint64_t* C, const uint64_t* A, con
 }
 
 // Pointwise Goldilocks product (polynomialNTTMul, src/cuda_functions.cu:460-474).
-__global__ void k_pointwise(uint64_t* A, const uint64_t* B, size_t count) {
+static __global__ void k_pointwise(uint64_t* A, const uint64_t* B, size_t count) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < count) A[t] = gl::mul(A[t], B[t]);
 }
@@ -1556,7 +1556,7 @@ __global__ void k_scalar_q(RingDev rd, const uint64_t* A, uint64_t* C, int n, si
 // Residues [R][spacing] (modulus i in row i, values in [0, m_i)): the same
 // operations mod each CRT modulus, with the scalar's residue s_i = sa.v[i].
 // c may alias a.
-__global__ void k_residue_scalar(RingDev rd, uint64_t* C, const uint64_t* A, int spacing, int op, ScalarArg sa) {
+static __global__ void k_residue_scalar(RingDev rd, uint64_t* C, const uint64_t* A, int spacing, int op, ScalarArg sa) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= spacing) return;
@@ -1638,7 +1638,7 @@ __global__ void k_negate_odd(RingDev rd, const uint64_t* F, uint64_t* G, int m) 
 }
 
 // out_j = in_{2j}, j < m/2 (an even polynomial of degree < m as one in x^2)
-__global__ void k_take_even(const uint64_t* in, uint64_t* out, int m, int LQ) {
+static __global__ void k_take_even(const uint64_t* in, uint64_t* out, int m, int LQ) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int h = m / 2;
   if (j >= h) return;
@@ -1646,7 +1646,7 @@ __global__ void k_take_even(const uint64_t* in, uint64_t* out, int m, int LQ) {
 }
 
 // out_{2j} = in_j, out_{2j+1} = 0, j < m/2 (p(y) -> p(x^2))
-__global__ void k_spread_even(const uint64_t* in, uint64_t* out, int m, int LQ) {
+static __global__ void k_spread_even(const uint64_t* in, uint64_t* out, int m, int LQ) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   const int h = m / 2;

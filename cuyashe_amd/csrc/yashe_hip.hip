@@ -17,6 +17,7 @@
 #include "bigint.hpp"
 #include "batch.hpp"
 #include "kernels.hpp"
+#include "launch.hpp"
 #include "ring_params.hpp"
 #include "wire.hpp"
 
@@ -28,6 +29,13 @@ int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+#define YH_LAUNCH(expr)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ == hipErrorInvalidValue) return fail(YH_ERR_UNSUPPORTED, "shape out of range");  \
+    if (e_ != hipSuccess) return fail(YH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
 
 #define YH_HIP(expr)                                                                        \
   do {                                                                                      \
@@ -298,103 +306,33 @@ int get_cyclic(yh_ctx* ctx, int N, DevTables* out) {
 
 int ilog2i(long v) { int l = 0; while ((1L << l) < v) ++l; return l; }
 
-size_t lds_bytes(int N) { return (size_t)(N + N / 16) * sizeof(uint64_t); }
-
-// Every transform kernel is instantiated per log2(N) in [4, 14].
-#define YH_LOGN_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
-
-template <int LOGN>
-void set_lds_attrs() {
-  const int bytes = (int)lds_bytes(1 << LOGN);
-  (void)hipFuncSetAttribute((const void*)yhk::k_prod<LOGN, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_prod<LOGN, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_ks_digits<LOGN, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_ks_digits<LOGN, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_ks_mac<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_evk_prep<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_ks_corr<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_ntt_cyclic<LOGN>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-}
+using yhl::lds_bytes;
 
 // Launch the fused product kernel for `batch` pairs on N-point transforms.
 int launch_prod(yh_ring* r, yhk::ProdArgs pa, int N, int batch, bool negacyclic) {
   pa.batch = batch;
-  hipStream_t s = r->stream;
-  int R = r->rd.set_R[pa.set];
+  const int R = r->rd.set_R[pa.set];
   LaunchTimer timer(r->ctx, YH_KERNEL_PROD, r->stream);
-  if (N >= 16) {
-    dim3 grid(yhk::xcd_grid(batch, R)), block(N / 16);
-    if (negacyclic) {
-      switch (pa.logN) {
-#define X(L) case L: hipLaunchKernelGGL((yhk::k_prod<L, true>), grid, block, lds_bytes(N), s, r->rd, pa); break;
-        YH_LOGN_CASES(X)
-#undef X
-        default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
-      }
-    } else {
-      switch (pa.logN) {
-#define X(L) case L: hipLaunchKernelGGL((yhk::k_prod<L, false>), grid, block, lds_bytes(N), s, r->rd, pa); break;
-        YH_LOGN_CASES(X)
-#undef X
-        default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
-      }
-    }
-  } else {
-    dim3 grid(R, batch), block(64);
-    hipLaunchKernelGGL(yhk::k_prod_small, grid, block, 0, s, r->rd, pa, negacyclic ? 1 : 0);
-  }
-  YH_HIP(hipGetLastError());
+  if (N >= 16)
+    YH_LAUNCH(yhl::prod(pa.logN, negacyclic, dim3(yhk::xcd_grid(batch, R)), dim3(N / 16), r->stream, r->rd, pa));
+  else
+    YH_LAUNCH(yhl::prod(pa.logN, negacyclic, dim3(R, batch), dim3(64), r->stream, r->rd, pa));
   return YH_OK;
-}
-
-template <int LQ>
-void launch_icrt_t(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
-  const int per = YH_ICRT_BLOCK * (yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode) ? 2 : YH_ICRT_COEFS);
-  dim3 block(YH_ICRT_BLOCK), grid((ia.ncoef + per - 1) / per, batch);
-  hipLaunchKernelGGL(yhk::k_icrt<LQ>, grid, block, 0, r->stream, r->rd, ia);
 }
 
 int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
   LaunchTimer timer(r->ctx, YH_KERNEL_ICRT, r->stream);
-  switch (r->rd.LQ) {
-    case 1: launch_icrt_t<1>(r, ia, batch); break;
-    case 2: launch_icrt_t<2>(r, ia, batch); break;
-    case 3: launch_icrt_t<3>(r, ia, batch); break;
-    case 4: launch_icrt_t<4>(r, ia, batch); break;
-    case 5: launch_icrt_t<5>(r, ia, batch); break;
-    case 6: launch_icrt_t<6>(r, ia, batch); break;
-    case 7: launch_icrt_t<7>(r, ia, batch); break;
-    case 8: launch_icrt_t<8>(r, ia, batch); break;
-    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
-  }
-  YH_HIP(hipGetLastError());
+  const int LQ = r->rd.LQ;
+  const int per = YH_ICRT_BLOCK * (yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode) ? 2 : YH_ICRT_COEFS);
+  YH_LAUNCH(yhl::icrt(LQ, dim3((ia.ncoef + per - 1) / per, batch), dim3(YH_ICRT_BLOCK), r->stream, r->rd, ia));
   return YH_OK;
-}
-
-template <int LQ>
-void launch_addsub_t(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* c, int batch, int sub,
-                     yhk::AddsubLists lists) {
-  size_t count = (size_t)batch * r->rd.n;
-  dim3 block(256), grid((unsigned)((count + 255) / 256));
-  hipLaunchKernelGGL(yhk::k_addsub_q<LQ>, grid, block, 0, r->stream, r->rd, a, b, c, r->rd.n, count, sub, lists);
 }
 
 int addsub_impl(yh_ring* r, const uint64_t* a, const uint64_t* b, uint64_t* c, int batch, int sub,
                 yhk::AddsubLists lists) {
-  switch (r->rd.LQ) {
-    case 1: launch_addsub_t<1>(r, a, b, c, batch, sub, lists); break;
-    case 2: launch_addsub_t<2>(r, a, b, c, batch, sub, lists); break;
-    case 3: launch_addsub_t<3>(r, a, b, c, batch, sub, lists); break;
-    case 4: launch_addsub_t<4>(r, a, b, c, batch, sub, lists); break;
-    case 5: launch_addsub_t<5>(r, a, b, c, batch, sub, lists); break;
-    case 6: launch_addsub_t<6>(r, a, b, c, batch, sub, lists); break;
-    case 7: launch_addsub_t<7>(r, a, b, c, batch, sub, lists); break;
-    case 8: launch_addsub_t<8>(r, a, b, c, batch, sub, lists); break;
-    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
-  }
-  YH_HIP(hipGetLastError());
+  const size_t count = (size_t)batch * r->rd.n;
+  YH_LAUNCH(yhl::addsub_q(r->rd.LQ, dim3((unsigned)((count + 255) / 256)), dim3(256), r->stream, r->rd, a, b, c,
+                          r->rd.n, count, sub, lists));
   return YH_OK;
 }
 
@@ -440,12 +378,6 @@ int ring_product(yh_ring* r, const Scratch& sc, const uint64_t* a, const uint64_
   return launch_icrt(r, ia, batch);
 }
 
-template <int LQ>
-void launch_reduce_t(yh_ring* r, const uint64_t* c, int lin, uint64_t* out, int batch) {
-  dim3 block(256), grid((r->rd.n + 255) / 256, batch);
-  hipLaunchKernelGGL(yhk::k_reduce<LQ>, grid, block, 0, r->stream, r->rd, c, lin, out);
-}
-
 size_t dig_bytes_for(const yh_ring* r, int batch) {
   return r->rd.n >= 16 ? (size_t)batch * r->rd.lw * r->rd.n * sizeof(uint64_t) : 0;
 }
@@ -468,36 +400,16 @@ int keyswitch_impl(yh_ring* r, const Scratch& sc, const yh_evk* k, const uint64_
     const bool w32 = r->rd.w_bits == 32;
     {
       LaunchTimer timer(r->ctx, YH_KERNEL_KS_DIGITS, r->stream);
-      dim3 grid(yhk::xcd_grid(batch, r->rd.lw));
-      switch (r->rd.logn) {
-#define X(L)                                                                                                   \
-  case L:                                                                                                        \
-    if (w32)                                                                                                     \
-      hipLaunchKernelGGL((yhk::k_ks_digits<L, true>), grid, dim3(n / 16), lds_bytes(n), r->stream, r->rd, ka);  \
-    else                                                                                                         \
-      hipLaunchKernelGGL((yhk::k_ks_digits<L, false>), grid, dim3(n / 16), lds_bytes(n), r->stream, r->rd, ka); \
-    break;
-        YH_LOGN_CASES(X)
-#undef X
-        default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
-      }
+      YH_LAUNCH(yhl::ks_digits(r->rd.logn, w32, dim3(yhk::xcd_grid(batch, r->rd.lw)), r->stream, r->rd, ka));
     }
-    YH_HIP(hipGetLastError());
     {
       LaunchTimer timer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
-      dim3 grid(yhk::xcd_grid(batch, Rks));
-      switch (r->rd.logn) {
-#define X(L) case L: hipLaunchKernelGGL((yhk::k_ks_mac<L>), grid, dim3(n / 16), lds_bytes(n), r->stream, r->rd, ka); break;
-        YH_LOGN_CASES(X)
-#undef X
-        default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
-      }
+      YH_LAUNCH(yhl::ks_mac(r->rd.logn, dim3(yhk::xcd_grid(batch, Rks)), r->stream, r->rd, ka));
     }
   } else {
     LaunchTimer timer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
-    hipLaunchKernelGGL(yhk::k_keyswitch_small, dim3(Rks, batch), dim3(64), 0, r->stream, r->rd, ka, k->d);
+    YH_LAUNCH(yhl::ks_small(dim3(Rks, batch), r->stream, r->rd, ka, k->d));
   }
-  YH_HIP(hipGetLastError());
   yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0);
   ia.out_list = out_list;
   return launch_icrt(r, ia, batch);
@@ -547,13 +459,10 @@ int yh_ctx_create(int device, yh_ctx** out) {
   YH_HIP(hipSetDevice(device));
   // Transforms keep a whole length-16384 vector (+1/16 padding) in LDS:
   // 139264 bytes, within gfx950's 160 KiB per workgroup.
-#define X(L) set_lds_attrs<L>();
-  YH_LOGN_CASES(X)
-#undef X
-  for (const void* f : {(const void*)yhk::k_batch_prod<4, 3>, (const void*)yhk::k_batch_prod<4, 4>,
-                        (const void*)yhk::k_batch_prod<8, 2>})
-    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)yhk::batch_lds_bytes(yhk::BATCH_MAX_THREADS));
+  yhl::prod_attrs();
+  yhl::ks_attrs();
+  yhl::misc_attrs();
+  yhl::batch_attrs();
   auto* c = new yh_ctx();
   c->device = device;
   {
@@ -839,9 +748,7 @@ int yh_crt(yh_ring* r, const uint64_t* d_coefs, int limbs, int ncoef, int spacin
   if (ncoef < 0 || spacing < ncoef || spacing <= 0) return fail(YH_ERR_ARG, "bad spacing");
   if (int s = ring_enter(r)) return s;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  hipLaunchKernelGGL(yhk::k_crt, grid, block, 0, r->stream, r->rd, d_coefs, limbs, ncoef, (size_t)ncoef,
-                     d_res, spacing);
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::crt(grid, block, r->stream, r->rd, d_coefs, limbs, ncoef, (size_t)ncoef, d_res, spacing));
   return YH_OK;
 }
 
@@ -852,8 +759,7 @@ int yh_icrt(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, uint64_t*
   Scratch sc;
   if (int st = ring_y(r, (size_t)yhk::y_rows(r->rd.R) * spacing * sizeof(uint32_t), &sc)) return st;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  hipLaunchKernelGGL(yhk::k_residue_to_y, grid, block, 0, r->stream, r->rd, d_res, spacing, sc.y, 0);
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::residue_to_y(grid, block, r->stream, r->rd, d_res, spacing, sc.y, 0));
   return launch_icrt(r, icrt_args(sc.y, spacing, ncoef, 0, yhk::EPI_RAW, d_coefs, out_limbs, ncoef, 0, 1), 1);
 }
 
@@ -866,13 +772,7 @@ int yh_ntt(yh_ctx* ctx, uint64_t* d_data, int N, int count, int direction) {
   if (int s = get_cyclic(ctx, N, &t)) return s;
   uint64_t Ninv = gl::inv((uint64_t)N);
   const int inv = direction == YH_INVERSE ? 1 : 0;
-  switch (ilog2i(N)) {
-#define X(L) case L: hipLaunchKernelGGL(yhk::k_ntt_cyclic<L>, dim3(count), dim3(N / 16), lds_bytes(N), ctx->stream, d_data, t.fwd, t.inv, Ninv, inv); break;
-    YH_LOGN_CASES(X)
-#undef X
-    default: return fail(YH_ERR_UNSUPPORTED, "transform length out of range");
-  }
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::ntt_cyclic(ilog2i(N), count, ctx->stream, d_data, t.fwd, t.inv, Ninv, inv));
   return YH_OK;
 }
 
@@ -880,9 +780,7 @@ int yh_pointwise_mulmod(yh_ctx* ctx, uint64_t* d_a, const uint64_t* d_b, size_t 
   if (!ctx || !d_a || !d_b) return fail(YH_ERR_ARG, "null argument");
   if (count == 0) return YH_OK;
   if (int s = set_dev(ctx)) return s;
-  hipLaunchKernelGGL(yhk::k_pointwise, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, d_a, d_b,
-                     count);
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::pointwise(dim3((unsigned)((count + 255) / 256)), dim3(256), ctx->stream, d_a, d_b, count));
   return YH_OK;
 }
 
@@ -890,9 +788,7 @@ int yh_residue_addsub(yh_ring* r, uint64_t* d_c, const uint64_t* d_a, const uint
   if (!r || !d_c || !d_a || !d_b || spacing <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (int s = ring_enter(r)) return s;
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  hipLaunchKernelGGL(yhk::k_residue_addsub, grid, block, 0, r->stream, r->rd, d_c, d_a, d_b, spacing,
-                     op == YH_SUB ? 1 : 0);
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::residue_addsub(grid, block, r->stream, r->rd, d_c, d_a, d_b, spacing, op == YH_SUB ? 1 : 0));
   return YH_OK;
 }
 
@@ -960,18 +856,8 @@ int yh_ring_reduce(yh_ring* r, const uint64_t* d_c, int in_limbs, uint64_t* d_ou
   if (!r || !d_c || !d_out || batch <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (in_limbs < 1 || in_limbs > r->rd.LW) return fail(YH_ERR_ARG, "in_limbs out of range");
   if (int s = ring_enter(r)) return s;
-  switch (r->rd.LQ) {
-    case 1: launch_reduce_t<1>(r, d_c, in_limbs, d_out, batch); break;
-    case 2: launch_reduce_t<2>(r, d_c, in_limbs, d_out, batch); break;
-    case 3: launch_reduce_t<3>(r, d_c, in_limbs, d_out, batch); break;
-    case 4: launch_reduce_t<4>(r, d_c, in_limbs, d_out, batch); break;
-    case 5: launch_reduce_t<5>(r, d_c, in_limbs, d_out, batch); break;
-    case 6: launch_reduce_t<6>(r, d_c, in_limbs, d_out, batch); break;
-    case 7: launch_reduce_t<7>(r, d_c, in_limbs, d_out, batch); break;
-    case 8: launch_reduce_t<8>(r, d_c, in_limbs, d_out, batch); break;
-    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
-  }
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::reduce(r->rd.LQ, dim3((r->rd.n + 255) / 256, batch), dim3(256), r->stream, r->rd, d_c, in_limbs,
+                        d_out));
   return YH_OK;
 }
 
@@ -993,13 +879,7 @@ int yh_ring_scalar(yh_ring* r, const uint64_t* d_a, const uint64_t* scalar, int 
   if (int s = ring_enter(r)) return s;
   const size_t count = (size_t)batch * r->rd.n;
   const dim3 block(256), grid((unsigned)((count + 255) / 256));
-  switch (r->rd.LQ) {
-#define X(L) case L: hipLaunchKernelGGL(yhk::k_scalar_q<L>, grid, block, 0, r->stream, r->rd, d_a, d_out, r->rd.n, count, op, sa); break;
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
-#undef X
-    default: return fail(YH_ERR_UNSUPPORTED, "unsupported limb count");
-  }
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::scalar_q(r->rd.LQ, grid, block, r->stream, r->rd, d_a, d_out, r->rd.n, count, op, sa));
   return YH_OK;
   YH_TRY_END
 }
@@ -1015,8 +895,7 @@ int yh_residue_scalar(yh_ring* r, uint64_t* d_c, const uint64_t* d_a, int spacin
     if (int st = scalar_mod(scalar, scalar_limbs, yh::BigInt(r->P->primes[i]), &sa.v[i], 1)) return st;
   if (int s = ring_enter(r)) return s;
   const dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  hipLaunchKernelGGL(yhk::k_residue_scalar, grid, block, 0, r->stream, r->rd, d_c, d_a, spacing, op, sa);
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::residue_scalar(grid, block, r->stream, r->rd, d_c, d_a, spacing, op, sa));
   return YH_OK;
   YH_TRY_END
 }
@@ -1040,26 +919,13 @@ int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
   size_t bytes = (size_t)r->rd.lw * r->rd.set_R[1] * r->rd.n * sizeof(uint64_t);
   hipError_t e = hipMalloc(&k->d, bytes);
   if (e != hipSuccess) { delete k; return fail(YH_ERR_NOMEM, hipGetErrorString(e)); }
-  dim3 grid(r->rd.set_R[1], r->rd.lw);
   if (r->rd.n >= 16) {
     e = hipMalloc(&k->corr, (size_t)r->rd.set_R[1] * r->rd.n * sizeof(uint64_t));
     if (e != hipSuccess) { (void)hipFree(k->d); delete k; return fail(YH_ERR_NOMEM, hipGetErrorString(e)); }
-    switch (r->rd.logn) {
-#define X(L)                                                                                                     \
-  case L:                                                                                                          \
-    hipLaunchKernelGGL(yhk::k_evk_prep<L>, grid, dim3(r->rd.n / 16), lds_bytes(r->rd.n), r->stream, r->rd, d_gamma, \
-                       k->d);                                                                                      \
-    hipLaunchKernelGGL(yhk::k_ks_corr<L>, dim3(r->rd.set_R[1]), dim3(r->rd.n / 16), lds_bytes(r->rd.n), r->stream, \
-                       r->rd, k->d, k->corr);                                                                      \
-    break;
-      YH_LOGN_CASES(X)
-#undef X
-      default: break;
-    }
+    e = yhl::evk_prepare(r->rd.logn, r->stream, r->rd, d_gamma, k->d, k->corr);
   } else {
-    hipLaunchKernelGGL(yhk::k_evk_residues, grid, dim3(64), 0, r->stream, r->rd, d_gamma, k->d);
+    e = yhl::evk_residues(r->stream, r->rd, d_gamma, k->d);
   }
-  e = hipGetLastError();
   if (e != hipSuccess) {
     (void)hipFree(k->d);
     if (k->corr) (void)hipFree(k->corr);
@@ -1303,12 +1169,6 @@ static int sub_ring(yh_ring* r, int m, yh_ring** out) {
   return YH_OK;
 }
 
-extern "C++" {
-template <int LQ>
-static void launch_negate_odd(yh_ring* r, const uint64_t* F, uint64_t* G, int m) {
-  hipLaunchKernelGGL(yhk::k_negate_odd<LQ>, dim3((m + 255) / 256), dim3(256), 0, r->stream, r->rd, F, G, m);
-}
-}
 
 int yh_ring_inverse(yh_ring* r, const uint64_t* d_f, uint64_t* d_out) {
   if (!r || !d_f || !d_out) return fail(YH_ERR_ARG, "null argument");
@@ -1336,20 +1196,9 @@ int yh_ring_inverse(yh_ring* r, const uint64_t* d_f, uint64_t* d_out) {
     uint64_t* G = Gs + off;
     goff.push_back(off);
     off += (size_t)LQ * m;
-    switch (LQ) {
-      case 1: launch_negate_odd<1>(s, F, G, m); break;
-      case 2: launch_negate_odd<2>(s, F, G, m); break;
-      case 3: launch_negate_odd<3>(s, F, G, m); break;
-      case 4: launch_negate_odd<4>(s, F, G, m); break;
-      case 5: launch_negate_odd<5>(s, F, G, m); break;
-      case 6: launch_negate_odd<6>(s, F, G, m); break;
-      case 7: launch_negate_odd<7>(s, F, G, m); break;
-      default: launch_negate_odd<8>(s, F, G, m); break;
-    }
-    YH_HIP(hipGetLastError());
+    YH_LAUNCH(yhl::negate_odd(LQ, dim3((m + 255) / 256), dim3(256), s->stream, s->rd, F, G, m));
     if (int st = yh_ring_mul(s, F, G, T, 1)) return st;  // N(F)(x^2)
-    hipLaunchKernelGGL(yhk::k_take_even, dim3((m / 2 + 255) / 256), dim3(256), 0, r->stream, T, F, m, LQ);
-    YH_HIP(hipGetLastError());
+    YH_LAUNCH(yhl::take_even(dim3((m / 2 + 255) / 256), dim3(256), r->stream, T, F, m, LQ));
   }
   // F now holds the scalar N(...)(y) in Z_q[y]/(y + 1): invert it on the host
   std::vector<uint64_t> h(LQ);
@@ -1370,8 +1219,7 @@ int yh_ring_inverse(yh_ring* r, const uint64_t* d_f, uint64_t* d_out) {
     const int m = sizes[k];
     yh_ring* s = nullptr;
     if (int st = sub_ring(r, m, &s)) return st;
-    hipLaunchKernelGGL(yhk::k_spread_even, dim3((m + 255) / 256), dim3(256), 0, r->stream, F, U, m, LQ);
-    YH_HIP(hipGetLastError());
+    YH_LAUNCH(yhl::spread_even(dim3((m + 255) / 256), dim3(256), r->stream, F, U, m, LQ));
     uint64_t* dst = k == 0 ? d_out : F;
     if (int st = yh_ring_mul(s, Gs + goff[k], U, dst, 1)) return st;
   }
@@ -1693,19 +1541,11 @@ int yh_batch_run(yh_batch* b) {
   if (b->ctx->capturing &&
       std::find(b->ctx->cap_batches.begin(), b->ctx->cap_batches.end(), b) == b->ctx->cap_batches.end())
     b->ctx->cap_batches.push_back(b);
-  hipLaunchKernelGGL(yhk::k_batch_reset, dim3(std::max(1, std::min(64, (b->ninst + 255) / 256))), dim3(256), 0, st,
-                     b->args.head, b->args.nqueues, b->args.done, b->ninst);
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::batch_reset(dim3(std::max(1, std::min(64, (b->ninst + 255) / 256))), dim3(256), st, b->args.head,
+                             b->args.nqueues, b->args.done, b->ninst));
   LaunchTimer timer(b->ctx, YH_KERNEL_BATCH, b->ctx->stream);
-  const dim3 grid(b->grid), block(b->W);
-  const size_t lds = yhk::batch_lds_bytes(b->W);
-  if (b->maxlq > 4)
-    hipLaunchKernelGGL((yhk::k_batch_prod<8, 2>), grid, block, lds, st, b->args);
-  else if (b->cfg.waves >= 4)
-    hipLaunchKernelGGL((yhk::k_batch_prod<4, 4>), grid, block, lds, st, b->args);
-  else
-    hipLaunchKernelGGL((yhk::k_batch_prod<4, 3>), grid, block, lds, st, b->args);
-  YH_HIP(hipGetLastError());
+  YH_LAUNCH(yhl::batch_prod(b->maxlq, b->cfg.waves, dim3(b->grid), dim3(b->W), yhk::batch_lds_bytes(b->W), st,
+                            b->args));
   return YH_OK;
 }
 
