@@ -94,7 +94,9 @@ _SIGS = {
     "yh_ntt": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "yh_pointwise_mulmod": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
     "yh_residue_addsub": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int]),
-    "yh_residue_scalar": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _u64p, ctypes.c_int]),
+    "yh_residue_scalar": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "yh_crt_bn": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "yh_icrt_bn": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "yh_ring_scalar": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]),
     "yh_poly_mul_plain": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int)]),
@@ -520,11 +522,19 @@ class Ring:
         check(lib().yh_residue_addsub(self.h, c.ptr, a.ptr, b.ptr, spacing, YH_SUB if sub else YH_ADD),
               "yh_residue_addsub")
 
-    def residue_scalar(self, c: DeviceBuffer, a: DeviceBuffer, spacing: int, op: int, scalar: int):
-        """op (YH_ADD / YH_SUB on coefficient 0, YH_MUL on all) with a scalar
-        on residues [R][spacing], mod each CRT modulus."""
-        s, L = _scalar_limbs(scalar)
-        check(lib().yh_residue_scalar(self.h, c.ptr, a.ptr, spacing, op, s, L), "yh_residue_scalar")
+    def residue_scalar(self, c: DeviceBuffer, a: DeviceBuffer, spacing: int, op: int, scalar_res: DeviceBuffer):
+        """op (YH_ADD / YH_SUB on coefficient 0, YH_MUL on all) on residues
+        [R][spacing], mod each CRT modulus, with the scalar given by its R
+        residues (a device buffer of R words)."""
+        check(lib().yh_residue_scalar(self.h, c.ptr, a.ptr, spacing, op, scalar_res.ptr), "yh_residue_scalar")
+
+    def crt_bn(self, coefs_bn: DeviceBuffer, ncoef: int, spacing: int, out: DeviceBuffer):
+        """CRT of coefficients held as a device array of yh_bn_t."""
+        check(lib().yh_crt_bn(self.h, coefs_bn.ptr, ncoef, spacing, out.ptr), "yh_crt_bn")
+
+    def icrt_bn(self, res: DeviceBuffer, spacing: int, ncoef: int, coefs_bn: DeviceBuffer):
+        """ICRT into a device array of yh_bn_t."""
+        check(lib().yh_icrt_bn(self.h, res.ptr, spacing, ncoef, coefs_bn.ptr), "yh_icrt_bn")
 
     def scalar(self, a: DeviceBuffer, op: int, scalar: int, out: DeviceBuffer, batch: int = 1):
         """Ring elements: op (YH_ADD / YH_SUB on coefficient 0, YH_MUL on all)

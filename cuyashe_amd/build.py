@@ -65,6 +65,7 @@ def build_hip(verbose: bool = True, jobs: int = None) -> str:
 HOST_SRCS = ["polynomial.cpp", "yashe.cpp"]
 LIB_HOST = os.path.join(LIBDIR, "libcuyashe_host.so")
 TEST_API = os.path.join(ROOT, "tests", "cpp", "build", "test_api")
+TEST_SHIM = os.path.join(ROOT, "tests", "cpp", "build", "test_reference_shim")
 
 
 def build_host() -> str:
@@ -77,6 +78,10 @@ def build_host() -> str:
           "-L", LIBDIR, "-lyashe_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, LIB_HOST)
     os.makedirs(os.path.dirname(TEST_API), exist_ok=True)
+    # the reference's device-layer interface over the ABI, and its test program
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(ROOT, "tests", "cpp", "test_reference_shim.cpp"),
+          os.path.join(HERE, "host", "reference_shim.cpp"), "-L", LIBDIR, "-lyashe_hip",
+          "-Wl,-rpath," + LIBDIR, "-Wl,-rpath,$ORIGIN/../../../cuyashe_amd/lib", "-o", TEST_SHIM])
     _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(ROOT, "tests", "cpp", "test_api.cpp"),
           "-L", LIBDIR, "-lcuyashe_host", "-lyashe_hip", "-Wl,-rpath," + LIBDIR, "-Wl,-rpath,$ORIGIN/../../../cuyashe_amd/lib",
           "-o", TEST_API])

@@ -38,8 +38,19 @@ hipError_t residue_addsub(dim3 grid, dim3 block, hipStream_t s, const RingDev& r
 }
 
 hipError_t residue_scalar(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, uint64_t* C, const uint64_t* A,
-                          int spacing, int op, const yhk::ScalarArg& sa) {
-  hipLaunchKernelGGL(yhk::k_residue_scalar, grid, block, 0, s, rd, C, A, spacing, op, sa);
+                          int spacing, int op, const uint64_t* S) {
+  hipLaunchKernelGGL(yhk::k_residue_scalar, grid, block, 0, s, rd, C, A, spacing, op, S);
+  return hipGetLastError();
+}
+
+hipError_t crt_bn(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const yhk::BnT* C, int ncoef, uint64_t* out,
+                  int spacing) {
+  hipLaunchKernelGGL(yhk::k_crt_bn, grid, block, 0, s, rd, C, ncoef, out, spacing);
+  return hipGetLastError();
+}
+
+hipError_t to_bn(dim3 grid, dim3 block, hipStream_t s, const uint64_t* V, int lw, int ncoef, yhk::BnT* C) {
+  hipLaunchKernelGGL(yhk::k_to_bn, grid, block, 0, s, V, lw, ncoef, C);
   return hipGetLastError();
 }
 

@@ -1366,6 +1366,62 @@ static __global__ void k_crt(RingDev rd, const uint64_t* A, int lin, int ncoef, 
   out[(size_t)i * spacing + j] = r;
 }
 
+// The reference's multi-word integer (bn_t, src/cuda_bn.h:18-23): words
+// dp[0..used) in device memory, least significant first. Same layout as
+// yh_bn_t of the C ABI.
+struct BnT {
+  int alloc;
+  int used;
+  int sign;
+  uint64_t* dp;
+};
+
+// CRT of coefficients held as BnT (callCRT's input, src/cuda_bn.cu:970-1002):
+// Horner steps over the words from the top, each r <- (r 2^64 + w) mod p_i
+// with 2^64 and 2^32 mod p_i from the ring's table (any word count).
+static __global__ void k_crt_bn(RingDev rd, const BnT* C, int ncoef, uint64_t* out, int spacing) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= spacing) return;
+  uint64_t r = 0;
+  if (j < ncoef) {
+    const BnT c = C[j];
+    if (i == 0) {
+      for (int l = c.used - 1; l >= 0; --l) r = gl::reduce128(r, c.dp[l]);
+    } else {
+      const uint64_t* pw = rd.pow32[0] + (size_t)i * 2 * rd.LW;  // pw[k] = 2^(32k) mod p_i
+      const SmallPrime& k = rd.sp[0][i];
+      const uint32_t c32 = (uint32_t)pw[1], c64 = (uint32_t)pw[2];
+      uint32_t acc = 0;
+      for (int l = c.used - 1; l >= 0; --l) {
+        const uint64_t w = c.dp[l];
+        // acc c64 < 2^56, hi c32 < 2^60, lo < 2^32: the sum stays below 2^63
+        acc = mod_small((uint64_t)acc * c64 + (uint64_t)(uint32_t)(w >> 32) * c32 + (uint32_t)w, k);
+      }
+      r = acc;
+    }
+  }
+  out[(size_t)i * spacing + j] = r;
+}
+
+// Coefficients [LW][ncoef] (limb-major) into BnT (callICRT's output,
+// src/cuda_bn.cu:1004-1045): the words that fit dp (alloc), used = the
+// significant words, sign positive.
+static __global__ void k_to_bn(const uint64_t* V, int lw, int ncoef, BnT* C) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncoef) return;
+  BnT c = C[j];
+  const int w = c.alloc < lw ? c.alloc : lw;
+  int used = 0;
+  for (int l = 0; l < w; ++l) {
+    const uint64_t v = V[(size_t)l * ncoef + j];
+    c.dp[l] = v;
+    if (v) used = l + 1;
+  }
+  C[j].used = used;
+  C[j].sign = 0;
+}
+
 // y_i = r_i * (M/p_i)^-1 mod p_i (the cuPreICRT inner step, src/cuda_bn.cu:875-879).
 static __global__ void k_residue_to_y(RingDev rd, const uint64_t* res, int spacing, uint32_t* Y, int set) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1554,14 +1610,17 @@ __global__ void k_scalar_q(RingDev rd, const uint64_t* A, uint64_t* C, int n, si
 }
 
 // Residues [R][spacing] (modulus i in row i, values in [0, m_i)): the same
-// operations mod each CRT modulus, with the scalar's residue s_i = sa.v[i].
-// c may alias a.
-static __global__ void k_residue_scalar(RingDev rd, uint64_t* C, const uint64_t* A, int spacing, int op, ScalarArg sa) {
+// operations mod each CRT modulus, with the scalar's residue
+// s_i = S[i] mod m_i (S: device array of R words, the reference's
+// integer_array). c may alias a.
+static __global__ void k_residue_scalar(RingDev rd, uint64_t* C, const uint64_t* A, int spacing, int op,
+                                        const uint64_t* S) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= spacing) return;
   const size_t o = (size_t)i * spacing + j;
-  const uint64_t a = A[o], s = sa.v[i];
+  const uint64_t a = A[o];
+  const uint64_t s = i == 0 ? (S[0] >= gl::P ? S[0] - gl::P : S[0]) : mod_p(S[i], rd.primes[0][i], rd.barrett[0][i]);
   uint64_t r = a;
   if (i == 0) {  // modulus P itself
     if (op == SC_MUL) r = gl::mul(a, s);

@@ -61,7 +61,10 @@ hipError_t residue_to_y(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd,
 hipError_t residue_addsub(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, uint64_t* C, const uint64_t* A,
                           const uint64_t* B, int spacing, int sub);
 hipError_t residue_scalar(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, uint64_t* C, const uint64_t* A,
-                          int spacing, int op, const yhk::ScalarArg& sa);
+                          int spacing, int op, const uint64_t* S);
+hipError_t crt_bn(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const yhk::BnT* C, int ncoef, uint64_t* out,
+                  int spacing);
+hipError_t to_bn(dim3 grid, dim3 block, hipStream_t s, const uint64_t* V, int lw, int ncoef, yhk::BnT* C);
 hipError_t pointwise(dim3 grid, dim3 block, hipStream_t s, uint64_t* A, const uint64_t* B, size_t count);
 hipError_t take_even(dim3 grid, dim3 block, hipStream_t s, const uint64_t* in, uint64_t* out, int m, int LQ);
 hipError_t spread_even(dim3 grid, dim3 block, hipStream_t s, const uint64_t* in, uint64_t* out, int m, int LQ);

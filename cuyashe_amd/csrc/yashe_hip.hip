@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -752,15 +753,26 @@ int yh_crt(yh_ring* r, const uint64_t* d_coefs, int limbs, int ncoef, int spacin
   return YH_OK;
 }
 
-int yh_icrt(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, uint64_t* d_coefs, int out_limbs) {
-  if (!r || !d_res || !d_coefs) return fail(YH_ERR_ARG, "null argument");
+// ICRT of residues into d_coefs, or (d_coefs null) into the ring's staging
+// buffer d_tmp.
+static int icrt_impl(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, uint64_t* d_coefs, int out_limbs) {
+  if (!r || !d_res) return fail(YH_ERR_ARG, "null argument");
   if (ncoef <= 0 || ncoef > spacing || out_limbs < 1) return fail(YH_ERR_ARG, "bad sizes");
   if (int s = ring_enter(r)) return s;
   Scratch sc;
   if (int st = ring_y(r, (size_t)yhk::y_rows(r->rd.R) * spacing * sizeof(uint32_t), &sc)) return st;
+  if (!d_coefs) {
+    if (int st = ensure(r, (void**)&r->d_tmp, &r->tmp_bytes, (size_t)out_limbs * ncoef * sizeof(uint64_t))) return st;
+    d_coefs = r->d_tmp;
+  }
   dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
   YH_LAUNCH(yhl::residue_to_y(grid, block, r->stream, r->rd, d_res, spacing, sc.y, 0));
   return launch_icrt(r, icrt_args(sc.y, spacing, ncoef, 0, yhk::EPI_RAW, d_coefs, out_limbs, ncoef, 0, 1), 1);
+}
+
+int yh_icrt(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, uint64_t* d_coefs, int out_limbs) {
+  if (!d_coefs) return fail(YH_ERR_ARG, "null argument");
+  return icrt_impl(r, d_res, spacing, ncoef, d_coefs, out_limbs);
 }
 
 int yh_ntt(yh_ctx* ctx, uint64_t* d_data, int N, int count, int direction) {
@@ -884,20 +896,37 @@ int yh_ring_scalar(yh_ring* r, const uint64_t* d_a, const uint64_t* scalar, int 
   YH_TRY_END
 }
 
-int yh_residue_scalar(yh_ring* r, uint64_t* d_c, const uint64_t* d_a, int spacing, int op, const uint64_t* scalar,
-                      int scalar_limbs) {
-  if (!r || !d_c || !d_a || spacing <= 0) return fail(YH_ERR_ARG, "bad argument");
+int yh_residue_scalar(yh_ring* r, uint64_t* d_c, const uint64_t* d_a, int spacing, int op,
+                      const uint64_t* d_scalar_res) {
+  if (!r || !d_c || !d_a || !d_scalar_res || spacing <= 0) return fail(YH_ERR_ARG, "bad argument");
   if (op != YH_ADD && op != YH_SUB && op != YH_MUL) return fail(YH_ERR_ARG, "op must be YH_ADD, YH_SUB or YH_MUL");
-  if (r->rd.R > 64) return fail(YH_ERR_UNSUPPORTED, "more than 64 CRT moduli");
-  YH_TRY_BEGIN
-  yhk::ScalarArg sa{};
-  for (int i = 0; i < r->rd.R; ++i)
-    if (int st = scalar_mod(scalar, scalar_limbs, yh::BigInt(r->P->primes[i]), &sa.v[i], 1)) return st;
   if (int s = ring_enter(r)) return s;
   const dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
-  YH_LAUNCH(yhl::residue_scalar(grid, block, r->stream, r->rd, d_c, d_a, spacing, op, sa));
+  YH_LAUNCH(yhl::residue_scalar(grid, block, r->stream, r->rd, d_c, d_a, spacing, op, d_scalar_res));
   return YH_OK;
-  YH_TRY_END
+}
+
+static_assert(sizeof(yh_bn_t) == sizeof(yhk::BnT) && offsetof(yh_bn_t, dp) == offsetof(yhk::BnT, dp),
+              "yh_bn_t mirrors the device view");
+
+int yh_crt_bn(yh_ring* r, const yh_bn_t* d_coefs, int ncoef, int spacing, uint64_t* d_res) {
+  if (!r || !d_res || (ncoef > 0 && !d_coefs)) return fail(YH_ERR_ARG, "null argument");
+  if (ncoef < 0 || spacing < ncoef || spacing <= 0) return fail(YH_ERR_ARG, "bad spacing");
+  if (int s = ring_enter(r)) return s;
+  const dim3 block(256), grid((spacing + 255) / 256, r->rd.R);
+  YH_LAUNCH(yhl::crt_bn(grid, block, r->stream, r->rd, reinterpret_cast<const yhk::BnT*>(d_coefs), ncoef, d_res,
+                        spacing));
+  return YH_OK;
+}
+
+int yh_icrt_bn(yh_ring* r, const uint64_t* d_res, int spacing, int ncoef, yh_bn_t* d_coefs) {
+  if (!r || !d_res || !d_coefs) return fail(YH_ERR_ARG, "null argument");
+  if (ncoef <= 0 || ncoef > spacing) return fail(YH_ERR_ARG, "bad sizes");
+  const int LW = r->rd.LW;
+  if (int st = icrt_impl(r, d_res, spacing, ncoef, nullptr, LW)) return st;  // into the ring's staging buffer
+  YH_LAUNCH(yhl::to_bn(dim3((ncoef + 255) / 256), dim3(256), r->stream, r->d_tmp, LW, ncoef,
+                       reinterpret_cast<yhk::BnT*>(d_coefs)));
+  return YH_OK;
 }
 
 int yh_yashe_mul(yh_ring* r, const uint64_t* d_c1, const uint64_t* d_c2, uint64_t* d_out, int batch) {

@@ -143,12 +143,32 @@ int yh_residue_addsub(yh_ring* ring, uint64_t* d_c, const uint64_t* d_a, const u
 /* Scalar operation on residues [R][spacing] (values in [0, p_i)): op YH_ADD /
  * YH_SUB adds / subtracts the scalar's residue to coefficient 0 of every
  * residue row, YH_MUL multiplies every residue by it, all mod p_i. The
- * scalar is a host integer of scalar_limbs 64-bit limbs (little-endian, any
- * value; its residues are taken on the host). d_c may alias d_a; other
- * coefficients are copied. Replaces CUDAFunctions::callPolynomialOPInteger
+ * scalar is given by its residues: d_scalar_res is a DEVICE array of R
+ * words, word i taken mod p_i (the reference's integer_array, Integer's
+ * CRT residues). d_c may alias d_a; other coefficients are copied.
+ * Replaces CUDAFunctions::callPolynomialOPInteger
  * (src/cuda_functions.cu:477-548), which returns a freshly allocated buffer. */
-int yh_residue_scalar(yh_ring* ring, uint64_t* d_c, const uint64_t* d_a, int spacing, int op, const uint64_t* scalar,
-                      int scalar_limbs);
+int yh_residue_scalar(yh_ring* ring, uint64_t* d_c, const uint64_t* d_a, int spacing, int op,
+                      const uint64_t* d_scalar_res);
+/* The reference's multi-word integer, bn_t (src/cuda_bn.h:18-23), byte for
+ * byte: `used` 64-bit words at dp (device memory), least significant first,
+ * `alloc` words allocated there; sign is ignored (values are non-negative). */
+typedef struct yh_bn_t {
+  int alloc;
+  int used;
+  int sign;
+  uint64_t* dp;
+} yh_bn_t;
+/* yh_crt over the reference's own coefficient layout: d_coefs is a DEVICE
+ * array of ncoef yh_bn_t (any word count). Replaces callCRT
+ * (src/cuda_bn.cu:970-1002) without changing the caller's data. */
+int yh_crt_bn(yh_ring* ring, const yh_bn_t* d_coefs, int ncoef, int spacing, uint64_t* d_res);
+/* yh_icrt into the reference's layout: coefficient j (in [0, M)) is written
+ * to d_coefs[j].dp (the words that fit its alloc; alloc >= the ring's
+ * work_limbs holds every value), used = its significant words, sign = 0.
+ * d_coefs is a DEVICE array of ncoef yh_bn_t. Replaces callICRT
+ * (src/cuda_bn.cu:1004-1045). */
+int yh_icrt_bn(yh_ring* ring, const uint64_t* d_res, int spacing, int ncoef, yh_bn_t* d_coefs);
 
 /* ------------------------------------------------------------ fused operations
  * All take `batch` independent operands laid out back to back. */

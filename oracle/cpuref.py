@@ -195,14 +195,15 @@ def poly_mul_plain(a: Sequence[int], b: Sequence[int], N: int = None, qbits: int
 ADD, SUB, MUL = 0, 1, 2  # add_mode_t (src/settings.h:36)
 
 
-def residue_scalar(res: Sequence[Sequence[int]], primes: Sequence[int], op: int, s: int) -> List[List[int]]:
+def residue_scalar(res: Sequence[Sequence[int]], primes: Sequence[int], op: int, s) -> List[List[int]]:
     """polynomialOPInteger (src/cuda_functions.cu:477-548) on residues
     [R][spacing]: ADD / SUB the scalar's residue on coefficient 0 of each
-    row, MUL every residue, all mod p_i (the scalar's residues are those of
-    Integer::crt, src/integer.cpp:62-82)."""
+    row, MUL every residue, all mod p_i. s is the scalar (an int, reduced
+    mod each p_i as Integer::crt does, src/integer.cpp:62-82) or the list of
+    its residues, one per prime (the reference's integer_array)."""
     out = []
-    for row, p in zip(res, primes):
-        sp = int(s) % p
+    for i, (row, p) in enumerate(zip(res, primes)):
+        sp = (int(s[i]) if isinstance(s, (list, tuple)) else int(s)) % p
         r = [int(v) for v in row]
         if op == MUL:
             r = [v * sp % p for v in r]

@@ -66,12 +66,67 @@ def test_residue_scalar_matches_oracle(gpu_ctx, n, q, spacing):
     res = [row + [0] * (spacing - n) for row in res]
     d_res = gpu_ctx.to_device(np.array(res, dtype=np.uint64))
     out = gpu_ctx.alloc(len(primes) * spacing * 8)
+    mask = (1 << 64) - 1
     for op in (ya.YH_ADD, ya.YH_SUB, ya.YH_MUL):
         for s in (0, 3, rng.randrange(q), rng.randrange(1 << 300)):
-            ring.residue_scalar(out, d_res, spacing, op, s)
-            got = out.download((len(primes), spacing))
-            want = cpuref.residue_scalar(res, primes, op, s)
-            assert [[int(v) for v in row] for row in got] == want, (op, s)
+            # the scalar as Integer's residues, and as unreduced words (the
+            # kernel takes each word mod its prime, as the reference's % does)
+            for words in ([s % p for p in primes], [(s % p + p * rng.randrange(1 << 30)) & mask if i else s & mask
+                                                   for i, p in enumerate(primes)]):
+                d_s = gpu_ctx.to_device(np.array(words, dtype=np.uint64))
+                ring.residue_scalar(out, d_res, spacing, op, d_s)
+                got = out.download((len(primes), spacing))
+                want = cpuref.residue_scalar(res, primes, op, [w % p for w, p in zip(words, primes)])
+                assert [[int(v) for v in row] for row in got] == want, (op, s)
+    ring.close()
+
+
+BN_T = np.dtype([("alloc", np.int32), ("used", np.int32), ("sign", np.int32), ("pad", np.int32),
+                 ("dp", np.uint64)])
+
+
+def _bn_array(ctx, values, alloc):
+    """Device coefficients in the reference's bn_t layout: (headers, words)."""
+    n = len(values)
+    words = np.zeros((n, alloc), dtype=np.uint64)
+    for j, v in enumerate(values):
+        for l in range(alloc):
+            words[j, l] = (v >> (64 * l)) & ((1 << 64) - 1)
+    d_words = ctx.to_device(words)
+    hdr = np.zeros(n, dtype=BN_T)
+    hdr["alloc"] = alloc
+    hdr["used"] = [max(1, (int(v).bit_length() + 63) // 64) if v else 0 for v in values]
+    hdr["dp"] = [d_words.ptr + 8 * alloc * j for j in range(n)]
+    return ctx.to_device(hdr), d_words
+
+
+@pytest.mark.parametrize("n,q,ncoef", [(16, Q_POLY, 16), (1024, Q127, 1000), (256, Q511, 256)])
+def test_crt_icrt_bn_layout(gpu_ctx, n, q, ncoef):
+    # callCRT / callICRT over the reference's bn_t coefficients
+    # (src/cuda_bn.cu:970-1045): residues equal the oracle's CRT, and the
+    # ICRT writes the coefficients back with their used word counts
+    rng = random.Random(n + ncoef)
+    ring = ya.Ring(gpu_ctx, n, q)
+    primes = ring.primes()
+    M = ring.modulus()
+    alloc = 32  # STD_BNT_WORDS_ALLOC
+    vals = [rng.randrange(M) for _ in range(ncoef)]
+    vals[0], vals[1] = 0, M - 1
+    hdr, words = _bn_array(gpu_ctx, vals, alloc)
+    res = gpu_ctx.alloc(len(primes) * n * 8)
+    ring.crt_bn(hdr, ncoef, n, res)
+    got = res.download((len(primes), n))
+    want = cpuref.crt(vals, primes)
+    for i in range(len(primes)):
+        assert [int(v) for v in got[i][:ncoef]] == want[i] and not got[i][ncoef:].any(), i
+    out_hdr, out_words = _bn_array(gpu_ctx, [rng.randrange(1 << 64) for _ in range(ncoef)], alloc)
+    ring.icrt_bn(res, n, ncoef, out_hdr)
+    h = out_hdr.download((ncoef,), dtype=BN_T)
+    w = out_words.download((ncoef, alloc))
+    back = [sum(int(w[j, l]) << (64 * l) for l in range(int(h["used"][j]))) for j in range(ncoef)]
+    assert back == vals
+    assert all(int(h["used"][j]) == (vals[j].bit_length() + 63) // 64 for j in range(ncoef))
+    assert not h["sign"].any()
     ring.close()
 
 
