@@ -28,7 +28,9 @@
  *                 largest keeping the residue convolution below P (the
  *                 reference's 29-bit bucket is not exact for n > 64).
  * Composite operations (fold mod x^n + 1, mod q, scale-and-round, keyswitch
- * digits) live in oracle/cpuref.py on Python integers.
+ * digits) are composed in oracle/cpuref.py on Python integers, and again in C
+ * at the end of this file (or_yashe_mul, or_keyswitch) for the CPU baseline;
+ * the tests check the two against each other.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -77,9 +79,10 @@ static void stockham_stage(const uint64_t* W, int N, int Ns, const uint64_t* in,
  * inverse != 0: uses WInv and multiplies by N^-1 mod P. */
 /* Twiddle tables are cached per length (init in src/cuda_functions.cu:878-918
  * rebuilds them whenever N changes). */
-static int tw_N = 0;
-static uint64_t* tw_W = NULL;
-static uint64_t* tw_WInv = NULL;
+/* (per thread: the CPU baseline runs products on several threads) */
+static __thread int tw_N = 0;
+static __thread uint64_t* tw_W = NULL;
+static __thread uint64_t* tw_WInv = NULL;
 
 static int twiddles(int N) {
   if (tw_N == N) return 0;
@@ -526,5 +529,182 @@ int or_poly_inv(const uint64_t* f, int n, const uint64_t* q, int QL, uint64_t* o
     if (ds1 >= cap - 1 || ds0 >= cap - 1) break; /* cannot happen for n-degree moduli */
   }
   free(r0); free(r1); free(s0); free(s1); free(tmp);
+  return rc;
+}
+
+/* ----------------------------------------------------------------- composite operations
+ * The YASHE multiply and keyswitch composed in C (the same steps as
+ * oracle/cpuref.py's yashe_mul / keyswitch, which the tests pin), so a CPU
+ * baseline spends no time in an interpreter. Operands are limb-major
+ * [QL][n] with coefficients in [0, q). */
+
+/* Signed bignum: magnitude + sign. */
+typedef struct { int neg; bn m; } sbn;
+
+static void bn_from_col(bn* a, const uint64_t* limbs, int L, int n, int j) {
+  memset(a, 0, sizeof(*a));
+  for (int l = 0; l < L; ++l) a->d[l] = limbs[(size_t)l * n + j];
+  a->n = L;
+  bn_trim(a);
+}
+static void bn_to_col(const bn* a, uint64_t* limbs, int L, int n, int j) {
+  for (int l = 0; l < L; ++l) limbs[(size_t)l * n + j] = l < a->n ? a->d[l] : 0;
+}
+/* q, r with a = q m + r, 0 <= r < m (bit-serial long division). */
+static void bn_divmod(const bn* a, const bn* m, bn* q, bn* r) {
+  memset(q, 0, sizeof(*q));
+  bn_set_u64(r, 0);
+  const int nb = bn_bits(a);
+  for (int i = nb - 1; i >= 0; --i) {
+    bn_shl1(r);
+    if ((a->d[i / 64] >> (i % 64)) & 1) {
+      if (r->n == 0) r->n = 1;
+      r->d[0] |= 1;
+    }
+    if (bn_cmp(r, m) >= 0) {
+      bn_sub(r, m);
+      q->d[i / 64] |= 1ull << (i % 64);
+      if (q->n < i / 64 + 1) q->n = i / 64 + 1;
+    }
+  }
+  bn_trim(q);
+}
+/* a - b as a signed value (a, b >= 0). */
+static void sbn_sub(const bn* a, const bn* b, sbn* out) {
+  if (bn_cmp(a, b) >= 0) {
+    out->neg = 0;
+    out->m = *a;
+    bn_sub(&out->m, b);
+  } else {
+    out->neg = 1;
+    out->m = *b;
+    bn_sub(&out->m, a);
+  }
+}
+/* v mod q in [0, q) for a signed v. */
+static void sbn_mod(const sbn* v, const bn* q, bn* out) {
+  bn quo;
+  bn_divmod(&v->m, q, &quo, out);
+  if (v->neg && out->n) {
+    bn t = *q;
+    bn_sub(&t, out);
+    *out = t;
+  }
+}
+
+/* c = [round(t * c1*c2 / q)]_q, the product mod x^n + 1 over Z
+ * (Ciphertext::operator*, src/ciphertext.cpp:38-73, with the rounding of
+ * :55-63 applied to t*g: floor division, non-negative remainder, round up
+ * when 2*rem > q). Returns 0 on success. */
+int or_yashe_mul(const uint64_t* c1, const uint64_t* c2, int n, const uint64_t* q, int QL, uint64_t t, uint64_t* out) {
+  bn Q;
+  memset(&Q, 0, sizeof(Q));
+  memcpy(Q.d, q, sizeof(uint64_t) * QL);
+  Q.n = QL;
+  bn_trim(&Q);
+  uint64_t pr[256];
+  const int R = or_gen_primes(n, bn_bits(&Q), pr, 256);
+  if (R < 0) return -1;
+  bn M;
+  bn_set_u64(&M, 1);
+  for (int i = 0; i < R; ++i) bn_mul_small(&M, pr[i]);
+  const int LO = M.n, N = 2 * n;
+  uint64_t* prod = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)LO * N);
+  if (!prod) return -2;
+  int rc = or_poly_mul_plain(c1, n, c2, n, QL, N, pr, R, prod, LO);
+  for (int j = 0; j < n && rc == 0; ++j) {
+    bn lo, hi, x, quo, rem;
+    sbn g;
+    bn_from_col(&lo, prod, LO, N, j);
+    bn_from_col(&hi, prod, LO, N, j + n);
+    sbn_sub(&lo, &hi, &g); /* fold mod x^n + 1 (src/cuda_functions.cu:1066-1072) */
+    x = g.m;
+    bn_mul_small(&x, t);
+    bn_divmod(&x, &Q, &quo, &rem);
+    /* floor division of the signed t*g: quotient -(quo + [rem != 0]) and
+     * remainder q - rem when g < 0 and rem != 0 */
+    sbn v;
+    if (g.neg && rem.n) {
+      bn one;
+      bn_set_u64(&one, 1);
+      bn_add(&quo, &one);
+      bn r2 = Q;
+      bn_sub(&r2, &rem);
+      rem = r2;
+    }
+    v.neg = g.neg && quo.n;
+    v.m = quo;
+    bn twice = rem;
+    bn_shl1(&twice);
+    if (bn_cmp(&twice, &Q) > 0) { /* round up: v + 1 */
+      bn one;
+      bn_set_u64(&one, 1);
+      if (!v.neg) {
+        bn_add(&v.m, &one);
+      } else {
+        bn_sub(&v.m, &one);
+        if (!v.m.n) v.neg = 0;
+      }
+    }
+    bn r;
+    sbn_mod(&v, &Q, &r);
+    bn_to_col(&r, out, QL, n, j);
+  }
+  free(prod);
+  return rc;
+}
+
+/* c' = [sum_k D_k(c) gamma_k]_(x^n + 1, q) with D_k(c) = floor(c / w^k) mod w,
+ * w = 2^w_bits, for k < lw (Ciphertext::keyswitch / worddecomp,
+ * src/ciphertext.cpp:81-130 with the base, mask and shift the reference
+ * intends). gammas: [lw][QL][n]. Returns 0 on success. */
+int or_keyswitch(const uint64_t* c, const uint64_t* gammas, int n, const uint64_t* q, int QL, int w_bits, int lw,
+                 uint64_t* out) {
+  bn Q;
+  memset(&Q, 0, sizeof(Q));
+  memcpy(Q.d, q, sizeof(uint64_t) * QL);
+  Q.n = QL;
+  bn_trim(&Q);
+  const int qbits = bn_bits(&Q) > w_bits ? bn_bits(&Q) : w_bits;
+  uint64_t pr[256];
+  const int R = or_gen_primes(n, qbits, pr, 256);
+  if (R < 0) return -1;
+  bn M;
+  bn_set_u64(&M, 1);
+  for (int i = 0; i < R; ++i) bn_mul_small(&M, pr[i]);
+  const int LO = M.n, N = 2 * n;
+  uint64_t* prod = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)LO * N);
+  uint64_t* dig = (uint64_t*)calloc((size_t)QL * n, sizeof(uint64_t)); /* [QL][n], digits in limb 0 */
+  bn* acc = (bn*)calloc((size_t)N, sizeof(bn));
+  if (!prod || !dig || !acc) { free(prod); free(dig); free(acc); return -2; }
+  const uint64_t mask = w_bits >= 64 ? ~0ull : ((1ull << w_bits) - 1);
+  int rc = 0;
+  for (int k = 0; k < lw && rc == 0; ++k) {
+    int any = 0;
+    for (int j = 0; j < n; ++j) {
+      const int bit = k * w_bits, l = bit / 64, off = bit % 64;
+      uint64_t v = l < QL ? c[(size_t)l * n + j] >> off : 0;
+      if (off && off + w_bits > 64 && l + 1 < QL) v |= c[(size_t)(l + 1) * n + j] << (64 - off);
+      dig[j] = v & mask;
+      any |= dig[j] != 0;
+    }
+    if (!any) continue;
+    rc = or_poly_mul_plain(dig, n, gammas + (size_t)k * QL * n, n, QL, N, pr, R, prod, LO);
+    for (int j = 0; j < N && rc == 0; ++j) {
+      bn p;
+      bn_from_col(&p, prod, LO, N, j);
+      bn_add(&acc[j], &p);
+    }
+  }
+  for (int j = 0; j < n && rc == 0; ++j) {
+    sbn g;
+    bn r;
+    sbn_sub(&acc[j], &acc[j + n], &g);
+    sbn_mod(&g, &Q, &r);
+    bn_to_col(&r, out, QL, n, j);
+  }
+  free(prod);
+  free(dig);
+  free(acc);
   return rc;
 }

@@ -62,6 +62,10 @@ def lib():
                                         _u64p, ctypes.c_int, _u64p, ctypes.c_int]
         L.or_poly_inv.restype = ctypes.c_int
         L.or_poly_inv.argtypes = [_u64p, ctypes.c_int, _u64p, ctypes.c_int, _u64p]
+        L.or_yashe_mul.restype = ctypes.c_int
+        L.or_yashe_mul.argtypes = [_u64p, _u64p, ctypes.c_int, _u64p, ctypes.c_int, ctypes.c_uint64, _u64p]
+        L.or_keyswitch.restype = ctypes.c_int
+        L.or_keyswitch.argtypes = [_u64p, _u64p, ctypes.c_int, _u64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64p]
         _lib = L
     return _lib
 
@@ -309,3 +313,37 @@ def poly_inv(f: Sequence[int], n: int, q: int) -> List[int]:
     if rc:
         raise ValueError("not invertible")
     return [sum(int(out[j, l]) << (64 * l) for l in range(QL)) for j in range(n)]
+
+
+# ------------------------------------------------------------------ composite operations in C
+# The same compositions as yashe_mul / keyswitch above, run entirely in the C
+# restatement (or_yashe_mul / or_keyswitch) on limb-major arrays; the ctypes
+# calls release the interpreter lock, so threads run them in parallel.
+
+def _q_array(q: int):
+    QL = max(1, (q.bit_length() + 63) // 64)
+    return np.ascontiguousarray(np.array([(q >> (64 * l)) & ((1 << 64) - 1) for l in range(QL)], dtype=np.uint64)), QL
+
+
+def yashe_mul_limbs(c1: np.ndarray, c2: np.ndarray, n: int, q: int, t: int) -> np.ndarray:
+    """[QL][n] limbs of [round(t c1 c2 / q)]_q (c1, c2: [QL][n] limbs)."""
+    Q, QL = _q_array(q)
+    a = np.ascontiguousarray(c1, dtype=np.uint64)
+    b = np.ascontiguousarray(c2, dtype=np.uint64)
+    out = np.zeros((QL, n), dtype=np.uint64)
+    rc = lib().or_yashe_mul(_ptr(a), _ptr(b), n, _ptr(Q), QL, int(t), _ptr(out))
+    if rc:
+        raise ValueError(f"or_yashe_mul failed ({rc})")
+    return out
+
+
+def keyswitch_limbs(c: np.ndarray, gammas: np.ndarray, n: int, q: int, w_bits: int) -> np.ndarray:
+    """[QL][n] limbs of [sum_k D_k(c) gamma_k]_(x^n+1, q) (gammas: [lw][QL][n])."""
+    Q, QL = _q_array(q)
+    a = np.ascontiguousarray(c, dtype=np.uint64)
+    g = np.ascontiguousarray(gammas, dtype=np.uint64)
+    out = np.zeros((QL, n), dtype=np.uint64)
+    rc = lib().or_keyswitch(_ptr(a), _ptr(g), n, _ptr(Q), QL, int(w_bits), int(g.shape[0]), _ptr(out))
+    if rc:
+        raise ValueError(f"or_keyswitch failed ({rc})")
+    return out

@@ -7,6 +7,8 @@ rounding) and with planted mistakes that one of those checks must catch.
 """
 import random
 
+import numpy as np
+
 import pytest
 
 from oracle import cpuref
@@ -299,3 +301,40 @@ def test_ring_scalar_through_the_product():
         assert lhs == [(u + s * v) % q for u, v in zip(ab, b)]
         lhs = cpuref.ring_mul(cpuref.ring_scalar(a, cpuref.SUB, s, q), b, n, q)
         assert lhs == [(u - s * v) % q for u, v in zip(ab, b)]
+
+
+# ------------------------------------------------------------------ compositions in C
+
+@pytest.mark.parametrize("n,q,t,wb", [(16, (1 << 127) - 1, 1024, 32), (64, (1 << 255) - 19, 1 << 16, 32),
+                                      (32, (1 << 511) - 187, 1024, 16), (8, 77287149995192912462927307869, 35951, 32)])
+def test_c_compositions_equal_python(n, q, t, wb):
+    # or_yashe_mul / or_keyswitch (used by the CPU baseline) against the
+    # Python compositions pinned above, including all-(q-1) coefficients
+    rng = random.Random(n + wb)
+    QL = (q.bit_length() + 63) // 64
+    c1 = [rng.randrange(q) for _ in range(n)]
+    c2 = [q - 1] * n
+    lw = cpuref.num_digits(q, wb)
+    gam = [[rng.randrange(q) for _ in range(n)] for _ in range(lw)]
+    g = cpuref.yashe_mul(c1, c2, n, q, t)
+    gc = cpuref.yashe_mul_limbs(cpuref.to_limbs(c1, QL), cpuref.to_limbs(c2, QL), n, q, t)
+    assert cpuref.from_limbs(gc) == g
+    G = np.stack([cpuref.to_limbs(x, QL) for x in gam])
+    assert cpuref.from_limbs(cpuref.keyswitch_limbs(gc, G, n, q, wb)) == cpuref.keyswitch(g, gam, n, q, wb)
+
+
+def test_c_compositions_thread_safe():
+    # the transform tables are per thread: concurrent products of different
+    # lengths give the single-threaded results
+    import concurrent.futures
+    q = (1 << 127) - 1
+    rng = random.Random(5)
+    cases = []
+    for n in (16, 32, 64, 128, 256, 512) * 2:
+        c1 = cpuref.to_limbs([rng.randrange(q) for _ in range(n)], 2)
+        c2 = cpuref.to_limbs([rng.randrange(q) for _ in range(n)], 2)
+        cases.append((c1, c2, n))
+    want = [cpuref.yashe_mul_limbs(a, b, n, q, 1024) for a, b, n in cases]
+    with concurrent.futures.ThreadPoolExecutor(max_workers=6) as ex:
+        got = list(ex.map(lambda c: cpuref.yashe_mul_limbs(c[0], c[1], c[2], q, 1024), cases))
+    assert all(np.array_equal(g, w) for g, w in zip(got, want))
