@@ -141,6 +141,7 @@ static void bn_mul_small(bn* a, uint64_t m) {
     carry = c >> 64;
   }
   if (carry) a->d[a->n++] = (uint64_t)carry;
+  bn_trim(a); /* m = 0 leaves zero limbs */
 }
 static void bn_add(bn* a, const bn* b) {
   int n = a->n > b->n ? a->n : b->n;
@@ -170,7 +171,12 @@ static void bn_sub(bn* a, const bn* b) { /* a >= b */
   }
   bn_trim(a);
 }
-static int bn_bits(const bn* a) { return a->n ? 64 * (a->n - 1) + (64 - __builtin_clzll(a->d[a->n - 1])) : 0; }
+/* (skips zero top limbs: never counts the leading zeros of 0) */
+static int bn_bits(const bn* a) {
+  int n = a->n;
+  while (n > 0 && a->d[n - 1] == 0) --n;
+  return n ? 64 * (n - 1) + (64 - __builtin_clzll(a->d[n - 1])) : 0;
+}
 static void bn_shl1(bn* a) {
   uint64_t c = 0;
   for (int i = 0; i < a->n; ++i) {

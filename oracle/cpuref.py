@@ -29,7 +29,9 @@ from typing import List, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "libcpuref.so")
+# CUYASHE_ORACLE_LIB points at another build of the same C file (the
+# sanitizer test loads an -fsanitize=address,undefined build).
+LIB_PATH = os.environ.get("CUYASHE_ORACLE_LIB") or os.path.join(HERE, "build", "libcpuref.so")
 P = (1 << 64) - (1 << 32) + 1
 
 _lib = None
