@@ -1,7 +1,7 @@
 // Persistent batched ring products over uneven rings in one launch.
 //
 // A batch is a list of independent products c = a*b mod (x^n + 1, q) whose
-// rings differ (n from 16 to 8192, q of any supported width, so the prime
+// rings differ (n from 16 to 16384, q of any supported width, so the prime
 // count R differs too). The host splits every product into R tasks (one per
 // CRT prime) and packs tasks of equal n into bundles of W / (n/16) tasks, so
 // one workgroup of W threads runs a bundle as W / (n/16) side-by-side
@@ -61,9 +61,13 @@ struct BatchArgs {
   int nqueues;
 };
 
-constexpr int BATCH_MAX_THREADS = 512;
+// Workgroups of up to 512 threads run two or more per CU; a batch with an
+// n = 16384 ring runs 1024-thread workgroups (one length-16384 transform
+// each, 139 KB of LDS, one workgroup per CU).
+constexpr int BATCH_MAX_THREADS = 1024;
+constexpr int BATCH_TUNED_THREADS = 512;  // largest workgroup whose waves per SIMD are tunable
 constexpr int BATCH_MIN_LOGN = 4;
-constexpr int BATCH_MAX_LOGN = 13;
+constexpr int BATCH_MAX_LOGN = 14;
 
 // Dynamic LDS of a W-thread batch workgroup: W/(n/16) transforms of n
 // elements plus padding = 17 W words whatever n is.
@@ -162,12 +166,13 @@ __device__ __forceinline__ void batch_icrt(const RingDev& rd, const BatchInst& i
 // MAXLQ bounds the limb count of q in the batch: with the in-kernel ICRT for
 // LQ <= 4 the kernel needs ~150 VGPRs, so WAVES = 3 waves per SIMD fit
 // without spills and WAVES = 4 (128 VGPRs) spills a little; LQ up to 8 needs
-// the full 256 (WAVES = 2). WAVES is a tuning knob of the launch shape.
-template <int MAXLQ, int WAVES>
-__global__ void __launch_bounds__(BATCH_MAX_THREADS, WAVES) k_batch_prod(BatchArgs ba) {
+// the full 256 (WAVES = 2). WAVES is a tuning knob of the launch shape for
+// WT <= 512 threads per workgroup; WT = 1024 is always four waves per SIMD.
+template <int MAXLQ, int WAVES, int WT>
+__global__ void __launch_bounds__(WT, WAVES) k_batch_prod(BatchArgs ba) {
   extern __shared__ uint64_t lds[];
   __shared__ int s_b;
-  __shared__ int s_last[BATCH_MAX_THREADS];
+  __shared__ int s_last[WT];
   const int q0 = blockIdx.x % ba.nqueues;  // consecutive workgroups go to consecutive XCDs
   int qi = 0;
   while (true) {

@@ -11,20 +11,30 @@ hipError_t batch_reset(dim3 grid, dim3 block, hipStream_t s, int* head, int nq, 
 
 hipError_t batch_prod(int maxlq, int waves, dim3 grid, dim3 block, size_t lds, hipStream_t s,
                       const yhk::BatchArgs& args) {
-  if (maxlq > 4)
-    hipLaunchKernelGGL((yhk::k_batch_prod<8, 2>), grid, block, lds, s, args);
-  else if (waves >= 4)
-    hipLaunchKernelGGL((yhk::k_batch_prod<4, 4>), grid, block, lds, s, args);
-  else
-    hipLaunchKernelGGL((yhk::k_batch_prod<4, 3>), grid, block, lds, s, args);
+  if (block.x > (unsigned)yhk::BATCH_TUNED_THREADS) {
+    if (maxlq > 4)
+      hipLaunchKernelGGL((yhk::k_batch_prod<8, 4, yhk::BATCH_MAX_THREADS>), grid, block, lds, s, args);
+    else
+      hipLaunchKernelGGL((yhk::k_batch_prod<4, 4, yhk::BATCH_MAX_THREADS>), grid, block, lds, s, args);
+  } else if (maxlq > 4) {
+    hipLaunchKernelGGL((yhk::k_batch_prod<8, 2, yhk::BATCH_TUNED_THREADS>), grid, block, lds, s, args);
+  } else if (waves >= 4) {
+    hipLaunchKernelGGL((yhk::k_batch_prod<4, 4, yhk::BATCH_TUNED_THREADS>), grid, block, lds, s, args);
+  } else {
+    hipLaunchKernelGGL((yhk::k_batch_prod<4, 3, yhk::BATCH_TUNED_THREADS>), grid, block, lds, s, args);
+  }
   return hipGetLastError();
 }
 
 void batch_attrs() {
-  for (const void* f : {(const void*)yhk::k_batch_prod<4, 3>, (const void*)yhk::k_batch_prod<4, 4>,
-                        (const void*)yhk::k_batch_prod<8, 2>})
-    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)yhk::batch_lds_bytes(yhk::BATCH_MAX_THREADS));
+  using namespace yhk;
+  for (const void* f : {(const void*)k_batch_prod<4, 3, BATCH_TUNED_THREADS>,
+                        (const void*)k_batch_prod<4, 4, BATCH_TUNED_THREADS>,
+                        (const void*)k_batch_prod<8, 2, BATCH_TUNED_THREADS>})
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)batch_lds_bytes(BATCH_TUNED_THREADS));
+  for (const void* f : {(const void*)k_batch_prod<4, 4, BATCH_MAX_THREADS>,
+                        (const void*)k_batch_prod<8, 4, BATCH_MAX_THREADS>})
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)batch_lds_bytes(BATCH_MAX_THREADS));
 }
 
 }  // namespace yhl

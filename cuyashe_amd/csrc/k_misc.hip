@@ -69,6 +69,12 @@ hipError_t spread_even(dim3 grid, dim3 block, hipStream_t s, const uint64_t* in,
   return hipGetLastError();
 }
 
+hipError_t join_halves(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const uint32_t* Yu,
+                        const uint32_t* Yv, uint32_t* Yc, int h, int N) {
+  hipLaunchKernelGGL(yhk::k_join_halves, grid, block, 0, s, rd, Yu, Yv, Yc, h, N);
+  return hipGetLastError();
+}
+
 template <int L>
 static void ntt_attr() {
   (void)hipFuncSetAttribute((const void*)yhk::k_ntt_cyclic<L>, hipFuncAttributeMaxDynamicSharedMemorySize,

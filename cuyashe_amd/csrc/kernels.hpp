@@ -1704,6 +1704,54 @@ static __global__ void k_take_even(const uint64_t* in, uint64_t* out, int m, int
   for (int l = 0; l < LQ; ++l) out[(size_t)l * h + j] = in[(size_t)l * m + 2 * j];
 }
 
+// x / 2 mod an odd modulus m for x in [0, m): x/2, or (x + m)/2 when x is odd
+__device__ __forceinline__ uint64_t halve_mod(uint64_t x, uint64_t m) { return (x >> 1) + ((x & 1) ? (m >> 1) + 1 : 0); }
+
+// Linear product c = ab of operands of at most h coefficients, assembled in
+// the reconstruction rows from its cyclic half u = ab mod (x^h - 1) and its
+// negacyclic half v = ab mod (x^h + 1): for j < h, c_j = (u_j + v_j)/2 and
+// c_{j+h} = (u_j - v_j)/2 modulo every CRT modulus (each y row is a fixed
+// multiple of the residue, so the halves combine row by row), and c_j = 0 for
+// j >= 2h. Without v (Yv null) the product ends below h: c_j = u_j for j < h.
+// Rows of Yu and Yv hold h values, rows of Yc N; grid.y runs over the moduli.
+static __global__ void k_join_halves(RingDev rd, const uint32_t* Yu, const uint32_t* Yv, uint32_t* Yc, int h, int N) {
+  const int i = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const bool low = j < h;
+  const int k = low ? j : j - h;
+  const bool zero = j >= 2 * h || (!Yv && !low);
+  if (i == 0) {
+    uint64_t c = 0;
+    if (!zero) {
+      const uint64_t u = Yu[k] | (uint64_t)Yu[h + k] << 32;
+      if (Yv) {
+        const uint64_t v = Yv[k] | (uint64_t)Yv[h + k] << 32;
+        c = halve_mod(low ? gl::add(u, v) : gl::sub(u, v), gl::P);
+      } else {
+        c = u;
+      }
+    }
+    Yc[j] = (uint32_t)c;
+    Yc[(size_t)N + j] = (uint32_t)(c >> 32);
+  } else {
+    const uint32_t p = rd.sp[0][i].p;
+    uint32_t c = 0;
+    if (!zero) {
+      const uint32_t u = Yu[(size_t)(i + 1) * h + k];
+      if (Yv) {
+        const uint32_t v = Yv[(size_t)(i + 1) * h + k];
+        uint32_t s = low ? u + v : u + (p - v);  // < 2p < 2^29
+        if (s >= p) s -= p;
+        c = (uint32_t)halve_mod(s, p);
+      } else {
+        c = u;
+      }
+    }
+    Yc[(size_t)(i + 1) * N + j] = c;
+  }
+}
+
 // out_{2j} = in_j, out_{2j+1} = 0, j < m/2 (p(y) -> p(x^2))
 static __global__ void k_spread_even(const uint64_t* in, uint64_t* out, int m, int LQ) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;

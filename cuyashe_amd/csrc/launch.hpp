@@ -66,6 +66,9 @@ hipError_t crt_bn(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const
                   int spacing);
 hipError_t to_bn(dim3 grid, dim3 block, hipStream_t s, const uint64_t* V, int lw, int ncoef, yhk::BnT* C);
 hipError_t pointwise(dim3 grid, dim3 block, hipStream_t s, uint64_t* A, const uint64_t* B, size_t count);
+// linear product rows from its cyclic and negacyclic halves of length h
+hipError_t join_halves(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const uint32_t* Yu,
+                       const uint32_t* Yv, uint32_t* Yc, int h, int N);
 hipError_t take_even(dim3 grid, dim3 block, hipStream_t s, const uint64_t* in, uint64_t* out, int m, int LQ);
 hipError_t spread_even(dim3 grid, dim3 block, hipStream_t s, const uint64_t* in, uint64_t* out, int m, int LQ);
 void misc_attrs();

@@ -47,6 +47,11 @@ int fail(int code, const std::string& msg) {
     }                                                                                       \
   } while (0)
 
+// Longest LDS-resident transform, and longest plain product (built from two
+// transforms of the longest length and zero coefficients above them).
+constexpr int YH_TRANSFORM_MAX = 16384;
+constexpr long YH_PLAIN_MAX_N = 1L << 20;
+
 #define YH_TRY_BEGIN try {
 #define YH_TRY_END                                                   \
   }                                                                  \
@@ -812,10 +817,47 @@ int yh_poly_mul_plain(yh_ring* r, const uint64_t* d_a, int deg_a, const uint64_t
   if (out_limbs < 1) return fail(YH_ERR_ARG, "out_limbs must be positive");
   if (min_N < 1 || (min_N & (min_N - 1))) return fail(YH_ERR_ARG, "min_N must be a power of two");
   const long N = yh_poly_mul_plain_length(deg_a, deg_b, min_N);
-  if (N > 16384) return fail(YH_ERR_UNSUPPORTED, "plain product longer than 16384");
+  if (N > YH_PLAIN_MAX_N) return fail(YH_ERR_UNSUPPORTED, "plain product longer than 2^20");
   if (int s = ring_enter(r)) return s;
   const int R = r->rd.R, LQ = r->rd.LQ;
   Scratch sc;
+  if (N > YH_TRANSFORM_MAX) {
+    // Every operand has at most n <= h coefficients (h the longest transform)
+    // and deg a + deg b <= 2n - 2 < N, so the length-N product is the linear
+    // one. It is assembled from a cyclic and a negacyclic product of length h;
+    // the negacyclic one is needed only when the product reaches x^h, and then
+    // n = h and the ring's own transform tables are the length-h ones.
+    const int h = YH_TRANSFORM_MAX;
+    const bool wraps = deg_a + deg_b >= h;
+    if (wraps && r->rd.n != h) return fail(YH_ERR_ARG, "operand degrees must be in [0, n)");
+    const size_t rows = yhk::y_rows(R);
+    int st = ring_y(r, rows * (2 * (size_t)h + N) * sizeof(uint32_t), &sc);
+    if (st) return st;
+    uint32_t* Yu = sc.y;
+    uint32_t* Yv = Yu + rows * h;
+    uint32_t* Yc = Yv + rows * h;
+    DevTables tw{};
+    if (int s = get_cyclic(r->ctx, h, &tw)) return s;
+    yhk::ProdArgs pa{};
+    pa.A = d_a; pa.B = d_b;
+    pa.lin_a = pa.lin_b = LQ;
+    pa.ncoef_a = deg_a + 1; pa.ncoef_b = deg_b + 1;
+    pa.lstride_a = deg_a + 1; pa.lstride_b = deg_b + 1;
+    pa.logN = ilog2i(h);
+    pa.set = 0;
+    pa.Y = Yu;
+    pa.tab = tw.tab;
+    if ((st = launch_prod(r, pa, h, 1, false))) return st;
+    if (wraps) {
+      pa.Y = Yv;
+      pa.tab = r->rd.tab;
+      if ((st = launch_prod(r, pa, h, 1, true))) return st;
+    }
+    YH_LAUNCH(yhl::join_halves(dim3((unsigned)((N + 255) / 256), R), dim3(256), r->stream, r->rd, Yu,
+                               wraps ? Yv : nullptr, Yc, h, (int)N));
+    if (N_out) *N_out = (int)N;
+    return launch_icrt(r, icrt_args(Yc, (int)N, (int)N, 0, yhk::EPI_RAW, d_out, out_limbs, N, 0, 1), 1);
+  }
   int st = ring_y(r, (size_t)yhk::y_rows(R) * N * sizeof(uint32_t), &sc);
   if (st) return st;
   DevTables tw{};
@@ -1340,7 +1382,7 @@ int validate_items(const yh_batch_item* items, int count, int* max_T) {
     if (!it.ring || !it.a || !it.b || !it.out) return fail(YH_ERR_ARG, "null pointer in batch item");
     const int ln = it.ring->rd.logn;
     if (ln < yhk::BATCH_MIN_LOGN || ln > yhk::BATCH_MAX_LOGN)
-      return fail(YH_ERR_ARG, "batched products take 16 <= n <= 8192");
+      return fail(YH_ERR_ARG, "batched products take 16 <= n <= 16384");
     if (it.ring->ctx != items[0].ring->ctx) return fail(YH_ERR_ARG, "batch items from different contexts");
     mt = std::max(mt, (1 << ln) / 16);
   }
@@ -1351,10 +1393,12 @@ int validate_items(const yh_batch_item* items, int count, int* max_T) {
 int check_config(const yh_batch_config& c, int max_T) {
   const int W = c.wg_threads;
   if (W < 64 || W > yhk::BATCH_MAX_THREADS || (W & (W - 1)) || W < max_T)
-    return fail(YH_ERR_ARG, "wg_threads must be a power of two in [max(64, n/16), 512]");
+    return fail(YH_ERR_ARG, "wg_threads must be a power of two in [max(64, n/16), 1024]");
   if (c.wgs_per_cu < 1 || c.wgs_per_cu > 16) return fail(YH_ERR_ARG, "wgs_per_cu out of range");
   if (c.queues < 1 || c.queues > 64) return fail(YH_ERR_ARG, "queues out of range");
   if (c.waves < 2 || c.waves > 4) return fail(YH_ERR_ARG, "waves must be 2, 3 or 4");
+  if (W > yhk::BATCH_TUNED_THREADS && c.waves != 4)
+    return fail(YH_ERR_ARG, "1024-thread workgroups run four waves per SIMD (waves = 4)");
   return YH_OK;
 }
 
@@ -1429,11 +1473,11 @@ int yh_batch_default_config(const yh_batch_item* items, int count, yh_batch_conf
   int mt = 0;
   if (int s = validate_items(items, count, &mt)) return s;
   cfg->wg_threads = std::max(256, mt);
-  cfg->wgs_per_cu = 2;
+  cfg->wgs_per_cu = cfg->wg_threads > yhk::BATCH_TUNED_THREADS ? 1 : 2;
   cfg->queues = yhk::XCDS;
   int maxlq = 0;
   for (int k = 0; k < count; ++k) maxlq = std::max(maxlq, items[k].ring->rd.LQ);
-  cfg->waves = maxlq > 4 ? 2 : 4;
+  cfg->waves = maxlq > 4 && cfg->wg_threads <= yhk::BATCH_TUNED_THREADS ? 2 : 4;
   return YH_OK;
 }
 
@@ -1614,7 +1658,8 @@ int yh_batch_autotune(yh_ctx* ctx, const yh_batch_item* items, int count, const 
   std::vector<yh_batch_config> cand;
   for (int W = std::max(64, mt); W <= yhk::BATCH_MAX_THREADS; W *= 2)
     for (int waves : {2, 3, 4}) {
-      if ((maxlq > 4) != (waves == 2)) continue;  // the kernels that exist
+      // the kernels that exist
+      if (W > yhk::BATCH_TUNED_THREADS ? waves != 4 : (maxlq > 4) != (waves == 2)) continue;
       for (int wgs : {1, 2, 3, 4}) {
         if ((W / 64) * wgs > 4 * waves) continue;  // more workgroups than can be resident
         for (int nq : {1, yhk::XCDS}) cand.push_back({W, wgs, nq, waves});

@@ -178,7 +178,8 @@ int yh_icrt_bn(yh_ring* ring, const uint64_t* d_res, int spacing, int ncoef, yh_
  * power-of-two degree sums wrap cyclically exactly like the reference).
  * a: [q_limbs][deg_a+1], b: [q_limbs][deg_b+1], coefficients in [0, q),
  * deg_a, deg_b < n. out: [out_limbs][N] with the exact coefficients in
- * [0, M). *N_out receives N. Replaces Polynomial::operator*
+ * [0, M). *N_out receives N (at most 2^20; above 16384 the product cannot
+ * wrap and is the linear one, zero above degree deg_a + deg_b). Replaces Polynomial::operator*
  * (src/polynomial.cpp:52-60) -> callPolynomialMul
  * (src/cuda_functions.cu:617-738) -> icrt. */
 int yh_poly_mul_plain(yh_ring* ring, const uint64_t* d_a, int deg_a, const uint64_t* d_b, int deg_b,
@@ -303,7 +304,7 @@ int yh_graph_destroy(yh_graph* graph);
 
 /* ------------------------------------------------------------ uneven batches
  * Many independent ring products c_k = a_k * b_k mod (x^n_k + 1, q_k) over
- * rings of different n (16..8192) and q, in ONE persistent launch: every
+ * rings of different n (16..16384) and q, in ONE persistent launch: every
  * product becomes R_k per-prime tasks, packed into workgroup-sized bundles
  * and drained from per-XCD work queues; the task that finishes a product's
  * last prime runs its ICRT in the same launch. Operands and outputs are
@@ -318,10 +319,11 @@ typedef struct yh_batch_item {
   uint64_t* out;
 } yh_batch_item;
 
-/* Launch shape of a batch: threads per workgroup (64..512, a power of two,
+/* Launch shape of a batch: threads per workgroup (64..1024, a power of two,
  * at least the largest n/16), workgroups per CU in the persistent grid, the
  * number of work queues (1, or 8 = one per XCD) and the waves per SIMD the
- * kernel is compiled for (3 or 4 when every q has at most 4 limbs, else 2). */
+ * kernel is compiled for (3 or 4 when every q has at most 4 limbs, else 2;
+ * always 4 for 1024-thread workgroups, which a batch with n = 16384 needs). */
 typedef struct yh_batch_config {
   int wg_threads;
   int wgs_per_cu;

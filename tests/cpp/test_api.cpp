@@ -328,6 +328,33 @@ int main() {
       return true;
     });
   }
+  // ------------------------------------------------ operands of degree 16383
+  {
+    const ZZ q = to_ZZ("170141183460469231731687303715884105727");
+    Polynomial phi;
+    setup_ring(q, 16384, phi);
+    run("operator* of two degree-16383 polynomials (N = 32768, exact over Z)", [&] {
+      srand(4242);
+      Polynomial a, b;
+      Polynomial::random(&a, 16383);
+      Polynomial::random(&b, 16383);
+      a.set_coeff(16383, q - ZZ(1));
+      b.set_coeff(16383, q - ZZ(1));
+      const Polynomial c = a * b;
+      if (c.deg() != 32766) return false;
+      // evaluation at r modulo m is a ring map Z[x] -> Z_m, so an exact
+      // product satisfies c(r) = a(r) b(r) (mod m) for every r and m
+      auto eval = [](const Polynomial& p, const ZZ& r, const ZZ& m) {
+        ZZ v(0);
+        for (int k = p.deg(); k >= 0; --k) v = (v * r + p.get_coeff(k)) % m;
+        return v;
+      };
+      for (const ZZ m : {ZZ::pow2(61) - ZZ(1), to_ZZ("1000000000000000009")})
+        for (const ZZ r : {ZZ(3), to_ZZ("1234567891011"), m - ZZ(2)})
+          if (eval(c, r, m) != (eval(a, r, m) * eval(b, r, m)) % m) return false;
+      return true;
+    });
+  }
   std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
   return failures ? 1 : 0;
 }

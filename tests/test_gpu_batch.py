@@ -85,6 +85,39 @@ def test_batch_c3_equals_alone_and_oracle(gpu_ctx):
         r.close()
 
 
+def test_batch_with_n16384_rings(gpu_ctx):
+    """A mixed batch reaching n = 16384 (q 127-, 255- and 511-bit beside
+    smaller rings) runs in one launch of 1024-thread workgroups: every product
+    equals the product alone, and each n = 16384 ring matches the oracle."""
+    spec = [(16384, Q127), (16384, Q255), (16384, Q511), (8192, Q511), (4096, Q255), (2048, Q127),
+            (16384, Q127), (1024, Q511), (64, Q127), (16384, Q255)]
+    rings, bufs, items = make_batch(gpu_ctx, spec, 13)
+    b = ya.Batch(gpu_ctx, items)
+    b.run()
+    ref = alone(gpu_ctx, rings, bufs)
+    for key, (A, B, OUT, ha, hb, idx) in bufs.items():
+        got = OUT.download(ha.shape)
+        assert np.array_equal(got, ref[key]), key
+        n, q = key
+        if n == 16384:
+            want = cpuref.ring_mul(ya.array_to_ints(ha[0]), ya.array_to_ints(hb[0]), n, q)
+            assert ya.array_to_ints(got[0]) == want, key
+    # the same plan again, and a second shape of the wide workgroup
+    for key, (A, B, OUT, ha, hb, idx) in bufs.items():
+        ya.lib().yh_memset(gpu_ctx.h, OUT.ptr, 0, OUT.nbytes)
+    b.run()
+    for key, (A, B, OUT, ha, hb, idx) in bufs.items():
+        assert np.array_equal(OUT.download(ha.shape), ref[key]), key
+    b.close()
+    b = ya.Batch(gpu_ctx, items, {"wg_threads": 1024, "wgs_per_cu": 1, "queues": 1, "waves": 4})
+    b.run()
+    for key, (A, B, OUT, ha, hb, idx) in bufs.items():
+        assert np.array_equal(OUT.download(ha.shape), ref[key]), key
+    b.close()
+    for r in rings.values():
+        r.close()
+
+
 @pytest.mark.parametrize("cfg", [
     {"wg_threads": 512, "wgs_per_cu": 1, "queues": 1, "waves": 3},
     {"wg_threads": 512, "wgs_per_cu": 2, "queues": 8, "waves": 4},
@@ -113,8 +146,10 @@ def test_batch_rejects_bad_shapes(gpu_ctx):
         ya.Batch(gpu_ctx, items, {"wg_threads": 512, "wgs_per_cu": 1, "queues": 8, "waves": 5})
     r16k = ya.Ring(gpu_ctx, 16384, Q127)
     buf = gpu_ctx.alloc(r16k.L * 16384 * 8)
-    with pytest.raises(ya.YashError):
-        ya.Batch(gpu_ctx, [(r16k, buf, buf, buf)])
+    with pytest.raises(ya.YashError, match="wg_threads"):  # n/16 = 1024 > 512
+        ya.Batch(gpu_ctx, [(r16k, buf, buf, buf)], {"wg_threads": 512, "wgs_per_cu": 1, "queues": 8, "waves": 4})
+    with pytest.raises(ya.YashError, match="four waves"):
+        ya.Batch(gpu_ctx, [(r16k, buf, buf, buf)], {"wg_threads": 1024, "wgs_per_cu": 1, "queues": 8, "waves": 3})
     r16k.close()
     for r in rings.values():
         r.close()

@@ -80,3 +80,49 @@ def test_large_ring_mul_keyswitch(gpu_ctx, n, q):
     assert ring.unpack(out.download((batch, ring.L, n))) == want_ks
     evk.close()
     ring.close()
+
+
+# ------------------------------------------------------------------ plain products above the longest transform
+
+@pytest.mark.parametrize("q,da,db,min_N", [
+    (Q127, 16383, 16383, 1),        # N = 32768: both halves, the top of the linear product
+    (Q511, 16383, 9000, 1),         # N = 32768, uneven degrees, 511-bit q
+    (Q127, 16383, 16383, 65536),    # N = 65536: zero coefficients above 32766
+    (Q127, 5000, 6000, 32768),      # degree sum below 16384: the cyclic half alone
+    (Q127, 16383, 0, 32768),        # times a constant
+])
+def test_plain_longer_than_transform(gpu_ctx, q, da, db, min_N):
+    # Polynomial::operator* of two degree-16383 operands
+    # (src/cuda_functions.cu:617-738 at CUDAFunctions::N = 32768)
+    rng = random.Random(da * 7 + db + min_N)
+    ring = ya.Ring(gpu_ctx, 16384, q)
+    a, b = rand_poly(rng, da + 1, q), rand_poly(rng, db + 1, q)
+    a[-1], b[-1] = q - 1, q - 1  # full degree, largest coefficients
+    N = cpuref.ntt_length(da, db, min_N)
+    assert N > 16384
+    A = gpu_ctx.to_device(ya.ints_to_array(a, ring.L))
+    B = gpu_ctx.to_device(ya.ints_to_array(b, ring.L))
+    out = gpu_ctx.alloc(ring.LW * N * 8)
+    assert ring.poly_mul_plain(A, da, B, db, out, ring.LW, min_N) == N
+    got = ya.array_to_ints(out.download((ring.LW, N)))
+    assert got == cpuref.poly_mul_plain(a, b, N=N, qbits=q.bit_length())
+    assert not any(got[da + db + 1:])
+    ring.close()
+
+
+def test_plain_longer_than_transform_small_ring(gpu_ctx):
+    # a ring below the longest transform has no negacyclic half of that
+    # length: its operands (degree < n) never reach x^16384
+    rng = random.Random(5)
+    q = Q127
+    ring = ya.Ring(gpu_ctx, 4096, q)
+    a, b = rand_poly(rng, 4096, q), rand_poly(rng, 4096, q)
+    A = gpu_ctx.to_device(ya.ints_to_array(a, ring.L))
+    B = gpu_ctx.to_device(ya.ints_to_array(b, ring.L))
+    out = gpu_ctx.alloc(ring.LW * 32768 * 8)
+    assert ring.poly_mul_plain(A, 4095, B, 4095, out, ring.LW, 32768) == 32768
+    got = ya.array_to_ints(out.download((ring.LW, 32768)))
+    assert got == cpuref.poly_mul_plain(a, b, N=32768, qbits=q.bit_length())
+    with pytest.raises(ya.YashError, match="2\\^20"):
+        ring.poly_mul_plain(A, 4095, B, 4095, out, ring.LW, 1 << 21)
+    ring.close()
