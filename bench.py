@@ -151,26 +151,27 @@ def bench_uneven(ya, ctx, reps=10):
             "config": cfg, "rings": hist, "identical_to_per_ring": exact}
 
 
-def bench_serving(ya, ctx, qps_list, count):
+def bench_serving(ya, ctx, points):
     """Continuous batching over 12 resident rings (n in {2048..16384}, q
     127/255/511-bit, t=1024): Poisson arrivals, 60% add / 30% mul+keyswitch /
     10% encrypt+decrypt, admission by HBM footprint, every result decrypted
-    and checked."""
+    and checked. points: (offered requests/s, request count) pairs."""
     from cuyashe_amd.serving import Server, c5_rings, poisson_requests
     rings = c5_rings()
     t_setup = time.perf_counter()
     srv = Server(ctx, rings, t=1024, pool=32, slots=64, max_batch=64, seed=11)
     setup_s = time.perf_counter() - t_setup
-    srv.run(poisson_requests(200, 20000.0, rings, 1024, 32, seed=99))  # warm-up: scratch and staging sized
-    points = []
-    for k, qps in enumerate(qps_list):
+    # warm-up: a burst sizes every ring's scratch and the loop's staging for full batches
+    srv.run(poisson_requests(3000, 1e6, rings, 1024, 32, seed=99))
+    out = []
+    for k, (qps, count) in enumerate(points):
         st = srv.run(poisson_requests(count, qps, rings, 1024, 32, seed=k))
         st["offered_qps"] = qps
-        points.append(st)
+        out.append(st)
     srv.close()
     return {"workload": "serving: 12 rings n in {2048,4096,8192,16384} x q in {2^127-1, 2^255-19, 2^511-187}, t=1024, "
                         "Poisson arrivals, 60% add / 30% mul+ks / 10% enc+dec",
-            "setup_s": round(setup_s, 2), "points": points}
+            "setup_s": round(setup_s, 2), "points": out}
 
 
 def bench_circuit(ya, ctx, reps=5):
@@ -315,7 +316,7 @@ def main():
     if args.only_uneven or args.only_circuit or args.serving:
         ctx = ya.Context(0)
         if args.serving:
-            out = bench_serving(ya, ctx, [float(v) for v in args.serving_qps.split(",")], args.serving_requests)
+            out = bench_serving(ya, ctx, [(float(v), args.serving_requests) for v in args.serving_qps.split(",")])
         else:
             out = bench_uneven(ya, ctx) if args.only_uneven else bench_circuit(ya, ctx)
         print(json.dumps(out))
@@ -503,7 +504,8 @@ def main():
     secondary = None
     if not args.no_secondary:
         secondary = {"uneven_products": bench_uneven(ya, ctx), "graph_circuit": bench_circuit(ya, ctx),
-                     "serving": bench_serving(ya, ctx, [10000.0], 1500)}
+                     # below saturation (latency) and far above it (the loop's ceiling)
+                     "serving": bench_serving(ya, ctx, [(1000.0, 1500), (1e6, 30000)])}
 
     res = {
         "metric": METRIC,

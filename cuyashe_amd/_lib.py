@@ -65,6 +65,29 @@ class BatchConfig(ctypes.Structure):
                 "waves": self.waves}
 
 
+class ServeRing(ctypes.Structure):
+    _fields_ = [("ring", ctypes.c_void_p), ("evk", ctypes.c_void_p), ("d_f", ctypes.c_void_p),
+                ("d_h", ctypes.c_void_p), ("d_pool", ctypes.c_void_p), ("pool", ctypes.c_int), ("slots", ctypes.c_int)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("groups", ctypes.c_long), ("mean_batch", ctypes.c_double),
+                ("peak_reserved", ctypes.c_size_t), ("max_inflight", ctypes.c_int)]
+
+
+class ServeRequest(ctypes.Structure):
+    """yh_request; REQUEST_DTYPE is the same layout as a numpy record."""
+    _fields_ = [("arrival", ctypes.c_double), ("rid", ctypes.c_uint64), ("kind", ctypes.c_int),
+                ("ring", ctypes.c_int), ("a", ctypes.c_int), ("b", ctypes.c_int), ("msg", ctypes.c_void_p),
+                ("result", ctypes.c_void_p), ("output", ctypes.c_void_p), ("done", ctypes.c_double)]
+
+
+REQUEST_DTYPE = np.dtype({"names": [f[0] for f in ServeRequest._fields_],
+                          "formats": ["<f8", "<u8", "<i4", "<i4", "<i4", "<i4", "<u8", "<u8", "<u8", "<f8"],
+                          "offsets": [getattr(ServeRequest, f[0]).offset for f in ServeRequest._fields_],
+                          "itemsize": ctypes.sizeof(ServeRequest)})
+YH_REQ_ADD, YH_REQ_MULKS, YH_REQ_ENCDEC = 0, 1, 2
+
 _lib = None
 
 # name -> (restype, argtypes)
@@ -147,6 +170,12 @@ _SIGS = {
     "yh_batch_destroy": (ctypes.c_int, [_vp]),
     "yh_batch_autotune": (ctypes.c_int, [_vp, ctypes.POINTER(BatchItem), ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                          ctypes.POINTER(BatchConfig)]),
+    "yh_server_create": (ctypes.c_int, [_vp, ctypes.POINTER(ServeRing), ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "yh_server_set_budget": (ctypes.c_int, [_vp, ctypes.c_size_t]),
+    "yh_server_footprint": (ctypes.c_size_t, [_vp, ctypes.c_int, ctypes.c_int]),
+    "yh_server_run": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ServeStats)]),
+    "yh_server_destroy": (ctypes.c_int, [_vp]),
 }
 
 

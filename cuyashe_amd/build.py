@@ -34,7 +34,7 @@ def _run(cmd):
 # Translation units of the library: the C ABI's host logic and one per kernel
 # family (compiled in parallel, then linked; each object carries its own
 # gfx950 code object).
-HIP_SRCS = ["yashe_hip.hip", "k_prod.hip", "k_keyswitch.hip", "k_recon.hip", "k_misc.hip", "k_batch.hip"]
+HIP_SRCS = ["yashe_hip.hip", "k_prod.hip", "k_keyswitch.hip", "k_recon.hip", "k_misc.hip", "k_batch.hip", "serve.hip"]
 OBJDIR = os.path.join(ROOT, "_build", "hip")
 
 

@@ -17,6 +17,7 @@
 #include "../../include/yashe_hip.h"
 #include "bigint.hpp"
 #include "batch.hpp"
+#include "internal.hpp"
 #include "kernels.hpp"
 #include "launch.hpp"
 #include "ring_params.hpp"
@@ -134,6 +135,7 @@ struct yh_ring {
   // cannot be destroyed
   int graph_users = 0;
   int plan_users = 0;
+  int server_users = 0;  // request loops (yh_server) serving this ring
 };
 
 struct yh_event {
@@ -146,6 +148,7 @@ struct yh_evk {
   uint64_t* d = nullptr;     // [lw][R_ks][n]: transform domain (n >= 16) or centred residues
   uint64_t* corr = nullptr;  // [R_ks][n]: correction of the centred digits (n >= 16)
   mutable int users = 0;     // captured graphs that read this key
+  mutable int server_users = 0;  // request loops that use this key
 };
 
 namespace {
@@ -494,7 +497,7 @@ int yh_ctx_destroy(yh_ctx* ctx) {
   if (!ctx) return YH_OK;
   if (ctx->live > 0)
     return fail(YH_ERR_ARG, "context still has " + std::to_string(ctx->live) +
-                                " ring(s), batch plan(s) or graph(s); destroy them first");
+                                " ring(s), batch plan(s), graph(s) or request loop(s); destroy them first");
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   drain_timing(ctx);
@@ -674,9 +677,10 @@ int yh_ring_create(yh_ctx* ctx, int n, const uint64_t* q, int q_limbs, uint64_t 
 
 int yh_ring_destroy(yh_ring* r) {
   if (!r) return YH_OK;
-  if (r->graph_users + r->plan_users > 0)
-    return fail(YH_ERR_ARG, "ring is referenced by " + std::to_string(r->graph_users) + " captured graph(s) and " +
-                                std::to_string(r->plan_users) + " batch plan(s); destroy them first");
+  if (r->graph_users + r->plan_users + r->server_users > 0)
+    return fail(YH_ERR_ARG, "ring is referenced by " + std::to_string(r->graph_users) + " captured graph(s), " +
+                                std::to_string(r->plan_users) + " batch plan(s) and " +
+                                std::to_string(r->server_users) + " request loop(s); destroy them first");
   (void)hipSetDevice(r->ctx->device);
   (void)hipStreamSynchronize(r->stream);
   for (auto& kv : r->sub) yh_ring_destroy(kv.second);
@@ -1009,9 +1013,9 @@ int yh_evk_create(yh_ring* r, const uint64_t* d_gamma, yh_evk** out) {
 
 int yh_evk_destroy(yh_evk* k) {
   if (!k) return YH_OK;
-  if (k->users > 0)
-    return fail(YH_ERR_ARG, "keyswitch key is referenced by " + std::to_string(k->users) +
-                                " captured graph(s); destroy them first");
+  if (k->users + k->server_users > 0)
+    return fail(YH_ERR_ARG, "keyswitch key is referenced by " + std::to_string(k->users) + " captured graph(s) and " +
+                                std::to_string(k->server_users) + " request loop(s); destroy them first");
   (void)hipSetDevice(k->ring->ctx->device);
   (void)hipStreamSynchronize(k->ring->stream);
   (void)hipFree(k->d);
@@ -1766,3 +1770,20 @@ size_t yh_mem_footprint(const yh_ring* r, int op) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- internals for other translation units
+namespace yhi {
+int fail(int code, const std::string& msg) { return ::fail(code, msg); }
+int ctx_device(const yh_ctx* ctx) { return ctx->device; }
+hipStream_t ctx_stream(const yh_ctx* ctx) { return ctx->stream; }
+hipStream_t ring_stream(const yh_ring* ring) { return ring->stream; }
+const uint64_t* ring_q(const yh_ring* ring, int* limbs) {
+  *limbs = ring->P->LQ;
+  return ring->P->q_limbs.data();
+}
+yh_ctx* ring_ctx(const yh_ring* ring) { return ring->ctx; }
+const yh_ring* evk_ring(const yh_evk* evk) { return evk->ring; }
+void hold_ring(yh_ring* ring, int delta) { ring->server_users += delta; }
+void hold_evk(const yh_evk* evk, int delta) { evk->server_users += delta; }
+void hold_ctx(yh_ctx* ctx, int delta) { ctx->live += delta; }
+}  // namespace yhi

@@ -421,6 +421,76 @@ int yh_wire_peek(const void* in, size_t len, yh_wire_header* hdr);
  * included: the admission unit of a serving loop. */
 size_t yh_mem_footprint(const yh_ring* ring, int op);
 
+/* ------------------------------------------------------------ request loop
+ * Continuous batching of homomorphic requests over resident keys and
+ * ciphertexts of several rings. One host thread queues arrived requests per
+ * (ring, kind); a queue with fewer than `depth` groups in flight forms its
+ * next group from up to max_batch of its oldest requests (queues taken in
+ * the order of their oldest request) while the HBM reserved for work in
+ * flight (per request: yh_mem_footprint of its operation, or 3 ring elements
+ * for an encryption, plus its result element and its decryption) stays
+ * within hbm_budget and the ring has a free result slot, so batches grow
+ * with the load; it launches each group as one batched list operation on
+ * the ring's own stream; decrypts
+ * every result on the device and copies the decryption to the request's
+ * host buffer. Encryption noise (ternary s, D_{Z,8} on [-48, 48]) is drawn
+ * on the device from a counter-based generator keyed by (seed, rid,
+ * coefficient), so results do not depend on how requests were batched.
+ * Calls that overlap on one server are not allowed. No counterpart in the
+ * reference: it runs one Ciphertext operator at a time
+ * (src/ciphertext.cpp:38-107; Yashe::encrypt / decrypt, src/yashe.cpp:104-197),
+ * each followed by cudaDeviceSynchronize. */
+#define YH_REQ_ADD 0    /* c_a + c_b (Ciphertext::operator+) */
+#define YH_REQ_MULKS 1  /* c_a * c_b then keyswitch (Ciphertext::operator*, convert) */
+#define YH_REQ_ENCDEC 2 /* encrypt msg under the ring's public key h */
+
+typedef struct yh_serve_ring {
+  yh_ring* ring;          /* with t set; the server gives it its own stream */
+  const yh_evk* evk;      /* keyswitch key (NULL: no YH_REQ_MULKS on this ring) */
+  const uint64_t* d_f;    /* secret key f [L][n] (decryption) */
+  const uint64_t* d_h;    /* public key h [L][n] (encryption) */
+  const uint64_t* d_pool; /* resident ciphertexts [pool][L][n] */
+  int pool;
+  int slots;              /* result ciphertexts in flight at once */
+} yh_serve_ring;
+
+typedef struct yh_request {
+  double arrival;         /* seconds after the start of the run */
+  uint64_t rid;           /* request id: keys the encryption noise */
+  int kind;               /* YH_REQ_* */
+  int ring;               /* index into the server's rings */
+  int a, b;               /* pool indices (YH_REQ_ADD, YH_REQ_MULKS) */
+  const uint64_t* msg;    /* YH_REQ_ENCDEC: host message [n], values in [0, t) */
+  uint64_t* result;       /* host [n]: receives the decryption of the result */
+  uint64_t* output;       /* host [L][n] or NULL: receives the result ciphertext */
+  double done;            /* out: seconds (device clock, same origin) when the result was complete */
+} yh_request;
+
+typedef struct yh_serve_stats {
+  double wall_s;          /* from the start until the last result was on the host */
+  long groups;            /* batched launches */
+  double mean_batch;      /* requests per launch */
+  size_t peak_reserved;   /* most HBM reserved at once */
+  int max_inflight;       /* most groups in flight at once */
+} yh_serve_stats;
+
+typedef struct yh_server yh_server;
+/* rings: nrings descriptors (copied; the rings and keys are held until
+ * yh_server_destroy, and refuse destruction meanwhile). max_batch: most
+ * requests per launch, 1..4096; depth: groups in flight per (ring, kind),
+ * 1..64. */
+int yh_server_create(yh_ctx* ctx, const yh_serve_ring* rings, int nrings, size_t hbm_budget, int max_batch,
+                     int depth, uint64_t seed, yh_server** out);
+int yh_server_set_budget(yh_server* server, size_t hbm_budget);
+/* Admission footprint of one request of `kind` on ring index `ring` (bytes). */
+size_t yh_server_footprint(const yh_server* server, int ring, int kind);
+/* Serves `count` requests (any order; arrivals relative to this call) and
+ * returns when every result is on the host. Refuses (YH_ERR_ARG, nothing
+ * run) a request with a bad kind, ring or pool index, a message value >= t,
+ * or a footprint above the whole budget. */
+int yh_server_run(yh_server* server, yh_request* reqs, int count, yh_serve_stats* stats);
+int yh_server_destroy(yh_server* server);
+
 #ifdef __cplusplus
 }
 #endif

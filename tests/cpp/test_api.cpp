@@ -349,8 +349,8 @@ int main() {
         for (int k = p.deg(); k >= 0; --k) v = (v * r + p.get_coeff(k)) % m;
         return v;
       };
-      for (const ZZ m : {ZZ::pow2(61) - ZZ(1), to_ZZ("1000000000000000009")})
-        for (const ZZ r : {ZZ(3), to_ZZ("1234567891011"), m - ZZ(2)})
+      for (const ZZ& m : {ZZ::pow2(61) - ZZ(1), to_ZZ("1000000000000000009")})
+        for (const ZZ& r : {ZZ(3), to_ZZ("1234567891011"), m - ZZ(2)})
           if (eval(c, r, m) != (eval(a, r, m) * eval(b, r, m)) % m) return false;
       return true;
     });

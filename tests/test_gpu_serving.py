@@ -26,7 +26,24 @@ def test_serving_mix_all_decrypts(server):
     assert st["all_decrypts_ok"], [r.rid for r in reqs if not r.ok][:10]
     assert all(r.done is not None and r.latency > 0 for r in reqs)
     assert st["p99_ms"] >= st["p50_ms"] > 0
-    assert st["mean_batch"] > 1.0  # arrivals during earlier work were batched
+    assert st["mean_batch"] >= 1.0
+
+
+def test_serving_burst_is_batched(server):
+    # a burst: requests arriving while their (ring, kind) has groups in
+    # flight wait and go out together
+    reqs = poisson_requests(600, 1e6, RINGS, 1024, 16, seed=3)
+    st = server.run(reqs)
+    assert st["all_decrypts_ok"]
+    assert st["mean_batch"] > 2.0, st
+
+
+def test_serving_holds_rings_and_keys(server):
+    rs = next(iter(server.rings.values()))
+    with pytest.raises(ya.YashError, match="request loop"):
+        rs.ring.close()
+    with pytest.raises(ya.YashError, match="request loop"):
+        rs.Y.evk.close()
 
 
 def test_serving_alone_equals_mix(server):

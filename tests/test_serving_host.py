@@ -39,3 +39,37 @@ def test_poisson_requests_shape():
             assert r.msg.shape == (r.ring[0],) and r.msg.max() < 1024
         else:
             assert 0 <= r.a < 32 and 0 <= r.b < 32
+
+
+def test_fft_negacyclic_batched_rows():
+    # the checker runs one FFT over a stack of requests: rows are independent
+    rng = np.random.default_rng(1)
+    a = rng.integers(0, 1024, size=(5, 256))
+    b = rng.integers(0, 1024, size=(5, 256))
+    got = negacyclic_mul_mod_t(a, b, 1024)
+    for k in range(5):
+        assert np.array_equal(got[k], negacyclic_mul_mod(a[k], b[k], 1024))
+
+
+def test_request_record_matches_c_layout(tmp_path):
+    # the numpy record handed to yh_server_run has yh_request's layout
+    import shutil
+    import subprocess
+    from pathlib import Path
+    import pytest
+    from cuyashe_amd._lib import REQUEST_DTYPE
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    inc = Path(__file__).resolve().parents[1] / "include"
+    src = tmp_path / "layout.cpp"
+    fields = REQUEST_DTYPE.names
+    src.write_text('#include <cstdio>\n#include <cstddef>\n#include "yashe_hip.h"\nint main() {\n'
+                   '  std::printf("%zu", sizeof(yh_request));\n' +
+                   "".join(f'  std::printf(" %zu", offsetof(yh_request, {f}));\n' for f in fields) +
+                   '  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run([gxx, "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == REQUEST_DTYPE.itemsize
+    assert out[1:] == [REQUEST_DTYPE.fields[f][1] for f in fields]
