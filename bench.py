@@ -387,7 +387,9 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = B * args.steps / elapsed
     if args.profile_batched:
+        full = out.download((B, L, n))
         print(json.dumps({"profile": "batched mul+keyswitch", "batch": B, "steps": args.steps,
+                          "tiled_outputs_equal": bool(np.array_equal(full, full[np.arange(B) % uniq])),
                           "value": round(value, 1), "dominant": dom_name,
                           "dominant_avg_ms": dom_ms / max(dom_n, 1),
                           "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()}}))

@@ -377,45 +377,85 @@ struct ProdArgs {
 // at most eight coefficients' limbs are in registers at once (the compiler
 // would otherwise issue all loads up front and spill while the first
 // operand's transform is held in registers).
+#ifndef YH_LOAD_GROUP_B
+#define YH_LOAD_GROUP_B 4
+#endif
+template <int G = 8>
 __device__ __forceinline__ void group_fence(uint64_t* x) {
-  asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]), "+v"(x[6]),
-               "+v"(x[7])::"memory");
+  if constexpr (G == 4) {
+    asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3])::"memory");
+  } else {
+    asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]), "+v"(x[3]), "+v"(x[4]), "+v"(x[5]), "+v"(x[6]),
+                 "+v"(x[7])::"memory");
+  }
 }
 
 // MERGED: negacyclic input of ntt16_fwd<LOGN, true> (slot shifts only; the
 // twist table is not read).
-template <int LOGN, bool NEGA, bool MERGED = false>
+// Within a group the limbs are the outer loop, so the group's eight loads of
+// one limb are in flight together (crt_residue's steps, eight coefficients
+// side by side).
+template <int LOGN, bool NEGA, bool MERGED = false, int G = 8>
 __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                          const uint64_t* pw, const SmallPrime& k, const uint64_t* twist,
                                          int g = threadIdx.x) {
   constexpr int stride = 1 << (LOGN - 4);
 #pragma unroll
-  for (int h = 0; h < E; h += 8) {
+  for (int h = 0; h < E; h += G) {
+    uint64_t acc[G];
 #pragma unroll
-    for (int c = h; c < h + 8; ++c) {
+    for (int c = 0; c < G; ++c) acc[c] = 0;
+    for (int l = 0; l < lin; ++l) {
+      const uint32_t w0 = (uint32_t)pw[2 * l], w1 = (uint32_t)pw[2 * l + 1];
+      const uint64_t* al = a + (size_t)l * ls;
+#pragma unroll
+      for (int c = 0; c < G; ++c) {
+        const int j = g + (h + c) * stride;
+        const uint64_t v = j < ncoef ? al[j] : 0;
+        // each product < 2^60; reducing every 4 limbs keeps the sum < 2^63
+        acc[c] += (uint64_t)(uint32_t)v * w0 + (uint64_t)(uint32_t)(v >> 32) * w1;
+      }
+      if ((l & 3) == 3) {
+#pragma unroll
+        for (int c = 0; c < G; ++c) acc[c] = mod_small(acc[c], k);
+      }
+    }
+#pragma unroll
+    for (int c = h; c < h + G; ++c) {
       const int j = g + c * stride;
-      const uint32_t r = j < ncoef ? crt_residue(a + j, lin, ls, pw, k) : 0u;
+      const uint32_t r = j < ncoef ? mod_small(acc[c - h], k) : 0u;
       x[c] = MERGED ? twist_slot(r, k.p, c) : NEGA ? twist_centered(r, k.p, twist[j]) : center(r, k.p);
     }
-    if (h + 8 < E) group_fence(x + h);
+    if (h + G < E) group_fence<G>(x + h);
   }
 }
 
 // Modulus 0 (P): exact residues mod P, twisted by psi^j, not centred
 // (MERGED: by the slot factor 2^(78 c) only).
-template <int LOGN, bool NEGA, bool MERGED = false>
+template <int LOGN, bool NEGA, bool MERGED = false, int G = 8>
 __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                            const uint64_t* twist, int g) {
   constexpr int stride = 1 << (LOGN - 4);
 #pragma unroll
-  for (int h = 0; h < E; h += 8) {
+  for (int h = 0; h < E; h += G) {
+    // crt_residue_P's Horner steps, limbs outer
+    uint64_t r[G];
 #pragma unroll
-    for (int c = h; c < h + 8; ++c) {
-      const int j = g + c * stride;
-      const uint64_t r = j < ncoef ? crt_residue_P(a + j, lin, ls) : 0;
-      x[c] = MERGED ? gl::mul_pow2(r, (78 * c) % 192) : NEGA ? gl::mul(r, twist[j]) : r;
+    for (int c = 0; c < G; ++c) r[c] = 0;
+    for (int l = lin - 1; l >= 0; --l) {
+      const uint64_t* al = a + (size_t)l * ls;
+#pragma unroll
+      for (int c = 0; c < G; ++c) {
+        const int j = g + (h + c) * stride;
+        r[c] = gl::reduce128(r[c], j < ncoef ? al[j] : 0);
+      }
     }
-    if (h + 8 < E) group_fence(x + h);
+#pragma unroll
+    for (int c = h; c < h + G; ++c) {
+      const int j = g + c * stride;
+      x[c] = MERGED ? gl::mul_pow2(r[c - h], (78 * c) % 192) : NEGA ? gl::mul(r[c - h], twist[j]) : r[c - h];
+    }
+    if (h + G < E) group_fence<G>(x + h);
   }
 }
 
@@ -454,8 +494,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   const uint64_t* fwd02 = pa.tab.fwd0;
   asm volatile("" : "+s"(twist2), "+s"(fwd2), "+s"(fwd02));
   const int g2 = opaque_tid(xa[E - 1]);
-  if (gold) load_crt_P<LOGN, NEGA, M>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
-  else load_crt<LOGN, NEGA, M>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, sk, twist2, g2);
+  if (gold) load_crt_P<LOGN, NEGA, M, YH_LOAD_GROUP_B>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
+  else load_crt<LOGN, NEGA, M, YH_LOAD_GROUP_B>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, sk, twist2, g2);
   __syncthreads();  // every thread has left the LDS of the first transform
   ntt16_fwd<LOGN, M>(xb, lds, fwd2, g2, fwd02);
 #pragma unroll
