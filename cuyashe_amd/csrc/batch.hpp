@@ -82,6 +82,47 @@ static __global__ void k_batch_reset(int* head, int nq, int* done, int ninst) {
 // maxima): this wave's global loads and stores have completed.
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// CRT residues of a task's 16 coefficients j = g + c T (limb stride N),
+// twisted for the negacyclic transform: in groups of G coefficients, limbs
+// outer, so a group's loads of one limb are in flight together.
+template <bool GOLD, int G>
+__device__ __forceinline__ void batch_load(uint64_t* x, const uint64_t* a, int LQ, int N, int T, int g,
+                                           const uint64_t* pw, const SmallPrime& sk, const uint64_t* twist) {
+#pragma unroll
+  for (int h = 0; h < E; h += G) {
+    uint64_t acc[G];
+#pragma unroll
+    for (int c = 0; c < G; ++c) acc[c] = 0;
+    if (GOLD) {
+      for (int l = LQ - 1; l >= 0; --l) {
+        const uint64_t* al = a + (size_t)l * N + g;
+#pragma unroll
+        for (int c = 0; c < G; ++c) acc[c] = gl::reduce128(acc[c], al[(h + c) * T]);
+      }
+    } else {
+      for (int l = 0; l < LQ; ++l) {
+        const uint32_t w0 = (uint32_t)pw[2 * l], w1 = (uint32_t)pw[2 * l + 1];
+        const uint64_t* al = a + (size_t)l * N + g;
+#pragma unroll
+        for (int c = 0; c < G; ++c) {
+          const uint64_t v = al[(h + c) * T];
+          acc[c] += (uint64_t)(uint32_t)v * w0 + (uint64_t)(uint32_t)(v >> 32) * w1;
+        }
+        if ((l & 3) == 3) {
+#pragma unroll
+          for (int c = 0; c < G; ++c) acc[c] = mod_small(acc[c], sk);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      const int j = g + (h + c) * T;
+      x[h + c] = GOLD ? gl::mul(acc[c], twist[j]) : twist_centered(mod_small(acc[c], sk), sk.p, twist[j]);
+    }
+    if (h + G < E) group_fence<G>(x + h);
+  }
+}
+
 // One bundle of tasks of length-2^logn rings (runtime length: one copy of the
 // transform code serves every n). Sets s_last[sub] for the sub-groups whose
 // task finished the last prime of its product.
@@ -109,25 +150,11 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   uint64_t* my = lds + (size_t)sub * (N + T);
   uint64_t xa[E], xb[E];
   constexpr bool gold = GOLD;  // modulus 0 is P itself (exact residues, not centred)
-#pragma unroll
-  for (int c = 0; c < E; ++c) {
-    const int j = g + c * T;
-    xa[c] = gold ? gl::mul(crt_residue_P(in.A + j, LQ, N), tab.twist[j])
-                 : twist_centered(crt_residue(in.A + j, LQ, N, pw, sk), sk.p, tab.twist[j]);
-  }
+  batch_load<GOLD, 8>(xa, in.A, LQ, N, T, g, pw, sk, tab.twist);
   ntt16_fwd_rt(xa, my, tab.fwd, g, logn, wave);
-  // second operand in two groups of eight (group_fence): fewer loads in
-  // flight while the first operand's transform is held in registers
-#pragma unroll
-  for (int h = 0; h < E; h += 8) {
-#pragma unroll
-    for (int c = h; c < h + 8; ++c) {
-      const int j = g + c * T;
-      xb[c] = gold ? gl::mul(crt_residue_P(in.B + j, LQ, N), tab.twist[j])
-                   : twist_centered(crt_residue(in.B + j, LQ, N, pw, sk), sk.p, tab.twist[j]);
-    }
-    if (h + 8 < E) group_fence(xb + h);
-  }
+  // second operand in groups of four (group_fence): fewer loads in flight
+  // while the first operand's transform is held in registers
+  batch_load<GOLD, 4>(xb, in.B, LQ, N, T, g, pw, sk, tab.twist);
   xsync(wave);
   ntt16_fwd_rt(xb, my, tab.fwd, g, logn, wave);
 #pragma unroll
