@@ -52,6 +52,7 @@ def parse():
                     help="run only the uneven batched products (for rocprofv3 runs)")
     ap.add_argument("--only-circuit", action="store_true",
                     help="run only the graph-captured circuit (for rocprofv3 runs)")
+    ap.add_argument("--only-ntt", action="store_true", help="run only the yh_ntt primitive measurement")
     ap.add_argument("--serving", action="store_true",
                     help="run only the serving loop over a sweep of arrival rates")
     ap.add_argument("--serving-requests", type=int, default=2000)
@@ -100,6 +101,37 @@ def measured_traffic(kernel, batch):
     except (OSError, ValueError, KeyError):
         pass
     return None
+
+
+def bench_ntt(ya, ctx, N=4096, count=2048, reps=20):
+    """The drop-in transform primitive (yh_ntt, callNTT's replacement): cyclic
+    forward and inverse Goldilocks NTTs of `count` vectors of length N in
+    natural order, timed with events; algorithmic traffic 16 bytes per
+    element (one read, one write)."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    x = rng.integers(0, (1 << 63), size=(count, N), dtype=np.uint64)
+    buf = ctx.to_device(x)
+    e0, e1, e2 = ya.Event(ctx), ya.Event(ctx), ya.Event(ctx)
+    ctx.ntt(buf, N, count)
+    ctx.ntt(buf, N, count, inverse=True)
+    e0.record()
+    for _ in range(reps):
+        ctx.ntt(buf, N, count)
+    e1.record()
+    for _ in range(reps):
+        ctx.ntt(buf, N, count, inverse=True)
+    e2.record()
+    ctx.sync()
+    fwd = e0.elapsed_ms(e1) / reps
+    inv = e1.elapsed_ms(e2) / reps
+    round_trip = bool(np.array_equal(buf.download((count, N)), x))
+    gbytes = 16.0 * N * count / 1e9
+    return {"workload": f"cyclic NTT over P, {count} vectors of N={N}, natural order", "forward_ms": round(fwd, 4),
+            "inverse_ms": round(inv, 4), "forward_GBps": round(gbytes / (fwd * 1e-3), 1),
+            "inverse_GBps": round(gbytes / (inv * 1e-3), 1),
+            "butterflies_per_s": round(count * (N // 2) * (N.bit_length() - 1) / (fwd * 1e-3), 1),
+            "round_trip_exact": round_trip}
 
 
 def bench_uneven(ya, ctx, reps=10):
@@ -313,9 +345,11 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(ya)
         return
-    if args.only_uneven or args.only_circuit or args.serving:
+    if args.only_uneven or args.only_circuit or args.serving or args.only_ntt:
         ctx = ya.Context(0)
-        if args.serving:
+        if args.only_ntt:
+            out = bench_ntt(ya, ctx)
+        elif args.serving:
             out = bench_serving(ya, ctx, [(float(v), args.serving_requests) for v in args.serving_qps.split(",")])
         else:
             out = bench_uneven(ya, ctx) if args.only_uneven else bench_circuit(ya, ctx)
@@ -506,6 +540,7 @@ def main():
     secondary = None
     if not args.no_secondary:
         secondary = {"uneven_products": bench_uneven(ya, ctx), "graph_circuit": bench_circuit(ya, ctx),
+                     "ntt_primitive": bench_ntt(ya, ctx),
                      # below saturation (latency) and far above it (the loop's ceiling)
                      "serving": bench_serving(ya, ctx, [(1000.0, 1500), (1e6, 30000)])}
 

@@ -6,12 +6,12 @@ namespace yhl {
 
 #define YH_NTT_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14)
 
-hipError_t ntt_cyclic(int logN, int count, hipStream_t s, uint64_t* data, const uint64_t* T, const uint64_t* Ti,
-                      uint64_t Ninv, int inverse) {
+hipError_t ntt_cyclic(int logN, int count, hipStream_t s, uint64_t* data, const yhk::NttTab& tab, uint64_t Ninv,
+                      int inverse) {
   const dim3 grid(count), block((1 << logN) / 16);
   const size_t lds = lds_bytes(1 << logN);
   switch (logN) {
-#define X(L) case L: hipLaunchKernelGGL(yhk::k_ntt_cyclic<L>, grid, block, lds, s, data, T, Ti, Ninv, inverse); break;
+#define X(L) case L: hipLaunchKernelGGL(yhk::k_ntt_cyclic<L>, grid, block, lds, s, data, tab, Ninv, inverse); break;
     YH_NTT_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;

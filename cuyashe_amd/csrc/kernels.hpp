@@ -232,118 +232,6 @@ __device__ __forceinline__ void store_y(uint32_t* Yp, size_t N, int i, int j, ui
   }
 }
 
-// ---------------------------------------------------------------- NTT passes
-//
-// Transform of length N = 2^logN held by N/16 threads. A pass covers up to four
-// consecutive radix-2 stages; thread g holds the 16 elements
-//   j = base(g) + (c << b0),  c = 0..15,
-// base(g) = ((g >> b0) << (b0 + 4)) | (g & (2^b0 - 1)).
-// Forward (Cooley-Tukey, natural in -> bit-reversed out) stage s toggles bit
-// logN-1-s and uses T[2^s + (j >> (logN - s))]; a pass starting at stage s0
-// has b0 = logN - 4 - s0 (first pass short when logN % 4 != 0).
-// Inverse (Gentleman-Sande, bit-reversed in -> natural out) stage u toggles
-// bit u and uses Ti[(N >> (u+1)) + (j >> (u+1))]; passes start at u0 = 0, 4,
-// ... with b0 = min(u0, logN - 4) (last pass short).
-// LDS rows are padded by one element every 16 to spread the b0 = 0 pass over
-// banks.
-
-__device__ __forceinline__ int lds_idx(int j) { return j + (j >> 4); }
-__device__ __forceinline__ int pass_base(int g, int b0) {
-  return ((g >> b0) << (b0 + 4)) | (g & ((1 << b0) - 1));
-}
-
-__device__ __forceinline__ void lds_store(uint64_t* lds, const uint64_t* x, int base, int b0) {
-#pragma unroll
-  for (int c = 0; c < E; ++c) lds[lds_idx(base + (c << b0))] = x[c];
-}
-__device__ __forceinline__ void lds_load(const uint64_t* lds, uint64_t* x, int base, int b0) {
-#pragma unroll
-  for (int c = 0; c < E; ++c) x[c] = lds[lds_idx(base + (c << b0))];
-}
-
-// Forward transform, all pass/stage/bit positions fixed at compile time.
-// On entry x holds the first-pass layout (b0 = LOGN - 4: j = g + c*N/16); on
-// exit the last-pass layout (b0 = 0: j = 16*g + c), ready for a thread-local
-// pointwise step.
-template <int LOGN>
-__device__ __forceinline__ void ntt2_fwd(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ T) {
-  constexpr int FIRST = (LOGN & 3) ? (LOGN & 3) : 4;
-  constexpr int NPASS = 1 + (LOGN - FIRST) / 4;
-  const int g = threadIdx.x;
-#pragma unroll
-  for (int p = 0; p < NPASS; ++p) {
-    const int s0 = p == 0 ? 0 : FIRST + 4 * (p - 1);
-    const int ns = p == 0 ? FIRST : 4;
-    const int b0 = LOGN - 4 - s0;
-    const int base = pass_base(g, b0);
-    if (p > 0) {
-      lds_load(lds, x, base, b0);
-      __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k >= ns) continue;
-      const int s = s0 + k;
-      const int m = 1 << (3 - k);  // stage s toggles bit LOGN-1-s = b0 + 3 - k
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        if (c & m) continue;
-        const int j = base + (c << b0);
-        const uint64_t w = T[(1 << s) + (j >> (LOGN - s))];
-        const uint64_t U = x[c];
-        const uint64_t V = gl::mul(x[c | m], w);
-        x[c] = gl::add(U, V);
-        x[c | m] = gl::sub(U, V);
-      }
-    }
-    if (p < NPASS - 1) {
-      lds_store(lds, x, base, b0);
-      __syncthreads();
-    }
-  }
-}
-
-// Inverse transform (not scaled by N^-1). On entry the b0 = 0 layout (as
-// left by ntt_fwd); on exit the b0 = LOGN - 4 layout (j = g + c*N/16).
-template <int LOGN>
-__device__ __forceinline__ void ntt2_inv(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ Ti) {
-  constexpr int N = 1 << LOGN;
-  constexpr int NPASS = (LOGN + 3) / 4;
-  const int g = threadIdx.x;
-#pragma unroll
-  for (int p = 0; p < NPASS; ++p) {
-    const int u0 = 4 * p;
-    const int ns = LOGN - u0 < 4 ? LOGN - u0 : 4;
-    const int b0 = u0 < LOGN - 4 ? u0 : LOGN - 4;
-    const int off = u0 - b0;  // a short last pass works on the top bits of c
-    const int base = pass_base(g, b0);
-    if (p > 0) {
-      lds_load(lds, x, base, b0);
-      __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k >= ns) continue;
-      const int u = u0 + k;
-      const int m = 1 << (k + off);
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        if (c & m) continue;
-        const int j = base + (c << b0);
-        const uint64_t w = Ti[(N >> (u + 1)) + (j >> (u + 1))];
-        const uint64_t U = x[c];
-        const uint64_t V = x[c | m];
-        x[c] = gl::add(U, V);
-        x[c | m] = gl::mul(gl::sub(U, V), w);
-      }
-    }
-    if (p < NPASS - 1) {
-      lds_store(lds, x, base, b0);
-      __syncthreads();
-    }
-  }
-}
-
 // ---------------------------------------------------------------- fused products
 //
 // k_prod: one workgroup per (pair, prime). CRT both operands on load, forward
@@ -1501,12 +1389,15 @@ static __global__ void k_pointwise(uint64_t* A, const uint64_t* B, size_t count)
 }
 
 // Standalone cyclic transform over global data, natural order in and out
-// (callNTT, src/cuda_functions.cu:447-458, with N^-1 scaling on the inverse).
+// (callNTT, src/cuda_functions.cu:447-458, with N^-1 scaling on the inverse):
+// the radix-16 transforms of ntt16.hpp with the cyclic tables. The forward
+// transform leaves X_k at position bitrev(k) (position 16 g + c in thread g's
+// slot c) and the inverse takes that layout, so both reorder through the LDS.
 __device__ __forceinline__ int bitrev(int v, int bits) { return (int)(__brev((unsigned)v) >> (32 - bits)); }
 
 template <int LOGN>
-__global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ntt_cyclic(uint64_t* data, const uint64_t* T, const uint64_t* Ti,
-                                                                 uint64_t Ninv, int inverse) {
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ntt_cyclic(uint64_t* data, NttTab tab,
+                                                                              uint64_t Ninv, int inverse) {
   extern __shared__ uint64_t lds[];
   constexpr int N = 1 << LOGN;
   constexpr int stride = N >> 4;
@@ -1516,29 +1407,27 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ntt_cyclic(u
   if (!inverse) {
 #pragma unroll
     for (int c = 0; c < E; ++c) x[c] = d[g + c * stride];
-    ntt2_fwd<LOGN>(x, lds, T);
-    // bit-reversed -> natural
+    ntt16_fwd<LOGN>(x, lds, tab.fwd, g);
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < E; ++c) lds[lds_idx(g * E + c)] = x[c];
+    for (int c = 0; c < E; ++c) lds[lds_pad(g * E + c)] = x[c];
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int k = g + c * stride;
-      d[k] = lds[lds_idx(bitrev(k, LOGN))];
+      d[k] = lds[lds_pad(bitrev(k, LOGN))];
     }
   } else {
-    // natural -> bit-reversed into the b0 = 0 layout
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int k = g + c * stride;
-      lds[lds_idx(bitrev(k, LOGN))] = d[k];
+      lds[lds_pad(bitrev(k, LOGN))] = d[k];
     }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < E; ++c) x[c] = lds[lds_idx(g * E + c)];
+    for (int c = 0; c < E; ++c) x[c] = lds[lds_pad(g * E + c)];
     __syncthreads();
-    ntt2_inv<LOGN>(x, lds, Ti);
+    ntt16_inv<LOGN>(x, lds, tab.inv, g);
 #pragma unroll
     for (int c = 0; c < E; ++c) d[g + c * stride] = gl::mul(x[c], Ninv);
   }

@@ -52,8 +52,8 @@ hipError_t negate_odd(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDe
                       uint64_t* G, int m);
 
 // ---------------------------------------------------------------- k_misc.hip
-hipError_t ntt_cyclic(int logN, int count, hipStream_t s, uint64_t* data, const uint64_t* T, const uint64_t* Ti,
-                      uint64_t Ninv, int inverse);
+hipError_t ntt_cyclic(int logN, int count, hipStream_t s, uint64_t* data, const yhk::NttTab& tab, uint64_t Ninv,
+                      int inverse);
 hipError_t crt(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const uint64_t* A, int lin, int ncoef,
                size_t lstride, uint64_t* out, int spacing);
 hipError_t residue_to_y(dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const uint64_t* res, int spacing,

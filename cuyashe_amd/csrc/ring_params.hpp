@@ -403,27 +403,4 @@ inline Ntt16Tables ntt16_tables(int N) {
   return t;
 }
 
-// Twiddle tables of the cyclic length-N transform in the bit-reversed layout
-// consumed by the Cooley-Tukey / Gentleman-Sande passes:
-// T[2^s + i] = omega_{2^(s+1)}^{bitrev_s(i)}, omega_N = 7^((P-1)/N).
-inline void cyclic_tables(int N, std::vector<uint64_t>& fwd, std::vector<uint64_t>& inv) {
-  int logN = RingParams::ilog2(N);
-  uint64_t w = gl::pow(7, (gl::P - 1) / (uint64_t)N);
-  uint64_t wi = gl::inv(w);
-  fwd.assign(N, 1);
-  inv.assign(N, 1);
-  for (int s = 0; s < logN; ++s) {
-    int m = 1 << s;
-    // primitive 2m-th root: omega_N^(N/(2m))
-    uint64_t r = gl::pow(w, (uint64_t)(N / (2 * m)));
-    uint64_t ri = gl::pow(wi, (uint64_t)(N / (2 * m)));
-    for (int i = 0; i < m; ++i) {
-      int b = 0;
-      for (int k = 0; k < s; ++k) if (i >> k & 1) b |= 1 << (s - 1 - k);
-      fwd[m + i] = gl::pow(r, (uint64_t)b);
-      inv[m + i] = gl::pow(ri, (uint64_t)b);
-    }
-  }
-}
-
 }  // namespace yh

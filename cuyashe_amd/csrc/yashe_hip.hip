@@ -60,8 +60,6 @@ constexpr long YH_PLAIN_MAX_N = 1L << 20;
   catch (const std::exception& e) { return fail(YH_ERR_INTERNAL, e.what()); }
 
 struct DevTables {
-  uint64_t* fwd = nullptr;  // radix-2 tables of the standalone cyclic transform
-  uint64_t* inv = nullptr;
   uint64_t* t16 = nullptr;  // radix-16 tables (one allocation)
   yhk::NttTab tab{};
 };
@@ -300,14 +298,8 @@ int get_cyclic(yh_ctx* ctx, int N, DevTables* out) {
   auto it = ctx->cyclic.find(N);
   if (it != ctx->cyclic.end()) { *out = it->second; return YH_OK; }
   if (ctx->capturing) return fail(YH_ERR_ARG, "transform tables would be built during graph capture");
-  std::vector<uint64_t> f, iv;
-  yh::cyclic_tables(N, f, iv);
   DevTables t;
   if (int st = upload_ntt16(N, &t.t16, &t.tab)) return st;
-  YH_HIP(hipMalloc(&t.fwd, N * sizeof(uint64_t)));
-  YH_HIP(hipMalloc(&t.inv, N * sizeof(uint64_t)));
-  YH_HIP(hipMemcpy(t.fwd, f.data(), N * sizeof(uint64_t), hipMemcpyHostToDevice));
-  YH_HIP(hipMemcpy(t.inv, iv.data(), N * sizeof(uint64_t), hipMemcpyHostToDevice));
   ctx->cyclic[N] = t;
   *out = t;
   return YH_OK;
@@ -502,11 +494,7 @@ int yh_ctx_destroy(yh_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   drain_timing(ctx);
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-  for (auto& kv : ctx->cyclic) {
-    (void)hipFree(kv.second.fwd);
-    (void)hipFree(kv.second.inv);
-    (void)hipFree(kv.second.t16);
-  }
+  for (auto& kv : ctx->cyclic) (void)hipFree(kv.second.t16);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return YH_OK;
@@ -793,7 +781,7 @@ int yh_ntt(yh_ctx* ctx, uint64_t* d_data, int N, int count, int direction) {
   if (int s = get_cyclic(ctx, N, &t)) return s;
   uint64_t Ninv = gl::inv((uint64_t)N);
   const int inv = direction == YH_INVERSE ? 1 : 0;
-  YH_LAUNCH(yhl::ntt_cyclic(ilog2i(N), count, ctx->stream, d_data, t.fwd, t.inv, Ninv, inv));
+  YH_LAUNCH(yhl::ntt_cyclic(ilog2i(N), count, ctx->stream, d_data, t.tab, Ninv, inv));
   return YH_OK;
 }
 
