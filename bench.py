@@ -103,6 +103,46 @@ def measured_traffic(kernel, batch):
     return None
 
 
+UBENCH_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cuyashe_amd", "lib", "libyashe_ubench.so")
+
+
+def arith_floor(R, B, n, k_prod_ms, prod_bytes):
+    """Arithmetic floor of k_prod from the register-resident microbenchmark
+    (cuyashe_amd/csrc/ubench.hip): SIMD cycles per wave of a forward
+    radix-16 pass (32 shift butterflies + 15 twiddle products), of the DFT-16
+    alone forward and inverse, and of 16 Montgomery products. One k_prod wave
+    runs, per lane, two forward transforms (twiddled passes, one pass without
+    twiddles, one product more for the folded twist), one inverse (the same
+    DFTs, 15 twiddles per twiddled pass plus 16 for the folded untwist) and
+    the pointwise product; loads, the CRT, the LDS exchanges and the
+    reconstruction output are left out, so this bounds the kernel from
+    below. floor_ms = waves * cycles per wave / (SIMDs * clock)."""
+    import ctypes
+    if not os.path.exists(UBENCH_LIB):
+        return None
+    lib = ctypes.CDLL(UBENCH_LIB)
+    cyc = (ctypes.c_double * 4)()
+    info = (ctypes.c_double * 3)()
+    if lib.yhu_transform_cycles(0, 2000, cyc, info) != 0:
+        return None
+    c_pass, c_dif, c_dit, c_mont16 = (float(v) for v in cyc)
+    mont = c_mont16 / 16
+    logn = n.bit_length() - 1
+    nfull = logn // 4
+    tw = sum(1 for p in range(nfull) if logn - 4 - 4 * p > 0)
+    fwd = tw * c_pass + (nfull - tw) * c_dif + mont
+    inv = nfull * c_dit + (15 * tw + 1) * mont
+    per_wave = 2 * fwd + inv + c_mont16
+    waves = B * R * max(1, n // 16 // 64)
+    floor_ms = waves * per_wave / (info[0] * 4 * info[1])
+    return {"kernel": "k_prod", "cycles_per_wave": round(per_wave, 1),
+            "block_cycles": {"fwd_pass": round(c_pass, 1), "dif16": round(c_dif, 1), "dit16": round(c_dit, 1),
+                             "mont_x16": round(c_mont16, 1)},
+            "clock_khz": int(info[1]), "floor_ms": round(floor_ms, 4),
+            "floor_frac": round(prod_bytes / (floor_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "measured_over_floor": round(k_prod_ms / floor_ms, 3)}
+
+
 def bench_ntt(ya, ctx, N=4096, count=2048, reps=20):
     """The drop-in transform primitive (yh_ntt, callNTT's replacement): cyclic
     forward and inverse Goldilocks NTTs of `count` vectors of length N in
@@ -519,6 +559,8 @@ def main():
         valu_roofline = {"bound": "valu", "kernel": dom_name, "achieved": round(ach, 1), "peak": VALU_PEAK_G,
                          "unit": "G wave-instructions/s", "frac": round(ach / VALU_PEAK_G, 4),
                          "instructions_per_launch": valu}
+    floor = arith_floor(R, B, n, dom_avg if dom_name == "k_prod" else kernels["k_prod"]["avg_ms"],
+                        kernels["k_prod"]["bytes_per_launch"])
     ntt_k = [k for k in kernels.values() if k["bytes_per_launch"]]
     ntt_bytes = sum(k["bytes_per_launch"] * k["launches_per_step"] for k in ntt_k)
     ntt_ms = sum(k["avg_ms"] * k["launches_per_step"] for k in ntt_k)
@@ -572,6 +614,7 @@ def main():
                     for k, v in kernels.items()},
         "roofline": roofline,
         "valu_roofline": valu_roofline,
+        "arith_floor": floor,
         "cpu_baseline": cpu,
         "bit_exact_sample": exact,
         "outputs_checked": {"timed_batch": B, "plain": 1, "distinct_pairs_vs_oracle": min(B, uniq),

@@ -62,6 +62,19 @@ def build_hip(verbose: bool = True, jobs: int = None) -> str:
     return LIB_HIP
 
 
+LIB_UBENCH = os.path.join(LIBDIR, "libyashe_ubench.so")
+
+
+def build_ubench() -> str:
+    """The transform-arithmetic microbenchmark (a library of its own, read
+    by bench.py; no product kernel depends on it)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB_UBENCH + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "ubench.hip")])
+    os.replace(tmp, LIB_UBENCH)
+    return LIB_UBENCH
+
+
 HOST_SRCS = ["polynomial.cpp", "yashe.cpp"]
 LIB_HOST = os.path.join(LIBDIR, "libcuyashe_host.so")
 TEST_API = os.path.join(ROOT, "tests", "cpp", "build", "test_api")
@@ -89,7 +102,7 @@ def build_host() -> str:
 
 
 def build_all():
-    return [build_hip(), build_host()]
+    return [build_hip(), build_host(), build_ubench()]
 
 
 if __name__ == "__main__":
