@@ -53,6 +53,11 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_KSMAC_WAVES
 #define YH_KSMAC_WAVES 4
 #endif
+// Coefficients whose l_w products are accumulated side by side (four: no
+// spills at 128 VGPRs; eight spilled 11 dwords per lane and ran 1 % slower).
+#ifndef YH_KSMAC_GROUP
+#define YH_KSMAC_GROUP 4
+#endif
 // The digit transforms (k_ks_digits, 79 VGPRs) may run six waves; forcing
 // eight (64 VGPRs, small spills) measured slower. At n = 4096 the LDS of a
 // transform (34 KB) admits four workgroups per CU, so every transform kernel
@@ -525,12 +530,13 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
   // The l_w products of a slot are summed exactly as 128-bit values (top
   // counts the carries out of bit 128, < l_w <= 16) and reduced once.
   uint64_t acc[E];
+  constexpr int G = YH_KSMAC_GROUP;
 #pragma unroll
-  for (int h = 0; h < E; h += 8) {
-    uint64_t lo[8], hi[8];
-    uint32_t top[8];
+  for (int h = 0; h < E; h += G) {
+    uint64_t lo[G], hi[G];
+    uint32_t top[G];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < G; ++c) {
       lo[c] = hi[c] = 0;
       top[c] = 0;
     }
@@ -538,7 +544,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
       const uint64_t* dk = dg + (size_t)k * N;
       const uint64_t* ek = ev + (size_t)k * Rks * N;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
+      for (int c = 0; c < G; ++c) {
         uint64_t ph, pl;
         gl::mul128(dk[(h + c) * stride], ek[(h + c) * stride], ph, pl);
         lo[c] += pl;
@@ -548,7 +554,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
       }
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[h + c] = gl::reduce_wide(top[c], hi[c], lo[c]);
+    for (int c = 0; c < G; ++c) acc[h + c] = gl::reduce_wide(top[c], hi[c], lo[c]);
   }
   const int g2 = opaque_tid(acc[0]);
   constexpr bool M = true;  // untwist folded into the last inverse pass
