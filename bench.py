@@ -63,9 +63,6 @@ def parse():
 TUNE_CACHE = os.path.join(ROOT, "cuyashe_amd", "tune_cache.json")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
 SQ_FILE = os.path.join(ROOT, "profiles", "sq_counters_mul_keyswitch_n4096.md")
-# VALU issue peak of MI355X: 256 CUs x 4 SIMDs x one wave64 instruction per 4
-# cycles at 2.4 GHz = 614.4 G wave-instructions/s
-VALU_PEAK_G = 256 * 2.4
 
 
 def measured_valu(kernel_symbol, batch):
@@ -552,13 +549,16 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name, B),
                 "kernel": dom_name, "avg_ms": round(dom_avg, 5), "launches": dom_n}
     symbols = {"k_prod": "k_prod<12, true>", "k_ks_digits": "k_ks_digits<12, true>", "k_keyswitch": "k_ks_mac<12>"}
+    # VALU instruction rate of the dominant kernel (instruction count from the
+    # committed SQ counters, time from the events above). Reported without a
+    # fraction: on gfx950 the register-resident microbenchmark retires this
+    # code's mix faster than one wave64 instruction per 4 cycles per SIMD, so
+    # that figure is no ceiling; `arith_floor` is the measured one.
     valu = measured_valu(symbols[dom_name], B)
-    valu_roofline = None
+    valu_rate = None
     if valu:
-        ach = valu / (dom_avg * 1e-3) / 1e9
-        valu_roofline = {"bound": "valu", "kernel": dom_name, "achieved": round(ach, 1), "peak": VALU_PEAK_G,
-                         "unit": "G wave-instructions/s", "frac": round(ach / VALU_PEAK_G, 4),
-                         "instructions_per_launch": valu}
+        valu_rate = {"kernel": dom_name, "instructions_per_launch": valu,
+                     "achieved": round(valu / (dom_avg * 1e-3) / 1e9, 1), "unit": "G wave-instructions/s"}
     floor = arith_floor(R, B, n, dom_avg if dom_name == "k_prod" else kernels["k_prod"]["avg_ms"],
                         kernels["k_prod"]["bytes_per_launch"])
     ntt_k = [k for k in kernels.values() if k["bytes_per_launch"]]
@@ -613,7 +613,7 @@ def main():
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items() if kk != "id"}
                     for k, v in kernels.items()},
         "roofline": roofline,
-        "valu_roofline": valu_roofline,
+        "valu_rate": valu_rate,
         "arith_floor": floor,
         "cpu_baseline": cpu,
         "bit_exact_sample": exact,
