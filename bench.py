@@ -206,6 +206,8 @@ def bench_uneven(ya, ctx, reps=10):
     wall_ring = (time.perf_counter() - t0) / reps
     exact = all(bool((g[2].download(g[3].shape) == refs[key].download(g[3].shape)).all())
                 for key, g in groups.items())
+    # algorithmic bytes of the launch: 3 transforms of 16 n bytes per (product, modulus)
+    alg = sum(3 * rings[(n, q)].R * 16 * n for n, q in spec)
     b.close()
     for r in rings.values():
         r.close()
@@ -213,9 +215,13 @@ def bench_uneven(ya, ctx, reps=10):
     for n, q in spec:
         k = f"n{n}/q{q.bit_length()}"
         hist[k] = hist.get(k, 0) + 1
+    k_avg = k_ms / max(k_n, 1)
     return {"workload": "512 uneven ring products, n in {1024,2048,4096,8192} (.4/.3/.2/.1), q 2^127-1 or 2^255-19",
             "value": round(512 / wall, 1), "unit": "products/s", "batch_ms": round(wall * 1e3, 4),
-            "kernel_avg_ms": round(k_ms / max(k_n, 1), 4), "launches_per_batch": 2,
+            "kernel_avg_ms": round(k_avg, 4), "launches_per_batch": 2,
+            "roofline": {"bound": "hbm", "kernel": "k_batch_prod", "bytes_per_launch": alg,
+                         "achieved": round(alg / (k_avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(alg / (k_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
             "per_ring_launches_ms": round(wall_ring * 1e3, 4), "speedup_vs_per_ring": round(wall_ring / wall, 3),
             "config": cfg, "rings": hist, "identical_to_per_ring": exact}
 

@@ -132,12 +132,36 @@ __device__ __forceinline__ void ld16(const uint64_t* lds, uint64_t* x, int base,
   for (int c = 0; c < 16; ++c) x[c] = lds[lds_pad(base + (c << b0))];
 }
 
+// Synchronisation of one transform's threads around its LDS exchanges: the
+// whole workgroup, or (wave = true) only the wavefront, whose LDS operations
+// execute in order. `wave` must be the same for the whole workgroup.
+//
+// Which exchanges need the workgroup: in the layout of pass b0 thread g owns
+// the 16 positions base(g) + c 2^b0, so the layouts of b0 and b0 - 4 (and the
+// final layout 16 g + c, that of b0 = 0) map a block of 2^(b0 + 4) positions
+// onto the same 2^b0 threads g >> b0 (B = the larger b0 of the two layouts).
+// Threads g are consecutive within a wavefront from a multiple of the
+// transform's thread count, so an exchange with B <= 6 stays inside one
+// wavefront. And a pass writes back exactly the positions it read, which no
+// other thread touches in that pass: the load of a pass needs no barrier.
+__device__ __forceinline__ void xsync(bool wave) {
+  if (wave) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Forward. Entry: x holds elements g + c*N/16 (natural positions). Exit:
 // x holds the transform at positions 16g + c (permuted order). Leaves the LDS
-// in use: callers synchronise before the next transform writes it.
+// in use (other wavefronts may still be reading it): callers synchronise the
+// workgroup before anything writes it again.
 // g is the thread's index within the transform (threadIdx.x unless several
-// transforms share a workgroup); __syncthreads spans the whole workgroup, so
-// every thread of it calls the same transform.
+// transforms share a workgroup, each from a multiple of N/16 threads);
+// __syncthreads spans the whole workgroup, so every thread of it calls the
+// same transform.
 //
 // MERGED (negacyclic): the caller has multiplied input slot c by
 // 2^(78 c) only, and the first pass multiplies every output slot c by
@@ -155,10 +179,7 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
   for (int p = 0; p < NFULL; ++p) {
     const int b0 = LOGN - 4 - 4 * p;
     const int base = pbase(g, b0);
-    if (p > 0) {
-      ld16(lds, x, base, b0);
-      __syncthreads();
-    }
+    if (p > 0) ld16(lds, x, base, b0);
     dif_r<16>(x);
     if (MERGED && p == 0) {
       const uint64_t* t = tw0 + g;
@@ -172,7 +193,7 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
     }
     if (p < NFULL - 1 || SHORT) {
       st16(lds, x, base, b0);
-      __syncthreads();
+      xsync(b0 <= 6);
     }
   }
   if (SHORT) {
@@ -184,8 +205,9 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
 }
 
 // Inverse (unscaled, no untwist). Entry: positions 16g + c as left by
-// ntt16_fwd. Exit: x holds elements g + c*N/16. Callers synchronise before
-// calling if another transform used the LDS just before.
+// ntt16_fwd. Exit: x holds elements g + c*N/16. Callers synchronise the
+// workgroup before calling if another transform used the LDS just before,
+// and after it before anything writes the LDS again.
 //
 // MERGED (negacyclic): the last pass multiplies every input slot c
 // by twi0[c N/16 + g] = w_N^(-brev4(c) g) psi^-g N^-1 (times the caller's
@@ -201,16 +223,13 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
 #pragma unroll
     for (int blk = 0; blk < 16; blk += R) dit_r<R>(x + blk);
     st16(lds, x, 16 * g, 0);
-    __syncthreads();
+    xsync(true);
   }
 #pragma unroll
   for (int p = NFULL - 1; p >= 0; --p) {
     const int b0 = LOGN - 4 - 4 * p;
     const int base = pbase(g, b0);
-    if (p < NFULL - 1 || SHORT) {
-      ld16(lds, x, base, b0);
-      __syncthreads();
-    }
+    if (p < NFULL - 1 || SHORT) ld16(lds, x, base, b0);
     if (MERGED && p == 0) {
       const uint64_t* t = twi0 + g;
 #pragma unroll
@@ -224,7 +243,7 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
     dit_r<16>(x);
     if (p > 0) {
       st16(lds, x, base, b0);
-      __syncthreads();
+      xsync(b0 + 4 <= 6);
     }
   }
   if (MERGED) {
@@ -262,20 +281,6 @@ __device__ __forceinline__ void short_dit(uint64_t* x, int r) {
   }
 }
 
-// Synchronisation of one transform's threads around its LDS exchanges: the
-// whole workgroup, or (wave = true: transforms of at most 64 threads, each
-// inside one wavefront) only the wavefront, whose LDS operations execute in
-// order. `wave` must be the same for the whole workgroup.
-__device__ __forceinline__ void xsync(bool wave) {
-  if (wave) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  } else {
-    __syncthreads();
-  }
-}
-
 __device__ __forceinline__ void ntt16_fwd_rt(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ tw, int g,
                                              int logn, bool wave = false) {
   const int nfull = logn >> 2;
@@ -284,10 +289,7 @@ __device__ __forceinline__ void ntt16_fwd_rt(uint64_t* x, uint64_t* lds, const u
   for (int p = 0; p < nfull; ++p) {
     const int b0 = logn - 4 - 4 * p;
     const int base = pbase(g, b0);
-    if (p > 0) {
-      ld16(lds, x, base, b0);
-      xsync(wave);
-    }
+    if (p > 0) ld16(lds, x, base, b0);
     dif_r<16>(x);
     if (b0 > 0) {
       const int s = 1 << b0;
@@ -298,7 +300,7 @@ __device__ __forceinline__ void ntt16_fwd_rt(uint64_t* x, uint64_t* lds, const u
     }
     if (p < nfull - 1 || sh) {
       st16(lds, x, base, b0);
-      xsync(wave);
+      xsync(wave || b0 <= 6);
     }
   }
   if (sh) {
@@ -314,7 +316,7 @@ __device__ __forceinline__ void ntt16_inv_rt(uint64_t* x, uint64_t* lds, const u
   if (sh) {
     short_dit(x, 1 << sh);
     st16(lds, x, 16 * g, 0);
-    xsync(wave);
+    xsync(true);
   }
   // table offset of pass p = sum of 15 * 2^b0 over the passes before it
   int off = 0;
@@ -325,10 +327,7 @@ __device__ __forceinline__ void ntt16_inv_rt(uint64_t* x, uint64_t* lds, const u
   for (int p = nfull - 1; p >= 0; --p) {
     const int b0 = logn - 4 - 4 * p;
     const int base = pbase(g, b0);
-    if (p < nfull - 1 || sh) {
-      ld16(lds, x, base, b0);
-      xsync(wave);
-    }
+    if (p < nfull - 1 || sh) ld16(lds, x, base, b0);
     if (b0 > 0) {
       const int s = 1 << b0;
       off -= 15 * s;
@@ -339,7 +338,7 @@ __device__ __forceinline__ void ntt16_inv_rt(uint64_t* x, uint64_t* lds, const u
     dit_r<16>(x);
     if (p > 0) {
       st16(lds, x, base, b0);
-      xsync(wave);
+      xsync(wave || b0 + 4 <= 6);
     }
   }
 }

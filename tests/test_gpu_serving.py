@@ -101,3 +101,19 @@ def workloads_c5():
     rings = c5_rings()
     assert len(rings) == 12
     return rings
+
+
+def test_serving_refuses_bad_requests(server):
+    # the native loop validates every request before running any
+    import numpy as np
+    key = RINGS[0]
+    with pytest.raises(ValueError):
+        server.run([Request(1, "nope", key)])
+    with pytest.raises(ya.YashError, match="pool index"):
+        server.run([Request(2, "add", key, a=0, b=10 ** 6)])
+    msg = np.zeros(key[0], dtype=np.int64)
+    msg[3] = 1024  # = t
+    with pytest.raises(ya.YashError, match="below t"):
+        server.run([Request(3, "encdec", key, msg=msg)])
+    st = server.run([Request(4, "add", key, a=0, b=1), Request(5, "encdec", key, msg=np.ones(key[0], dtype=np.int64))])
+    assert st["all_decrypts_ok"] and st["requests"] == 2
