@@ -136,6 +136,16 @@ __device__ __forceinline__ bool xcd_map(int w, int R, int batch, int& pair, int&
   return pair < batch;
 }
 
+// The same map with the operands of each XCD taken from the last to the
+// first (operand groups in reverse order; an operand keeps its XCD).
+__device__ __forceinline__ bool xcd_map_rev(int w, int R, int batch, int& pair, int& prime) {
+  const int x = w % XCDS, k = w / XCDS;
+  const int groups = (batch + XCDS - 1) / XCDS;
+  pair = (groups - 1 - k / R) * XCDS + x;
+  prime = k % R;
+  return pair < batch;
+}
+
 // ---------------------------------------------------------------- small mod p
 
 // x mod p for x < 2^64 with b = floor(2^64/p) (p < 2^40).
@@ -523,7 +533,10 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
   constexpr int stride = N >> 4;
   const int Rks = rd.set_R[1];
   int ct, i;
-  if (!xcd_map(blockIdx.x, Rks, ka.batch, ct, i)) return;
+  // ciphertexts from the last to the first: k_ks_digits wrote their digit
+  // transforms from the first to the last, so the last ones are the
+  // likeliest still cached
+  if (!xcd_map_rev(blockIdx.x, Rks, ka.batch, ct, i)) return;
   const int g = threadIdx.x;
   const uint64_t* dg = ka.D + (size_t)ct * rd.lw * N + g;
   const uint64_t* ev = ka.evk + (size_t)i * N + g;
@@ -907,6 +920,11 @@ struct IcrtArgs {
   size_t out_pstride;    // elements between pairs
   int positive;          // 1: result in [0, M) (plain product); 0: centred
   uint64_t* const* out_list;  // optional per-pair output addresses (replace out + pair*out_pstride)
+  // 1: pairs are taken from the last to the first. A producer that wrote the
+  // rows from the first pair to the last leaves the last ones in the
+  // memory-side cache (the rows of a large batch exceed it), so reading them
+  // first turns part of the reads into cache hits.
+  int reverse;
 };
 
 // 32-bit word w of a little-endian limb array.
@@ -1236,7 +1254,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
 template <int LQ>
 __global__ void __launch_bounds__(YH_ICRT_BLOCK) __attribute__((amdgpu_waves_per_eu(icrt_min_waves(LQ))))
 k_icrt(RingDev rd, IcrtArgs ia) {
-  const int pair = blockIdx.y;
+  const int pair = ia.reverse ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
   const uint32_t* Y = ia.Y + (size_t)pair * y_rows(RS) * ia.N;

@@ -376,6 +376,7 @@ int ring_product(yh_ring* r, const Scratch& sc, const uint64_t* a, const uint64_
   if (st) return st;
   yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 0, mode, out, LQ, n, (size_t)LQ * n, 0);
   ia.out_list = out_list;
+  ia.reverse = 1;  // k_prod wrote the rows from the first pair to the last
   return launch_icrt(r, ia, batch);
 }
 
@@ -413,6 +414,7 @@ int keyswitch_impl(yh_ring* r, const Scratch& sc, const yh_evk* k, const uint64_
   }
   yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0);
   ia.out_list = out_list;
+  ia.reverse = 0;  // k_ks_mac wrote the rows from the last ciphertext to the first
   return launch_icrt(r, ia, batch);
 }
 
@@ -1085,6 +1087,7 @@ static int decrypt_impl(yh_ring* r, const uint64_t* d_f, const uint64_t* d_c, ui
   if (st) return st;
   yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 0, yhk::EPI_DECRYPT, d_m, 1, n, n, 0);
   ia.out_list = m_list;
+  ia.reverse = 1;  // k_prod wrote the rows from the first pair to the last
   return launch_icrt(r, ia, batch);
 }
 
