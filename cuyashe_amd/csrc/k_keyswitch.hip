@@ -35,6 +35,23 @@ hipError_t ks_mac(int logn, dim3 grid, hipStream_t s, const RingDev& rd, const K
   return hipGetLastError();
 }
 
+hipError_t ks_scale_digits(int logn, int LQ, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka) {
+  const dim3 block((1 << logn) / 16);
+  const size_t lds = lds_bytes(1 << logn);
+  if (LQ != 1 && LQ != 2) return hipErrorInvalidValue;
+  switch (logn) {
+#define X(L)                                                                                    \
+  case L:                                                                                       \
+    if (LQ == 1) hipLaunchKernelGGL((yhk::k_ks_scale_digits<L, 1>), grid, block, lds, s, rd, ka); \
+    else hipLaunchKernelGGL((yhk::k_ks_scale_digits<L, 2>), grid, block, lds, s, rd, ka);         \
+    break;
+    YH_KS_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t ks_small(dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka, const uint64_t* gamma_res) {
   hipLaunchKernelGGL(yhk::k_keyswitch_small, grid, dim3(64), 0, s, rd, ka, gamma_res);
   return hipGetLastError();
@@ -69,6 +86,10 @@ static void ks_attr() {
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_digits<L, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_digits<L, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_mac<L>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)yhk::k_ks_scale_digits<L, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes);
+  (void)hipFuncSetAttribute((const void*)yhk::k_ks_scale_digits<L, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_evk_prep<L>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_corr<L>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }

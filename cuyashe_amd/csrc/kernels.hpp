@@ -66,6 +66,16 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 #ifndef YH_KSDIG_WAVES
 #define YH_KSDIG_WAVES 6
 #endif
+// k_ks_scale_digits: the 16 coefficients' 32-bit words (64 VGPRs for a
+// two-limb q) stay in registers through the transforms
+#ifndef YH_KSSD_WAVES
+#define YH_KSSD_WAVES 4
+#endif
+// Rows of the next YH_KSSD_AHEAD coefficients requested before the current
+// one is reconstructed (one: 128 VGPRs, 6 dwords spilled; two spills 96)
+#ifndef YH_KSSD_AHEAD
+#define YH_KSSD_AHEAD 1
+#endif
 // coefficients per thread of the reconstruction kernel outside its
 // pipelined path (two measured slower than one at n = 4096, two limbs)
 #ifndef YH_ICRT_COEFS
@@ -482,6 +492,11 @@ struct KsArgs {
   uint64_t* D;              // [ct][lw][n] digit transforms (scratch)
   uint32_t* Y;              // [ct][R_ks][n]
   int batch;
+  // k_ks_mac takes the ciphertexts from the last to the first when the digit
+  // transforms were written from the first to the last (the last ones are
+  // the likeliest still cached), and in order otherwise
+  int reverse;
+  const uint32_t* P;        // k_ks_scale_digits: the product's rows [ct][R + 1][n]
 };
 
 __device__ __forceinline__ uint64_t digit_of(const uint64_t* a, int n, int lq, int k, int wb) {
@@ -533,10 +548,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
   constexpr int stride = N >> 4;
   const int Rks = rd.set_R[1];
   int ct, i;
-  // ciphertexts from the last to the first: k_ks_digits wrote their digit
-  // transforms from the first to the last, so the last ones are the
-  // likeliest still cached
-  if (!xcd_map_rev(blockIdx.x, Rks, ka.batch, ct, i)) return;
+  const bool mine = ka.reverse ? xcd_map_rev(blockIdx.x, Rks, ka.batch, ct, i) : xcd_map(blockIdx.x, Rks, ka.batch, ct, i);
+  if (!mine) return;
   const int g = threadIdx.x;
   const uint64_t* dg = ka.D + (size_t)ct * rd.lw * N + g;
   const uint64_t* ev = ka.evk + (size_t)i * N + g;
@@ -1088,13 +1101,13 @@ __device__ __forceinline__ void icrt_mod_q(const RingDev& rd, const uint32_t* y,
 }
 
 // Scale-and-round epilogue from the column sums of icrt_pass (NF = 2 over
-// beta, alpha): writes round(t c / q) mod q (EPI_SCALE in icrt_coef).
+// beta, alpha): res = round(t c / q) mod q (EPI_SCALE in icrt_coef).
 template <int LQ>
-__device__ __forceinline__ void scale_finish(const RingDev& rd, int set, const uint64_t (&S)[2][2 * LQ], uint64_t yP,
-                                             uint32_t k, uint64_t* o, size_t out_stride) {
+__device__ __forceinline__ void scale_value(const RingDev& rd, int set, const uint64_t (&S)[2][2 * LQ], uint64_t yP,
+                                            uint32_t k, uint64_t (&rem)[LQ]) {
   constexpr int LMU = 2 * LQ + 3 - LQ;
   const int RS = rd.set_R[set];
-  uint64_t v[LQ + 1], quo[LMU], rem[LQ];
+  uint64_t v[LQ + 1], quo[LMU];
   fold_finish<LQ>(S[0], rd.beta, RS, rd.negbetaM, yP, k, v);
   divmod_v<LQ>(v, rd, quo, rem);  // S' = quo q + rem
   const uint64_t up = twice_gt_q<LQ>(rem, rd.q) ? 1 : 0;
@@ -1109,6 +1122,13 @@ __device__ __forceinline__ void scale_finish(const RingDev& rd, int set, const u
   for (int l = 0; l < LQ + 1; ++l) add[l] = l == 0 ? up : 0;
   add_n<LQ + 1>(v, add);
   divmod_v<LQ>(v, rd, quo, rem);
+}
+
+template <int LQ>
+__device__ __forceinline__ void scale_finish(const RingDev& rd, int set, const uint64_t (&S)[2][2 * LQ], uint64_t yP,
+                                             uint32_t k, uint64_t* o, size_t out_stride) {
+  uint64_t rem[LQ];
+  scale_value<LQ>(rd, set, S, yP, k, rem);
 #pragma unroll
   for (int l = 0; l < LQ; ++l) o[(size_t)l * out_stride] = rem[l];
 }
@@ -1299,6 +1319,89 @@ k_icrt(RingDev rd, IcrtArgs ia) {
     const int j = (blockIdx.x * YH_ICRT_COEFS + u) * blockDim.x + threadIdx.x;
     if (j < ia.ncoef)
       icrt_coef<LQ>(rd, Y + j, ia.N, ia.set, ia.mode, ia.positive, o + j, ia.out_limbs, ia.out_stride);
+  }
+}
+
+// The YASHE multiply's scale-and-round fused with the keyswitch digit
+// transforms (mul+keyswitch, q of at most two limbs, w = 2^32): one
+// workgroup per ciphertext. Thread g reconstructs round(t c / q) mod q for
+// the coefficients g + c n/16 straight from the product's rows -- exactly
+// the 16 elements it holds in every digit transform -- keeps their 32-bit
+// words (the digits) in registers and transforms one digit after the other
+// through the same LDS. The scaled product is never written to memory and
+// never read back, and one reconstruction launch disappears. Ciphertexts go
+// from the last to the first (k_prod wrote the rows from the first to the
+// last) and each keeps its XCD, that of k_ks_mac's workgroups that read its
+// transforms.
+// Digit d of ciphertext ct (32-bit words w of its 16 coefficients) centred,
+// twisted and transformed into the slot-major digit layout of k_ks_digits.
+template <int LOGN>
+__device__ __forceinline__ void ks_digit_transform(const RingDev& rd, const KsArgs& ka, int ct, int d,
+                                                   const uint32_t (&w)[E], bool after_other = false) {
+  extern __shared__ uint64_t lds[];
+  constexpr int N = 1 << LOGN;
+  constexpr int stride = N >> 4;
+  uint64_t x[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) x[c] = twist_slot_digit(w[c], 1ull << 31, c);
+  if (after_other) __syncthreads();  // every thread has left the previous digit's LDS
+  const int g = opaque_tid(x[E - 1]);
+  ntt16_fwd<LOGN, true>(x, lds, rd.tab.fwd, g, rd.tab.fwd0);
+  uint64_t* o = ka.D + ((size_t)ct * rd.lw + d) * N + g;
+#pragma unroll
+  for (int c = 0; c < E; ++c) o[c * stride] = x[c];
+}
+
+// Coefficients C, C + 1, ... E - 1 of k_ks_scale_digits' reconstruction:
+// round(t c / q) mod q from the rows in rows[C mod (AH + 1)] into the words
+// w[.][C], after requesting the rows of coefficient C + AH. Recursion on a
+// compile-time C keeps every index static, so w and rows stay in registers.
+template <int LOGN, int LQ, int AH, int C>
+__device__ __forceinline__ void scale_words(const RingDev& rd, const uint32_t* Y, int nrow,
+                                            uint32_t (&rows)[AH + 1][ICRT_PIPE_ROWS], uint32_t (&w)[2 * LQ][E]) {
+  constexpr int N = 1 << LOGN;
+  constexpr int stride = N >> 4;
+  constexpr int MR = ICRT_PIPE_ROWS;
+  if constexpr (C + AH < E) {
+#pragma unroll
+    for (int r = 0; r < MR; ++r) rows[(C + AH) % (AH + 1)][r] = r < nrow ? Y[(size_t)r * N + (C + AH) * stride] : 0u;
+  }
+  const uint64_t* X[2] = {rd.beta, rd.alpha};
+  uint64_t yP, S[2][2 * LQ], rem[LQ];
+  const uint32_t k = icrt_pass_regs<LQ, 2, MR>(rd, rows[C % (AH + 1)], 0, X, S, yP);
+  scale_value<LQ>(rd, 0, S, yP, k, rem);
+#pragma unroll
+  for (int v = 0; v < 2 * LQ; ++v) w[v][C] = (uint32_t)(rem[v >> 1] >> (32 * (v & 1)));
+  if constexpr (C + 1 < E) scale_words<LOGN, LQ, AH, C + 1>(rd, Y, nrow, rows, w);
+}
+
+template <int LOGN, int LQ>
+__global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSSD_WAVES) k_ks_scale_digits(RingDev rd, KsArgs ka) {
+  constexpr int N = 1 << LOGN;
+  constexpr int stride = N >> 4;
+  constexpr int MR = ICRT_PIPE_ROWS;
+  constexpr int NW = 2 * LQ;  // 32-bit words of round(t c / q) mod q, at least l_w
+  int ct, unit;
+  if (!xcd_map_rev(blockIdx.x, 1, ka.batch, ct, unit)) return;
+  const int g = threadIdx.x;
+  const int nrow = y_rows(rd.set_R[0]);
+  const uint32_t* Y = ka.P + (size_t)ct * nrow * N + g;
+  uint32_t w[NW][E];
+  // rows of coefficient g + c stride; the rows of the next AH coefficients
+  // are requested before the current one is computed
+  constexpr int AH = YH_KSSD_AHEAD;
+  uint32_t rows[AH + 1][MR];
+#pragma unroll
+  for (int c = 0; c < AH; ++c)
+#pragma unroll
+    for (int r = 0; r < MR; ++r) rows[c][r] = r < nrow ? Y[(size_t)r * N + c * stride] : 0u;
+  scale_words<LOGN, LQ, AH, 0>(rd, Y, nrow, rows, w);
+  // one call per digit with a compile-time index, so w stays in registers
+  ks_digit_transform<LOGN>(rd, ka, ct, 0, w[0]);
+  if (rd.lw > 1) ks_digit_transform<LOGN>(rd, ka, ct, 1, w[1], true);
+  if constexpr (NW > 2) {
+    if (rd.lw > 2) ks_digit_transform<LOGN>(rd, ka, ct, 2, w[2], true);
+    if (rd.lw > 3) ks_digit_transform<LOGN>(rd, ka, ct, 3, w[3], true);
   }
 }
 

@@ -32,6 +32,9 @@ void prod_attrs();
 // ---------------------------------------------------------------- k_keyswitch.hip
 hipError_t ks_digits(int logn, bool w32, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka);
 hipError_t ks_mac(int logn, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka);
+// the multiply's scale-and-round fused with the digit transforms (q of one or
+// two limbs, w = 2^32): one workgroup per ciphertext, reads ka.P
+hipError_t ks_scale_digits(int logn, int LQ, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka);
 hipError_t ks_small(dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka, const uint64_t* gamma_res);
 // resident key: pre-transform of gamma and the digit-centring correction
 hipError_t evk_prepare(int logn, hipStream_t s, const RingDev& rd, const uint64_t* gamma, uint64_t* evk,

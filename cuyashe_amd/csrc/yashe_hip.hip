@@ -355,9 +355,11 @@ yhk::IcrtArgs icrt_args(const uint32_t* Y, int N, int ncoef, int set, int mode, 
 
 // Ring product with one of the reconstruction epilogues; sc.y holds at least
 // y_bytes_for(r, 0, batch) bytes.
+// rows_only: only the product kernel runs (its rows stay in sc.y for a
+// fused consumer); out is not written.
 int ring_product(yh_ring* r, const Scratch& sc, const uint64_t* a, const uint64_t* b, uint64_t* out, int batch,
                  int mode, const uint64_t* const* a_list = nullptr, const uint64_t* const* b_list = nullptr,
-                 uint64_t* const* out_list = nullptr) {
+                 uint64_t* const* out_list = nullptr, bool rows_only = false) {
   const int n = r->rd.n, LQ = r->rd.LQ;
   int st;
   yhk::ProdArgs pa{};
@@ -373,7 +375,7 @@ int ring_product(yh_ring* r, const Scratch& sc, const uint64_t* a, const uint64_
   pa.a_list = a_list;
   pa.b_list = b_list;
   st = launch_prod(r, pa, n, batch, true);
-  if (st) return st;
+  if (st || rows_only) return st;
   yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 0, mode, out, LQ, n, (size_t)LQ * n, 0);
   ia.out_list = out_list;
   ia.reverse = 1;  // k_prod wrote the rows from the first pair to the last
@@ -385,9 +387,11 @@ size_t dig_bytes_for(const yh_ring* r, int batch) {
 }
 
 // Keyswitch of `batch` ciphertexts; sc.y holds y_bytes_for(r, 1, batch) bytes
-// and sc.dig dig_bytes_for(r, batch).
+// and sc.dig dig_bytes_for(r, batch). With `fused` (fusable_mulks) c is not
+// read: the digit transforms come from the product rows in sc.y, which
+// ring_product's k_prod left there, through k_ks_scale_digits.
 int keyswitch_impl(yh_ring* r, const Scratch& sc, const yh_evk* k, const uint64_t* c, uint64_t* out, int batch,
-                   uint64_t* const* out_list = nullptr) {
+                   uint64_t* const* out_list = nullptr, bool fused = false) {
   const int n = r->rd.n, LQ = r->rd.LQ, Rks = r->rd.set_R[1];
   note_evk(r, k);
   yhk::KsArgs ka{};
@@ -397,25 +401,60 @@ int keyswitch_impl(yh_ring* r, const Scratch& sc, const yh_evk* k, const uint64_
   ka.corr = k->corr;
   ka.Y = sc.y;
   ka.batch = batch;
+  // k_ks_digits writes the digit transforms from the first ciphertext to
+  // the last, k_ks_scale_digits from the last to the first; k_ks_mac reads
+  // them the other way round and writes its rows in its own order, which
+  // the reconstruction reverses again
+  bool mac_reverse = true;
   if (n >= 16) {
     ka.D = sc.dig;
     const bool w32 = r->rd.w_bits == 32;
-    {
+    if (fused) {
+      ka.P = sc.y;
+      LaunchTimer timer(r->ctx, YH_KERNEL_KS_DIGITS, r->stream);
+      YH_LAUNCH(yhl::ks_scale_digits(r->rd.logn, LQ, dim3(yhk::xcd_grid(batch, 1)), r->stream, r->rd, ka));
+      mac_reverse = false;
+    } else {
       LaunchTimer timer(r->ctx, YH_KERNEL_KS_DIGITS, r->stream);
       YH_LAUNCH(yhl::ks_digits(r->rd.logn, w32, dim3(yhk::xcd_grid(batch, r->rd.lw)), r->stream, r->rd, ka));
     }
+    ka.reverse = mac_reverse ? 1 : 0;
     {
       LaunchTimer timer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
       YH_LAUNCH(yhl::ks_mac(r->rd.logn, dim3(yhk::xcd_grid(batch, Rks)), r->stream, r->rd, ka));
     }
   } else {
+    if (fused) return fail(YH_ERR_INTERNAL, "fused keyswitch below n = 16");
     LaunchTimer timer(r->ctx, YH_KERNEL_KEYSWITCH, r->stream);
     YH_LAUNCH(yhl::ks_small(dim3(Rks, batch), r->stream, r->rd, ka, k->d));
   }
   yhk::IcrtArgs ia = icrt_args(sc.y, n, n, 1, yhk::EPI_MODQ, out, LQ, n, (size_t)LQ * n, 0);
   ia.out_list = out_list;
-  ia.reverse = 0;  // k_ks_mac wrote the rows from the last ciphertext to the first
+  ia.reverse = mac_reverse ? 0 : 1;
   return launch_icrt(r, ia, batch);
+}
+
+// mul+keyswitch rings whose scale-and-round runs inside the digit
+// transforms (k_ks_scale_digits): q of one or two limbs, w = 2^32, n >= 16,
+// product rows that the reconstruction holds in registers.
+bool fusable_mulks(const yh_ring* r) {
+  return r->rd.n >= 16 && r->rd.LQ <= 2 && r->rd.w_bits == 32 &&
+         yhk::icrt_pipelined(r->rd.LQ, r->rd.set_R[0], yhk::EPI_SCALE);
+}
+
+// YASHE multiply followed by the keyswitch; operands from arrays or
+// pointer lists as in ring_product, output to `out` or `out_list`. sc as
+// for mulks_scratch (sc.tmp is unused on fused rings).
+int mul_keyswitch_impl(yh_ring* r, const Scratch& sc, const yh_evk* k, const uint64_t* c1, const uint64_t* c2,
+                       uint64_t* out, int batch, const uint64_t* const* c1_list = nullptr,
+                       const uint64_t* const* c2_list = nullptr, uint64_t* const* out_list = nullptr) {
+  if (fusable_mulks(r)) {
+    if (int st = ring_product(r, sc, c1, c2, nullptr, batch, yhk::EPI_SCALE, c1_list, c2_list, nullptr, true))
+      return st;
+    return keyswitch_impl(r, sc, k, nullptr, out, batch, out_list, true);
+  }
+  if (int st = ring_product(r, sc, c1, c2, sc.tmp, batch, yhk::EPI_SCALE, c1_list, c2_list, nullptr)) return st;
+  return keyswitch_impl(r, sc, k, sc.tmp, out, batch, out_list);
 }
 
 // Scratch of the allocating keyswitch: reconstruction rows and digit
@@ -1032,8 +1071,7 @@ int yh_yashe_mul_keyswitch(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, co
   if (int s = ring_enter(r)) return s;
   Scratch sc;
   if (int st = mulks_scratch(r, batch, &sc)) return st;
-  if (int st = ring_product(r, sc, d_c1, d_c2, sc.tmp, batch, yhk::EPI_SCALE)) return st;
-  return keyswitch_impl(r, sc, k, sc.tmp, d_out, batch);
+  return mul_keyswitch_impl(r, sc, k, d_c1, d_c2, d_out, batch);
 }
 
 int yh_yashe_mul_keyswitch_static(yh_ring* r, const yh_evk* k, const uint64_t* d_c1, const uint64_t* d_c2,
@@ -1059,8 +1097,7 @@ int yh_yashe_mul_keyswitch_static(yh_ring* r, const yh_evk* k, const uint64_t* d
   sc.tmp = (uint64_t*)(base + yb);
   sc.dig = (uint64_t*)(base + yb + tb);
   t_no_alloc = true;
-  int st = ring_product(r, sc, d_c1, d_c2, sc.tmp, batch, yhk::EPI_SCALE);
-  if (!st) st = keyswitch_impl(r, sc, k, sc.tmp, d_out, batch);
+  int st = mul_keyswitch_impl(r, sc, k, d_c1, d_c2, d_out, batch);
   t_no_alloc = false;
   return st;
 }
@@ -1114,8 +1151,7 @@ int yh_yashe_mul_keyswitch_list(yh_ring* r, const yh_evk* k, const uint64_t* con
   if (int s = ring_enter(r)) return s;
   Scratch sc;
   if (int st = mulks_scratch(r, batch, &sc)) return st;
-  if (int st = ring_product(r, sc, nullptr, nullptr, sc.tmp, batch, yhk::EPI_SCALE, d_c1, d_c2, nullptr)) return st;
-  return keyswitch_impl(r, sc, k, sc.tmp, nullptr, batch, d_out);
+  return mul_keyswitch_impl(r, sc, k, nullptr, nullptr, nullptr, batch, d_c1, d_c2, d_out);
 }
 
 int yh_yashe_decrypt_list(yh_ring* r, const uint64_t* d_f, const uint64_t* const* d_c, uint64_t* const* d_m,

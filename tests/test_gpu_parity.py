@@ -397,6 +397,40 @@ def test_mul_keyswitch_fused_equals_steps(gpu_ctx):
     ring.close()
 
 
+# Rings whose scale-and-round runs inside the digit transforms
+# (k_ks_scale_digits: q of one or two limbs, w = 2^32) and rings that take
+# the separate reconstruction and digit kernels (w = 2^16, a four-limb q):
+# mul+keyswitch equals the multiply followed by the keyswitch, and one
+# operation equals the oracle, on random and all-(q - 1) operands.
+@pytest.mark.parametrize("n,q,wb", [
+    (16, Q127, 32), (64, Q61, 32), (256, (1 << 100) - 15, 32), (1024, Q_POLY, 32),
+    (2048, Q127, 32), (1024, Q127, 16), (512, Q255, 32)])
+def test_mul_keyswitch_fused_shapes(gpu_ctx, n, q, wb):
+    rng = random.Random(n + wb)
+    t = 1024
+    ring = ya.Ring(gpu_ctx, n, q, t=t, w_bits=wb)
+    gammas = [rand_poly(rng, n, q) for _ in range(ring.lw)]
+    evk = ring.evk(gpu_ctx.to_device(ring.pack(gammas)))
+    C1 = [rand_poly(rng, n, q), [q - 1] * n, rand_poly(rng, n, q)]
+    C2 = [rand_poly(rng, n, q), [q - 1] * n, [0] * n]
+    batch = len(C1)
+    d1, d2 = upload_polys(gpu_ctx, ring, C1), upload_polys(gpu_ctx, ring, C2)
+    fused = gpu_ctx.alloc(batch * ring.L * n * 8)
+    ring.mul_keyswitch(evk, d1, d2, fused, batch)
+    mid = gpu_ctx.alloc(batch * ring.L * n * 8)
+    steps = gpu_ctx.alloc(batch * ring.L * n * 8)
+    ring.yashe_mul(d1, d2, mid, batch)
+    ring.keyswitch(evk, mid, steps, batch)
+    got = download_polys(fused, ring, batch)
+    assert got == download_polys(steps, ring, batch)
+    for i in (0, 1):
+        want = cpuref.keyswitch(cpuref.yashe_mul(C1[i], C2[i], n, q, t), gammas, n, q, wb)
+        assert got[i] == want, f"operation {i}"
+    assert got[2] == [0] * n
+    evk.close()
+    ring.close()
+
+
 def test_decrypt_matches_oracle(gpu_ctx):
     rng = random.Random(14)
     n, q, t = 1024, Q127, 1024
