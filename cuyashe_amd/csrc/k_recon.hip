@@ -26,6 +26,15 @@ hipError_t icrt(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd,
 #undef C
 }
 
+hipError_t icrt_modq(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
+  switch (LQ) {
+    case 1: hipLaunchKernelGGL(yhk::k_icrt_modq<1>, grid, block, 0, s, rd, ia); break;
+    case 2: hipLaunchKernelGGL(yhk::k_icrt_modq<2>, grid, block, 0, s, rd, ia); break;
+    default: return hipErrorInvalidValue;  // the pipelined shapes have q of one or two limbs
+  }
+  return hipGetLastError();
+}
+
 hipError_t addsub_q(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const uint64_t* a,
                     const uint64_t* b, uint64_t* c, int n, size_t count, int sub, yhk::AddsubLists lists) {
 #define C(L) hipLaunchKernelGGL(yhk::k_addsub_q<L>, grid, block, 0, s, rd, a, b, c, n, count, sub, lists)

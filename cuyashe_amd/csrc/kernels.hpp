@@ -1405,6 +1405,55 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSSD_WAVES) k_ks_scale_di
   }
 }
 
+// Reconstruction mod q of the pipelined shapes (the keyswitch's last step):
+// a kernel of its own, so its registers are those of the mod-q epilogue
+// alone (k_icrt also holds the scale-and-round one) and more waves stay
+// resident; YH_ICRT_MQ_COEFS coefficients per thread, all their rows
+// requested before the first is computed. Workgroups go from the last pair
+// to the first when ia.reverse is set. Two coefficients at six requested
+// waves take 66 VGPRs (seven resident) and ran 0.126 -> 0.087 ms per launch
+// at n = 4096, batch 2048; eight waves spill 2 dwords and four coefficients
+// (four waves, 118 VGPRs) ran 0.118 ms.
+#ifndef YH_ICRT_MQ_COEFS
+#define YH_ICRT_MQ_COEFS 2
+#endif
+#ifndef YH_ICRT_MQ_WAVES
+#define YH_ICRT_MQ_WAVES 6
+#endif
+template <int LQ>
+__global__ void __launch_bounds__(YH_ICRT_BLOCK) __attribute__((amdgpu_waves_per_eu(YH_ICRT_MQ_WAVES)))
+k_icrt_modq(RingDev rd, IcrtArgs ia) {
+  constexpr int MR = ICRT_PIPE_ROWS;
+  constexpr int K = YH_ICRT_MQ_COEFS;
+  constexpr int LMU = 2 * LQ + 3 - LQ;
+  const int pair = ia.reverse ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
+  const int RS = rd.set_R[ia.set];
+  const int nrow = y_rows(RS);
+  uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
+  const uint32_t* Y = ia.Y + (size_t)pair * nrow * ia.N;
+  const int j0 = blockIdx.x * K * blockDim.x + threadIdx.x;
+  uint32_t rows[K][MR];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    const int j = j0 + u * blockDim.x;
+#pragma unroll
+    for (int r = 0; r < MR; ++r) rows[u][r] = (r < nrow && j < ia.ncoef) ? Y[(size_t)r * ia.N + j] : 0u;
+  }
+  const uint64_t* X[1] = {rd.Mq[ia.set]};
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    const int j = j0 + u * blockDim.x;
+    if (j < ia.ncoef) {
+      uint64_t yP, S[1][2 * LQ], v[LQ + 1], quo[LMU], rem[LQ];
+      const uint32_t k = icrt_pass_regs<LQ, 1, MR>(rd, rows[u], ia.set, X, S, yP);
+      fold_finish<LQ>(S[0], rd.Mq[ia.set], RS, rd.negMq[ia.set], yP, k, v);
+      divmod_v<LQ>(v, rd, quo, rem);
+#pragma unroll
+      for (int l = 0; l < LQ; ++l) o[j + (size_t)l * ia.out_stride] = rem[l];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- primitives
 
 // CRT primitive: residues r[i][j] = coef_j mod p_i in [0, p_i), layout
