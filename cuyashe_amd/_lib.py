@@ -143,6 +143,7 @@ _SIGS = {
     "yh_wire_peek": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(WireHeader)]),
     "yh_ctx_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "yh_ctx_set_timing_mask": (ctypes.c_int, [_vp, ctypes.c_uint]),
+    "yh_ctx_set_fuse_min_batch": (ctypes.c_int, [_vp, ctypes.c_int]),
     "yh_ctx_kernel_time": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_long)]),
     "yh_ctx_reset_timing": (ctypes.c_int, [_vp]),
@@ -384,6 +385,11 @@ class Context:
 
     def reset_timing(self):
         check(lib().yh_ctx_reset_timing(self.h), "yh_ctx_reset_timing")
+
+    def set_fuse_min_batch(self, min_batch: int):
+        """Batch size from which mul+keyswitch fuses the scale-and-round into
+        the digit transforms (0: the device's CU count)."""
+        check(lib().yh_ctx_set_fuse_min_batch(self.h, min_batch), "yh_ctx_set_fuse_min_batch")
 
     def kernel_time(self, kernel_id: int):
         """(total ms, launches) of one fused kernel since timing was enabled."""

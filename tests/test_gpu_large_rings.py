@@ -75,9 +75,16 @@ def test_large_ring_mul_keyswitch(gpu_ctx, n, q):
     dg = gpu_ctx.to_device(ring.pack(want_mul))
     ring.keyswitch(evk, dg, out, batch)
     assert ring.unpack(out.download((batch, ring.L, n))) == want_ks
-    # fused mul + keyswitch
+    # fused mul + keyswitch, at the default batch threshold and with the
+    # scale-and-round inside the digit transforms from every batch size
     ring.mul_keyswitch(evk, d1, d2, out, batch)
     assert ring.unpack(out.download((batch, ring.L, n))) == want_ks
+    try:
+        gpu_ctx.set_fuse_min_batch(1)
+        ring.mul_keyswitch(evk, d1, d2, out, batch)
+        assert ring.unpack(out.download((batch, ring.L, n))) == want_ks
+    finally:
+        gpu_ctx.set_fuse_min_batch(0)
     evk.close()
     ring.close()
 

@@ -415,18 +415,25 @@ def test_mul_keyswitch_fused_shapes(gpu_ctx, n, q, wb):
     C2 = [rand_poly(rng, n, q), [q - 1] * n, [0] * n]
     batch = len(C1)
     d1, d2 = upload_polys(gpu_ctx, ring, C1), upload_polys(gpu_ctx, ring, C2)
-    fused = gpu_ctx.alloc(batch * ring.L * n * 8)
-    ring.mul_keyswitch(evk, d1, d2, fused, batch)
     mid = gpu_ctx.alloc(batch * ring.L * n * 8)
     steps = gpu_ctx.alloc(batch * ring.L * n * 8)
     ring.yashe_mul(d1, d2, mid, batch)
     ring.keyswitch(evk, mid, steps, batch)
-    got = download_polys(fused, ring, batch)
-    assert got == download_polys(steps, ring, batch)
-    for i in (0, 1):
-        want = cpuref.keyswitch(cpuref.yashe_mul(C1[i], C2[i], n, q, t), gammas, n, q, wb)
-        assert got[i] == want, f"operation {i}"
-    assert got[2] == [0] * n
+    want_steps = download_polys(steps, ring, batch)
+    wants = [cpuref.keyswitch(cpuref.yashe_mul(C1[i], C2[i], n, q, t), gammas, n, q, wb) for i in (0, 1)]
+    out = gpu_ctx.alloc(batch * ring.L * n * 8)
+    try:
+        # fused from every batch size, then never fused
+        for min_batch in (1, 1 << 30):
+            gpu_ctx.set_fuse_min_batch(min_batch)
+            ring.mul_keyswitch(evk, d1, d2, out, batch)
+            got = download_polys(out, ring, batch)
+            assert got == want_steps, f"min_batch {min_batch}"
+            for i in (0, 1):
+                assert got[i] == wants[i], f"operation {i}, min_batch {min_batch}"
+            assert got[2] == [0] * n
+    finally:
+        gpu_ctx.set_fuse_min_batch(0)
     evk.close()
     ring.close()
 
