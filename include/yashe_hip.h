@@ -356,7 +356,8 @@ int yh_batch_autotune(yh_ctx* ctx, const yh_batch_item* items, int count, const 
  * times whole operators with clock_gettime, src/benchmark_yashe.cpp:152-291).
  * kernel ids: 0 product (CRT + NTTs + pointwise + INTT), 1 keyswitch
  * multiply-accumulate + inverse transform, 2 ICRT epilogue, 3 persistent
- * batch, 4 keyswitch digit transforms. */
+ * batch, 4 keyswitch digit transforms (in a fused mul+keyswitch together
+ * with the multiply's scale-and-round, see yh_ctx_set_fuse_min_batch). */
 #define YH_KERNEL_PROD 0
 #define YH_KERNEL_KEYSWITCH 1
 #define YH_KERNEL_ICRT 2
