@@ -34,7 +34,8 @@ def _run(cmd):
 # Translation units of the library: the C ABI's host logic and one per kernel
 # family (compiled in parallel, then linked; each object carries its own
 # gfx950 code object).
-HIP_SRCS = ["yashe_hip.hip", "k_prod.hip", "k_keyswitch.hip", "k_recon.hip", "k_misc.hip", "k_batch.hip", "serve.hip"]
+HIP_SRCS = ["k_ks_fused_lq2.hip", "k_ks_fused_lq1.hip", "k_prod.hip", "k_batch.hip", "k_keyswitch.hip", "k_recon.hip",
+            "k_misc.hip", "yashe_hip.hip", "serve.hip"]
 OBJDIR = os.path.join(ROOT, "_build", "hip")
 
 

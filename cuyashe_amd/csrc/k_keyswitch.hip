@@ -35,23 +35,6 @@ hipError_t ks_mac(int logn, dim3 grid, hipStream_t s, const RingDev& rd, const K
   return hipGetLastError();
 }
 
-hipError_t ks_scale_digits(int logn, int LQ, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka) {
-  const dim3 block((1 << logn) / 16);
-  const size_t lds = lds_bytes(1 << logn);
-  if (LQ != 1 && LQ != 2) return hipErrorInvalidValue;
-  switch (logn) {
-#define X(L)                                                                                    \
-  case L:                                                                                       \
-    if (LQ == 1) hipLaunchKernelGGL((yhk::k_ks_scale_digits<L, 1>), grid, block, lds, s, rd, ka); \
-    else hipLaunchKernelGGL((yhk::k_ks_scale_digits<L, 2>), grid, block, lds, s, rd, ka);         \
-    break;
-    YH_KS_CASES(X)
-#undef X
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
 hipError_t ks_small(dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka, const uint64_t* gamma_res) {
   hipLaunchKernelGGL(yhk::k_keyswitch_small, grid, dim3(64), 0, s, rd, ka, gamma_res);
   return hipGetLastError();
@@ -86,10 +69,6 @@ static void ks_attr() {
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_digits<L, true>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_digits<L, false>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_mac<L>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_ks_scale_digits<L, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes);
-  (void)hipFuncSetAttribute((const void*)yhk::k_ks_scale_digits<L, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_evk_prep<L>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)yhk::k_ks_corr<L>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
@@ -98,6 +77,14 @@ void ks_attrs() {
 #define X(L) ks_attr<L>();
   YH_KS_CASES(X)
 #undef X
+  ks_fused_attrs_lq1();
+  ks_fused_attrs_lq2();
+}
+
+hipError_t ks_scale_digits(int logn, int LQ, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka) {
+  if (LQ == 1) return ks_scale_digits_lq1(logn, grid, s, rd, ka);
+  if (LQ == 2) return ks_scale_digits_lq2(logn, grid, s, rd, ka);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace yhl

@@ -33,8 +33,15 @@ void prod_attrs();
 hipError_t ks_digits(int logn, bool w32, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka);
 hipError_t ks_mac(int logn, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka);
 // the multiply's scale-and-round fused with the digit transforms (q of one or
-// two limbs, w = 2^32): one workgroup per ciphertext, reads ka.P
+// two limbs, w = 2^32, n from 1024 to 16384): one workgroup per ciphertext,
+// reads ka.P. The instantiations live in k_ks_fused_lq1.hip and
+// k_ks_fused_lq2.hip (compiled in parallel: each is a large unrolled kernel).
 hipError_t ks_scale_digits(int logn, int LQ, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka);
+hipError_t ks_scale_digits_lq1(int logn, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka);
+hipError_t ks_scale_digits_lq2(int logn, dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka);
+void ks_fused_attrs_lq1();
+void ks_fused_attrs_lq2();
+constexpr int FUSED_MIN_LOGN = 10;
 hipError_t ks_small(dim3 grid, hipStream_t s, const RingDev& rd, const KsArgs& ka, const uint64_t* gamma_res);
 // resident key: pre-transform of gamma and the digit-centring correction
 hipError_t evk_prepare(int logn, hipStream_t s, const RingDev& rd, const uint64_t* gamma, uint64_t* evk,

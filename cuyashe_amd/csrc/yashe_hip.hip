@@ -445,13 +445,13 @@ int keyswitch_impl(yh_ring* r, const Scratch& sc, const yh_evk* k, const uint64_
 }
 
 // mul+keyswitch calls whose scale-and-round runs inside the digit
-// transforms (k_ks_scale_digits): q of one or two limbs, w = 2^32, n >= 16,
+// transforms (k_ks_scale_digits): q of one or two limbs, w = 2^32, n >= 1024,
 // product rows that the reconstruction holds in registers, and a batch that
 // fills the device with one workgroup per ciphertext (a single ciphertext
 // finishes sooner on the separate kernels: 0.09 vs 0.15 ms at n = 4096).
 bool fusable_mulks(const yh_ring* r, int batch) {
   const int min_batch = r->ctx->fuse_min_batch > 0 ? r->ctx->fuse_min_batch : r->ctx->cus;
-  return batch >= min_batch && r->rd.n >= 16 && r->rd.LQ <= 2 && r->rd.w_bits == 32 &&
+  return batch >= min_batch && r->rd.logn >= yhl::FUSED_MIN_LOGN && r->rd.LQ <= 2 && r->rd.w_bits == 32 &&
          yhk::icrt_pipelined(r->rd.LQ, r->rd.set_R[0], yhk::EPI_SCALE);
 }
 
