@@ -1,7 +1,8 @@
 """The allocation-free mul+keyswitch (yh_yashe_mul_keyswitch_static)
 against the allocating entry point, through the C ABI: two calls in a row on
 the shared static scratch, the second with a smaller batch, each bit for bit
-equal to yh_yashe_mul_keyswitch; a batch beyond the static array is refused."""
+equal to yh_yashe_mul_keyswitch, with and without the scale-and-round inside
+the digit transforms; a batch beyond the static array is refused."""
 import random
 
 import pytest
@@ -29,10 +30,21 @@ def test_static_scratch_equals_allocating(gpu_ctx, n, q, batches):
         d1, d2 = gpu_ctx.to_device(ring.pack(C1)), gpu_ctx.to_device(ring.pack(C2))
         ref = gpu_ctx.alloc(batch * ring.L * n * 8)
         got = gpu_ctx.alloc(batch * ring.L * n * 8)
-        ring.mul_keyswitch(evk, d1, d2, ref, batch)
-        ring.mul_keyswitch_static(evk, d1, d2, got, batch)
-        gpu_ctx.sync()
-        assert (got.download((batch, ring.L, n)) == ref.download((batch, ring.L, n))).all(), batch
+        try:
+            # reference: the allocating entry point with separate kernels;
+            # the static one as the default batch rule takes it, then with
+            # the scale-and-round inside the digit transforms where the ring
+            # allows it
+            gpu_ctx.set_fuse_min_batch(1 << 30)
+            ring.mul_keyswitch(evk, d1, d2, ref, batch)
+            want = ref.download((batch, ring.L, n))
+            for min_batch in (0, 1):
+                gpu_ctx.set_fuse_min_batch(min_batch)
+                ring.mul_keyswitch_static(evk, d1, d2, got, batch)
+                gpu_ctx.sync()
+                assert (got.download((batch, ring.L, n)) == want).all(), (batch, min_batch)
+        finally:
+            gpu_ctx.set_fuse_min_batch(0)
     evk.close()
     ring.close()
 
