@@ -61,8 +61,8 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 // The digit transforms (k_ks_digits, 79 VGPRs) may run six waves; forcing
 // eight (64 VGPRs, small spills) measured slower. At n = 4096 the LDS of a
 // transform (34 KB) admits four workgroups per CU, so every transform kernel
-// there runs four waves per SIMD. The reconstruction (k_icrt) has no LDS;
-// its wave bound is icrt_min_waves below.
+// there runs four waves per SIMD. The reconstructions (k_icrt, k_icrt_modq)
+// have no LDS; their wave bounds are icrt_min_waves and YH_ICRT_MQ_WAVES.
 #ifndef YH_KSDIG_WAVES
 #define YH_KSDIG_WAVES 6
 #endif
@@ -905,9 +905,10 @@ __device__ __forceinline__ bool twice_gt_q(const uint64_t* r, const uint64_t* q)
   return !geq_n<LQ + 1>(qe, d);
 }
 
-// k_icrt's pipelined path (two coefficients per thread, rows held in
-// registers): q of at most two limbs, at most ICRT_PIPE_ROWS scratch rows,
-// mod-q and scale-and-round epilogues. The host sizes the grid with it.
+// The pipelined reconstructions (rows held in registers: k_icrt's two
+// coefficients per thread, k_icrt_modq, k_ks_scale_digits): q of at most two
+// limbs, at most ICRT_PIPE_ROWS scratch rows, mod-q and scale-and-round
+// epilogues. The host sizes the grids and picks the kernels with it.
 #ifndef YH_ICRT_PIPE
 #define YH_ICRT_PIPE 1
 #endif
@@ -1135,7 +1136,7 @@ __device__ __forceinline__ void scale_finish(const RingDev& rd, int set, const u
 
 // icrt_pass over y rows already in registers (yv[r] = row r, rows 0..RS,
 // RS + 1 <= MR): the loop bound is the compile-time MR, so the rows stay in
-// registers (k_icrt's pipelined path).
+// registers (the pipelined reconstructions).
 template <int LQ, int NF, int MR>
 __device__ __forceinline__ uint32_t icrt_pass_regs(const RingDev& rd, const uint32_t (&yv)[MR], int set,
                                                    const uint64_t* const* X, uint64_t (&S)[NF][2 * LQ],
