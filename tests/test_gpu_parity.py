@@ -441,6 +441,12 @@ def test_mul_keyswitch_fused_shapes(gpu_ctx, n, q, wb):
     ring.close()
 
 
+def test_fuse_min_batch_rejects_negative(gpu_ctx):
+    with pytest.raises(ya.YashError, match="bad argument"):
+        gpu_ctx.set_fuse_min_batch(-1)
+    gpu_ctx.set_fuse_min_batch(0)
+
+
 def test_decrypt_matches_oracle(gpu_ctx):
     rng = random.Random(14)
     n, q, t = 1024, Q127, 1024
