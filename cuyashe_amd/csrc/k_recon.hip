@@ -35,6 +35,14 @@ hipError_t icrt_modq(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev
   return hipGetLastError();
 }
 
+hipError_t icrt_wide(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
+#define C(L)                                                                                   \
+  if (mode == yhk::EPI_MODQ) hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia); \
+  else hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia)
+  YH_LQ_SWITCH(LQ, C)
+#undef C
+}
+
 hipError_t addsub_q(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const uint64_t* a,
                     const uint64_t* b, uint64_t* c, int n, size_t count, int sub, yhk::AddsubLists lists) {
 #define C(L) hipLaunchKernelGGL(yhk::k_addsub_q<L>, grid, block, 0, s, rd, a, b, c, n, count, sub, lists)

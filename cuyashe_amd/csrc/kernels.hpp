@@ -1455,6 +1455,33 @@ k_icrt_modq(RingDev rd, IcrtArgs ia) {
   }
 }
 
+// Reconstructions of q of more than two limbs (or more rows than the
+// pipelined path holds) with one epilogue each: mod q (EPI_MODQ) or
+// scale-and-round (EPI_SCALE), one coefficient per thread, the rows loaded
+// eight at a time (icrt_pass). Kernels of their own for the same reason as
+// k_icrt_modq: k_icrt compiles every epilogue and the raw integer into one
+// register allocation.
+template <int LQ, int MODE>
+__global__ void __launch_bounds__(YH_ICRT_BLOCK) k_icrt_wide(RingDev rd, IcrtArgs ia) {
+  const int pair = ia.reverse ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
+  const int RS = rd.set_R[ia.set];
+  uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
+  const uint32_t* Y = ia.Y + (size_t)pair * y_rows(RS) * ia.N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ia.ncoef) return;
+  if constexpr (MODE == EPI_MODQ) {
+    uint64_t rem[LQ];
+    icrt_mod_q<LQ>(rd, Y + j, ia.N, ia.set, rem);
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) o[j + (size_t)l * ia.out_stride] = rem[l];
+  } else {
+    uint64_t yP, S[2][2 * LQ];
+    const uint64_t* X[2] = {rd.beta, rd.alpha};
+    const uint32_t k = icrt_pass<LQ, 2>(rd, Y + j, ia.N, ia.set, X, S, yP);
+    scale_finish<LQ>(rd, ia.set, S, yP, k, o + j, ia.out_stride);
+  }
+}
+
 // ---------------------------------------------------------------- primitives
 
 // CRT primitive: residues r[i][j] = coef_j mod p_i in [0, p_i), layout
