@@ -26,10 +26,17 @@ hipError_t icrt(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd,
 #undef C
 }
 
-hipError_t icrt_modq(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
+hipError_t icrt_modq(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
+  const bool scale = mode == yhk::EPI_SCALE;
   switch (LQ) {
-    case 1: hipLaunchKernelGGL(yhk::k_icrt_modq<1>, grid, block, 0, s, rd, ia); break;
-    case 2: hipLaunchKernelGGL(yhk::k_icrt_modq<2>, grid, block, 0, s, rd, ia); break;
+    case 1:
+      if (scale) hipLaunchKernelGGL((yhk::k_icrt_modq<1, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia);
+      else hipLaunchKernelGGL((yhk::k_icrt_modq<1, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);
+      break;
+    case 2:
+      if (scale) hipLaunchKernelGGL((yhk::k_icrt_modq<2, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia);
+      else hipLaunchKernelGGL((yhk::k_icrt_modq<2, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);
+      break;
     default: return hipErrorInvalidValue;  // the pipelined shapes have q of one or two limbs
   }
   return hipGetLastError();

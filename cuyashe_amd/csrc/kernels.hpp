@@ -1406,9 +1406,11 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSSD_WAVES) k_ks_scale_di
   }
 }
 
-// Reconstruction mod q of the pipelined shapes (the keyswitch's last step):
-// a kernel of its own, so its registers are those of the mod-q epilogue
-// alone (k_icrt also holds the scale-and-round one) and more waves stay
+// Reconstruction mod q of the pipelined shapes (the keyswitch's last step),
+// or with MODE = EPI_SCALE their scale-and-round (a YASHE multiply on its
+// own, or a mul+keyswitch batch too small for k_ks_scale_digits): a kernel
+// per epilogue, so its registers are those of that epilogue alone (k_icrt
+// holds every epilogue and the raw integer) and more waves stay
 // resident; YH_ICRT_MQ_COEFS coefficients per thread, all their rows
 // requested before the first is computed. Workgroups go from the last pair
 // to the first when ia.reverse is set. Two coefficients at six requested
@@ -1421,7 +1423,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSSD_WAVES) k_ks_scale_di
 #ifndef YH_ICRT_MQ_WAVES
 #define YH_ICRT_MQ_WAVES 6
 #endif
-template <int LQ>
+template <int LQ, int MODE = EPI_MODQ>
 __global__ void __launch_bounds__(YH_ICRT_BLOCK) __attribute__((amdgpu_waves_per_eu(YH_ICRT_MQ_WAVES)))
 k_icrt_modq(RingDev rd, IcrtArgs ia) {
   constexpr int MR = ICRT_PIPE_ROWS;
@@ -1440,15 +1442,23 @@ k_icrt_modq(RingDev rd, IcrtArgs ia) {
 #pragma unroll
     for (int r = 0; r < MR; ++r) rows[u][r] = (r < nrow && j < ia.ncoef) ? Y[(size_t)r * ia.N + j] : 0u;
   }
-  const uint64_t* X[1] = {rd.Mq[ia.set]};
 #pragma unroll
   for (int u = 0; u < K; ++u) {
     const int j = j0 + u * blockDim.x;
     if (j < ia.ncoef) {
-      uint64_t yP, S[1][2 * LQ], v[LQ + 1], quo[LMU], rem[LQ];
-      const uint32_t k = icrt_pass_regs<LQ, 1, MR>(rd, rows[u], ia.set, X, S, yP);
-      fold_finish<LQ>(S[0], rd.Mq[ia.set], RS, rd.negMq[ia.set], yP, k, v);
-      divmod_v<LQ>(v, rd, quo, rem);
+      uint64_t yP, rem[LQ];
+      if constexpr (MODE == EPI_MODQ) {
+        uint64_t S[1][2 * LQ], v[LQ + 1], quo[LMU];
+        const uint64_t* X[1] = {rd.Mq[ia.set]};
+        const uint32_t k = icrt_pass_regs<LQ, 1, MR>(rd, rows[u], ia.set, X, S, yP);
+        fold_finish<LQ>(S[0], rd.Mq[ia.set], RS, rd.negMq[ia.set], yP, k, v);
+        divmod_v<LQ>(v, rd, quo, rem);
+      } else {
+        uint64_t S[2][2 * LQ];
+        const uint64_t* X[2] = {rd.beta, rd.alpha};
+        const uint32_t k = icrt_pass_regs<LQ, 2, MR>(rd, rows[u], ia.set, X, S, yP);
+        scale_value<LQ>(rd, ia.set, S, yP, k, rem);
+      }
 #pragma unroll
       for (int l = 0; l < LQ; ++l) o[j + (size_t)l * ia.out_stride] = rem[l];
     }

@@ -329,9 +329,11 @@ int launch_prod(yh_ring* r, yhk::ProdArgs pa, int N, int batch, bool negacyclic)
 int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
   LaunchTimer timer(r->ctx, YH_KERNEL_ICRT, r->stream);
   const int LQ = r->rd.LQ;
-  if (ia.mode == yhk::EPI_MODQ && yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode)) {
+  if ((ia.mode == yhk::EPI_MODQ || ia.mode == yhk::EPI_SCALE) &&
+      yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode)) {
     const int per = YH_ICRT_BLOCK * YH_ICRT_MQ_COEFS;
-    YH_LAUNCH(yhl::icrt_modq(LQ, dim3((ia.ncoef + per - 1) / per, batch), dim3(YH_ICRT_BLOCK), r->stream, r->rd, ia));
+    YH_LAUNCH(yhl::icrt_modq(LQ, ia.mode, dim3((ia.ncoef + per - 1) / per, batch), dim3(YH_ICRT_BLOCK), r->stream,
+                             r->rd, ia));
     return YH_OK;
   }
   // other mod-q and scale-and-round shapes: a kernel per epilogue (k_icrt
