@@ -43,8 +43,10 @@ hipError_t icrt_modq(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, con
 }
 
 hipError_t icrt_wide(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
-#define C(L)                                                                                   \
-  if (mode == yhk::EPI_MODQ) hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia); \
+#define C(L)                                                                                            \
+  if (mode == yhk::EPI_MODQ) hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);     \
+  else if (mode == yhk::EPI_DECRYPT)                                                                       \
+    hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_DECRYPT>), grid, block, 0, s, rd, ia);               \
   else hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia)
   YH_LQ_SWITCH(LQ, C)
 #undef C

@@ -1168,6 +1168,40 @@ __device__ __forceinline__ uint32_t icrt_pass_regs(const RingDev& rd, const uint
   return (uint32_t)(fhi + (flo >> 63));
 }
 
+// Decryption epilogue of one coefficient: m = round(t [g]_q / q) mod t with
+// [g]_q in [0, q) the reconstruction mod q (src/yashe.cpp:167-197, applied to
+// every coefficient); o receives m.
+template <int LQ>
+__device__ __forceinline__ void decrypt_coef(const RingDev& rd, const uint32_t* y, size_t ystride, int set,
+                                             uint64_t* o) {
+  constexpr int LW = 2 * LQ + 1;
+  constexpr int LA = LW + 1;
+  constexpr int LMU = LW + 2 - LQ;
+  uint64_t rem[LQ], a[LA], quo[LMU];
+  icrt_mod_q<LQ>(rd, y, ystride, set, rem);
+#pragma unroll
+  for (int l = 0; l < LA; ++l) a[l] = 0;
+  uint64_t carry = 0;
+#pragma unroll
+  for (int l = 0; l < LQ; ++l) {
+    uint64_t lo = rem[l] * rd.t, hi = gl::umulhi(rem[l], rd.t);
+    uint64_t s = lo + carry;
+    hi += (s < lo);
+    a[l] = s;
+    carry = hi;
+  }
+  a[LQ] = carry;
+  divmod_q<LQ>(a, rd, quo, rem, LQ + 1);
+  if (twice_gt_q<LQ>(rem, rd.q)) {
+    uint64_t one[LMU];
+#pragma unroll
+    for (int k2 = 0; k2 < LMU; ++k2) one[k2] = k2 == 0;
+    add_n<LMU>(quo, one);
+  }
+  // quo <= t here
+  o[0] = quo[0] == rd.t ? 0 : quo[0];
+}
+
 // ICRT of one coefficient plus its epilogue. y points at the operand's first
 // scratch row (rows ystride apart, layout of store_y), o at limb 0 of the
 // output coefficient. EPI_RAW writes the reconstructed integer itself (in
@@ -1187,31 +1221,7 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
     return;
   }
   if (mode == EPI_DECRYPT) {
-    // m = round(t * [g]_q / q) mod t, [g]_q in [0, q)
-    // (src/yashe.cpp:167-197 on every coefficient)
-    uint64_t rem[LQ], a[LA], quo[LMU];
-    icrt_mod_q<LQ>(rd, y, ystride, set, rem);
-#pragma unroll
-    for (int l = 0; l < LA; ++l) a[l] = 0;
-    uint64_t carry = 0;
-#pragma unroll
-    for (int l = 0; l < LQ; ++l) {
-      uint64_t lo = rem[l] * rd.t, hi = gl::umulhi(rem[l], rd.t);
-      uint64_t s = lo + carry;
-      hi += (s < lo);
-      a[l] = s;
-      carry = hi;
-    }
-    a[LQ] = carry;
-    divmod_q<LQ>(a, rd, quo, rem, LQ + 1);
-    if (twice_gt_q<LQ>(rem, rd.q)) {
-      uint64_t one[LMU];
-#pragma unroll
-      for (int k2 = 0; k2 < LMU; ++k2) one[k2] = k2 == 0;
-      add_n<LMU>(quo, one);
-    }
-    // quo <= t here
-    o[0] = quo[0] == rd.t ? 0 : quo[0];
+    decrypt_coef<LQ>(rd, y, ystride, set, o);
     return;
   }
   if (mode == EPI_SCALE) {
@@ -1466,8 +1476,9 @@ k_icrt_modq(RingDev rd, IcrtArgs ia) {
 }
 
 // Reconstructions of q of more than two limbs (or more rows than the
-// pipelined path holds) with one epilogue each: mod q (EPI_MODQ) or
-// scale-and-round (EPI_SCALE), one coefficient per thread, the rows loaded
+// pipelined path holds) with one epilogue each: mod q (EPI_MODQ),
+// scale-and-round (EPI_SCALE) or decryption (EPI_DECRYPT, any q), one
+// coefficient per thread, the rows loaded
 // eight at a time (icrt_pass). Kernels of their own for the same reason as
 // k_icrt_modq: k_icrt compiles every epilogue and the raw integer into one
 // register allocation.
@@ -1484,6 +1495,8 @@ __global__ void __launch_bounds__(YH_ICRT_BLOCK) k_icrt_wide(RingDev rd, IcrtArg
     icrt_mod_q<LQ>(rd, Y + j, ia.N, ia.set, rem);
 #pragma unroll
     for (int l = 0; l < LQ; ++l) o[j + (size_t)l * ia.out_stride] = rem[l];
+  } else if constexpr (MODE == EPI_DECRYPT) {
+    decrypt_coef<LQ>(rd, Y + j, ia.N, ia.set, o + j);
   } else {
     uint64_t yP, S[2][2 * LQ];
     const uint64_t* X[2] = {rd.beta, rd.alpha};

@@ -336,12 +336,12 @@ int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
                              r->rd, ia));
     return YH_OK;
   }
-  // other mod-q and scale-and-round shapes: a kernel per epilogue (k_icrt
-  // holds every epilogue's registers: 200 VGPRs at eight limbs against 62
-  // for mod q alone and 98 for scale-and-round; 7-10 % more mul+keyswitch/s
-  // on 255- and 511-bit rings)
-  if ((ia.mode == yhk::EPI_MODQ || ia.mode == yhk::EPI_SCALE) &&
-      !yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode)) {
+  // decryption and the other mod-q and scale-and-round shapes: a kernel per
+  // epilogue (k_icrt holds every epilogue's registers: 200 VGPRs at eight
+  // limbs against 62 for mod q alone and 98 for scale-and-round; 7-10 % more
+  // mul+keyswitch/s on 255- and 511-bit rings); k_icrt keeps the raw integer
+  if (ia.mode == yhk::EPI_DECRYPT || ((ia.mode == yhk::EPI_MODQ || ia.mode == yhk::EPI_SCALE) &&
+                                       !yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode))) {
     YH_LAUNCH(yhl::icrt_wide(LQ, ia.mode, dim3((ia.ncoef + YH_ICRT_BLOCK - 1) / YH_ICRT_BLOCK, batch),
                              dim3(YH_ICRT_BLOCK), r->stream, r->rd, ia));
     return YH_OK;
