@@ -154,6 +154,39 @@ def test_scalar_in_graph_replays(gpu_ctx):
     ring.close()
 
 
+def test_graph_replays_fused_mul_keyswitch(gpu_ctx):
+    # the scale-and-round inside the digit transforms (k_ks_scale_digits),
+    # captured and replayed: every replay equals the eager run and the oracle
+    rng = random.Random(22)
+    n, q, t = 1024, Q127, 1024
+    ring = ya.Ring(gpu_ctx, n, q, t=t, w_bits=32)
+    gam = [rand_poly(rng, n, q) for _ in range(ring.lw)]
+    evk = ring.evk(put(gpu_ctx, ring, gam))
+    C1 = [rand_poly(rng, n, q) for _ in range(3)]
+    C2 = [rand_poly(rng, n, q) for _ in range(3)]
+    d1, d2 = put(gpu_ctx, ring, C1), put(gpu_ctx, ring, C2)
+    out = gpu_ctx.alloc(3 * ring.L * n * 8)
+    try:
+        gpu_ctx.set_fuse_min_batch(1)
+        ring.mul_keyswitch(evk, d1, d2, out, 3)
+        eager = get(out, ring, 3)
+        assert eager[1] == cpuref.keyswitch(cpuref.yashe_mul(C1[1], C2[1], n, q, t), gam, n, q, 32)
+        gpu_ctx.graph_begin()
+        ring.mul_keyswitch(evk, d1, d2, out, 3)
+        g = gpu_ctx.graph_end()
+        for rep in range(3):
+            ya.lib().yh_memset(gpu_ctx.h, out.ptr, 0, out.nbytes)
+            gpu_ctx.sync()
+            g.launch()
+            gpu_ctx.sync()
+            assert get(out, ring, 3) == eager, rep
+        g.close()
+    finally:
+        gpu_ctx.set_fuse_min_batch(0)
+    evk.close()
+    ring.close()
+
+
 # ------------------------------------------------------------------ lifetimes
 
 def test_graph_holds_ring_scratch_and_key(gpu_ctx):
