@@ -398,15 +398,17 @@ def test_mul_keyswitch_fused_equals_steps(gpu_ctx):
 
 
 # Rings whose scale-and-round can run inside the digit transforms
-# (k_ks_scale_digits: n >= 1024, q of one or two limbs, w = 2^32; one limb,
-# four digits with a partial top word, three digits, two limbs at n = 1024,
-# 2048 and 4096) and rings that always take the separate reconstruction and
+# (k_ks_scale_digits: n >= 1024, q of one or two limbs, w = 2^32; one limb
+# of 33 and 61 bits, two limbs of 65 bits (three digits, the third of one
+# bit), 90, 97 and 100 bits, and full 128 bits, and 127 bits at n = 2048 and
+# 4096) and rings that always take the separate reconstruction and
 # digit kernels (n < 1024, w = 2^16, a four-limb q): mul+keyswitch equals the
 # multiply followed by the keyswitch, and one operation equals the oracle, on
 # random and all-(q - 1) operands, fused from every batch size and never.
 @pytest.mark.parametrize("n,q,wb", [
     (16, Q127, 32), (256, (1 << 100) - 15, 32), (1024, Q61, 32), (1024, (1 << 100) - 15, 32),
     (1024, (1 << 90) - 33, 32), (1024, Q_POLY, 32), (2048, Q127, 32), (4096, Q127, 32),
+    (1024, (1 << 33) - 9, 32), (1024, (1 << 65) - 49, 32), (1024, (1 << 128) - 159, 32),
     (1024, Q127, 16), (512, Q255, 32)])
 def test_mul_keyswitch_fused_shapes(gpu_ctx, n, q, wb):
     rng = random.Random(n + wb)
