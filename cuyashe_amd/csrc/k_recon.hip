@@ -26,28 +26,28 @@ hipError_t icrt(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd,
 #undef C
 }
 
-hipError_t icrt_modq(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
+hipError_t icrt_pipe(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
   const bool scale = mode == yhk::EPI_SCALE;
   switch (LQ) {
     case 1:
-      if (scale) hipLaunchKernelGGL((yhk::k_icrt_modq<1, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia);
-      else hipLaunchKernelGGL((yhk::k_icrt_modq<1, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);
+      if (scale) hipLaunchKernelGGL((yhk::k_icrt_pipe<1, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia);
+      else hipLaunchKernelGGL((yhk::k_icrt_pipe<1, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);
       break;
     case 2:
-      if (scale) hipLaunchKernelGGL((yhk::k_icrt_modq<2, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia);
-      else hipLaunchKernelGGL((yhk::k_icrt_modq<2, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);
+      if (scale) hipLaunchKernelGGL((yhk::k_icrt_pipe<2, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia);
+      else hipLaunchKernelGGL((yhk::k_icrt_pipe<2, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);
       break;
     default: return hipErrorInvalidValue;  // the pipelined shapes have q of one or two limbs
   }
   return hipGetLastError();
 }
 
-hipError_t icrt_wide(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
+hipError_t icrt_epi(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia) {
 #define C(L)                                                                                            \
-  if (mode == yhk::EPI_MODQ) hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);     \
+  if (mode == yhk::EPI_MODQ) hipLaunchKernelGGL((yhk::k_icrt_epi<L, yhk::EPI_MODQ>), grid, block, 0, s, rd, ia);     \
   else if (mode == yhk::EPI_DECRYPT)                                                                       \
-    hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_DECRYPT>), grid, block, 0, s, rd, ia);               \
-  else hipLaunchKernelGGL((yhk::k_icrt_wide<L, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia)
+    hipLaunchKernelGGL((yhk::k_icrt_epi<L, yhk::EPI_DECRYPT>), grid, block, 0, s, rd, ia);               \
+  else hipLaunchKernelGGL((yhk::k_icrt_epi<L, yhk::EPI_SCALE>), grid, block, 0, s, rd, ia)
   YH_LQ_SWITCH(LQ, C)
 #undef C
 }

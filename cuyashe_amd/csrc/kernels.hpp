@@ -61,8 +61,8 @@ constexpr int E = 16;  // elements per thread in every NTT pass (4 radix-2 stage
 // The digit transforms (k_ks_digits, 79 VGPRs) may run six waves; forcing
 // eight (64 VGPRs, small spills) measured slower. At n = 4096 the LDS of a
 // transform (34 KB) admits four workgroups per CU, so every transform kernel
-// there runs four waves per SIMD. The reconstructions (k_icrt, k_icrt_modq)
-// have no LDS; their wave bounds are icrt_min_waves and YH_ICRT_MQ_WAVES.
+// there runs four waves per SIMD. The reconstructions have no LDS; their
+// registers set their waves (k_icrt_pipe asks for YH_ICRT_PIPE_WAVES).
 #ifndef YH_KSDIG_WAVES
 #define YH_KSDIG_WAVES 6
 #endif
@@ -906,7 +906,7 @@ __device__ __forceinline__ bool twice_gt_q(const uint64_t* r, const uint64_t* q)
 }
 
 // The pipelined reconstructions (rows held in registers: k_icrt's two
-// coefficients per thread, k_icrt_modq, k_ks_scale_digits): q of at most two
+// coefficients per thread, k_icrt_pipe, k_ks_scale_digits): q of at most two
 // limbs, at most ICRT_PIPE_ROWS scratch rows, mod-q and scale-and-round
 // epilogues. The host sizes the grids and picks the kernels with it.
 #ifndef YH_ICRT_PIPE
@@ -916,11 +916,6 @@ constexpr int ICRT_PIPE_ROWS = 12;
 __host__ __device__ inline bool icrt_pipelined(int LQ, int RS, int mode) {
   return YH_ICRT_PIPE && LQ <= 2 && RS + 1 <= ICRT_PIPE_ROWS && (mode == EPI_MODQ || mode == EPI_SCALE);
 }
-// Minimum resident waves per SIMD asked of the compiler for k_icrt<LQ>. The
-// pipelined path is latency-bound: unconstrained it takes 106-112 VGPRs
-// (four waves); at five (96 VGPRs, 11-17 dwords spilled) it ran 5 % faster
-// for a two-limb q at n = 4096. Wider moduli keep the compiler's choice.
-__host__ __device__ constexpr int icrt_min_waves(int LQ) { return LQ <= 2 ? 5 : 1; }
 
 struct IcrtArgs {
   const uint32_t* Y;     // [pair][R_S][N]
@@ -1282,49 +1277,16 @@ __device__ __forceinline__ void icrt_coef(const RingDev& rd, const uint32_t* y, 
   for (int l = LW; l < out_limbs; ++l) o[(size_t)l * out_stride] = 0;
 }
 
+// The raw integer of a reconstruction (plain products, EPI_RAW) and every
+// other epilogue through icrt_coef (the host sends mod q, scale-and-round and
+// decryption to k_icrt_pipe and k_icrt_epi, whose registers are those of one
+// epilogue).
 template <int LQ>
-__global__ void __launch_bounds__(YH_ICRT_BLOCK) __attribute__((amdgpu_waves_per_eu(icrt_min_waves(LQ))))
-k_icrt(RingDev rd, IcrtArgs ia) {
+__global__ void __launch_bounds__(YH_ICRT_BLOCK) k_icrt(RingDev rd, IcrtArgs ia) {
   const int pair = ia.reverse ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
   const uint32_t* Y = ia.Y + (size_t)pair * y_rows(RS) * ia.N;
-  if (icrt_pipelined(LQ, RS, ia.mode)) {
-    // two coefficients per thread: the second one's rows are requested
-    // before the first one is computed, so its loads overlap that work
-    constexpr int MR = ICRT_PIPE_ROWS;
-    const int nrow = y_rows(RS);
-    uint32_t cur[MR], nxt[MR];
-    const int j0 = blockIdx.x * 2 * blockDim.x + threadIdx.x, j1 = j0 + blockDim.x;
-#pragma unroll
-    for (int r = 0; r < MR; ++r) cur[r] = (r < nrow && j0 < ia.ncoef) ? Y[(size_t)r * ia.N + j0] : 0u;
-#pragma unroll
-    for (int r = 0; r < MR; ++r) nxt[r] = (r < nrow && j1 < ia.ncoef) ? Y[(size_t)r * ia.N + j1] : 0u;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int j = u ? j1 : j0;
-      const uint32_t(&yv)[MR] = u ? nxt : cur;
-      if (j < ia.ncoef) {
-        uint64_t yP;
-        if (ia.mode == EPI_SCALE) {
-          uint64_t S[2][2 * LQ];
-          const uint64_t* X[2] = {rd.beta, rd.alpha};
-          const uint32_t k = icrt_pass_regs<LQ, 2, MR>(rd, yv, ia.set, X, S, yP);
-          scale_finish<LQ>(rd, ia.set, S, yP, k, o + j, ia.out_stride);
-        } else {
-          constexpr int LMU = 2 * LQ + 3 - LQ;
-          uint64_t S[1][2 * LQ], v[LQ + 1], quo[LMU], rem[LQ];
-          const uint64_t* X[1] = {rd.Mq[ia.set]};
-          const uint32_t k = icrt_pass_regs<LQ, 1, MR>(rd, yv, ia.set, X, S, yP);
-          fold_finish<LQ>(S[0], rd.Mq[ia.set], RS, rd.negMq[ia.set], yP, k, v);
-          divmod_v<LQ>(v, rd, quo, rem);
-#pragma unroll
-          for (int l = 0; l < LQ; ++l) o[j + (size_t)l * ia.out_stride] = rem[l];
-        }
-      }
-    }
-    return;
-  }
 #pragma unroll
   for (int u = 0; u < YH_ICRT_COEFS; ++u) {
     const int j = (blockIdx.x * YH_ICRT_COEFS + u) * blockDim.x + threadIdx.x;
@@ -1333,17 +1295,6 @@ k_icrt(RingDev rd, IcrtArgs ia) {
   }
 }
 
-// The YASHE multiply's scale-and-round fused with the keyswitch digit
-// transforms (mul+keyswitch, q of at most two limbs, w = 2^32): one
-// workgroup per ciphertext. Thread g reconstructs round(t c / q) mod q for
-// the coefficients g + c n/16 straight from the product's rows -- exactly
-// the 16 elements it holds in every digit transform -- keeps their 32-bit
-// words (the digits) in registers and transforms one digit after the other
-// through the same LDS. The scaled product is never written to memory and
-// never read back, and one reconstruction launch disappears. Ciphertexts go
-// from the last to the first (k_prod wrote the rows from the first to the
-// last) and each keeps its XCD, that of k_ks_mac's workgroups that read its
-// transforms.
 // Digit d of ciphertext ct (32-bit words w of its 16 coefficients) centred,
 // twisted and transformed into the slot-major digit layout of k_ks_digits.
 template <int LOGN>
@@ -1386,6 +1337,17 @@ __device__ __forceinline__ void scale_words(const RingDev& rd, const uint32_t* Y
   if constexpr (C + 1 < E) scale_words<LOGN, LQ, AH, C + 1>(rd, Y, nrow, rows, w);
 }
 
+// The YASHE multiply's scale-and-round fused with the keyswitch digit
+// transforms (mul+keyswitch, q of at most two limbs, w = 2^32): one
+// workgroup per ciphertext. Thread g reconstructs round(t c / q) mod q for
+// the coefficients g + c n/16 straight from the product's rows -- exactly
+// the 16 elements it holds in every digit transform -- keeps their 32-bit
+// words (the digits) in registers and transforms one digit after the other
+// through the same LDS. The scaled product is never written to memory and
+// never read back, and one reconstruction launch disappears. Ciphertexts go
+// from the last to the first (k_prod wrote the rows from the first to the
+// last) and each keeps its XCD, that of k_ks_mac's workgroups that read its
+// transforms.
 template <int LOGN, int LQ>
 __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSSD_WAVES) k_ks_scale_digits(RingDev rd, KsArgs ka) {
   constexpr int N = 1 << LOGN;
@@ -1421,23 +1383,23 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSSD_WAVES) k_ks_scale_di
 // own, or a mul+keyswitch batch too small for k_ks_scale_digits): a kernel
 // per epilogue, so its registers are those of that epilogue alone (k_icrt
 // holds every epilogue and the raw integer) and more waves stay
-// resident; YH_ICRT_MQ_COEFS coefficients per thread, all their rows
+// resident; YH_ICRT_PIPE_COEFS coefficients per thread, all their rows
 // requested before the first is computed. Workgroups go from the last pair
 // to the first when ia.reverse is set. Two coefficients at six requested
 // waves take 66 VGPRs (seven resident) and ran 0.126 -> 0.087 ms per launch
 // at n = 4096, batch 2048; eight waves spill 2 dwords and four coefficients
 // (four waves, 118 VGPRs) ran 0.118 ms.
-#ifndef YH_ICRT_MQ_COEFS
-#define YH_ICRT_MQ_COEFS 2
+#ifndef YH_ICRT_PIPE_COEFS
+#define YH_ICRT_PIPE_COEFS 2
 #endif
-#ifndef YH_ICRT_MQ_WAVES
-#define YH_ICRT_MQ_WAVES 6
+#ifndef YH_ICRT_PIPE_WAVES
+#define YH_ICRT_PIPE_WAVES 6
 #endif
 template <int LQ, int MODE = EPI_MODQ>
-__global__ void __launch_bounds__(YH_ICRT_BLOCK) __attribute__((amdgpu_waves_per_eu(YH_ICRT_MQ_WAVES)))
-k_icrt_modq(RingDev rd, IcrtArgs ia) {
+__global__ void __launch_bounds__(YH_ICRT_BLOCK) __attribute__((amdgpu_waves_per_eu(YH_ICRT_PIPE_WAVES)))
+k_icrt_pipe(RingDev rd, IcrtArgs ia) {
   constexpr int MR = ICRT_PIPE_ROWS;
-  constexpr int K = YH_ICRT_MQ_COEFS;
+  constexpr int K = YH_ICRT_PIPE_COEFS;
   constexpr int LMU = 2 * LQ + 3 - LQ;
   const int pair = ia.reverse ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
   const int RS = rd.set_R[ia.set];
@@ -1480,10 +1442,10 @@ k_icrt_modq(RingDev rd, IcrtArgs ia) {
 // scale-and-round (EPI_SCALE) or decryption (EPI_DECRYPT, any q), one
 // coefficient per thread, the rows loaded
 // eight at a time (icrt_pass). Kernels of their own for the same reason as
-// k_icrt_modq: k_icrt compiles every epilogue and the raw integer into one
+// k_icrt_pipe: k_icrt compiles every epilogue and the raw integer into one
 // register allocation.
 template <int LQ, int MODE>
-__global__ void __launch_bounds__(YH_ICRT_BLOCK) k_icrt_wide(RingDev rd, IcrtArgs ia) {
+__global__ void __launch_bounds__(YH_ICRT_BLOCK) k_icrt_epi(RingDev rd, IcrtArgs ia) {
   const int pair = ia.reverse ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
   const int RS = rd.set_R[ia.set];
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;

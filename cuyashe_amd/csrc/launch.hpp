@@ -54,10 +54,10 @@ void ks_attrs();
 hipError_t icrt(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia);
 // mod-q or scale-and-round reconstruction of the other shapes (any LQ), one
 // coefficient per thread, a kernel per epilogue
-hipError_t icrt_wide(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia);
+hipError_t icrt_epi(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia);
 // mod-q or scale-and-round reconstruction of the pipelined shapes (LQ <= 2),
-// YH_ICRT_MQ_COEFS per thread
-hipError_t icrt_modq(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia);
+// YH_ICRT_PIPE_COEFS per thread
+hipError_t icrt_pipe(int LQ, int mode, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const IcrtArgs& ia);
 hipError_t addsub_q(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const uint64_t* a,
                     const uint64_t* b, uint64_t* c, int n, size_t count, int sub, yhk::AddsubLists lists);
 hipError_t scalar_q(int LQ, dim3 grid, dim3 block, hipStream_t s, const RingDev& rd, const uint64_t* a, uint64_t* c,

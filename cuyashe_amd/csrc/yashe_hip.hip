@@ -331,8 +331,8 @@ int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
   const int LQ = r->rd.LQ;
   if ((ia.mode == yhk::EPI_MODQ || ia.mode == yhk::EPI_SCALE) &&
       yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode)) {
-    const int per = YH_ICRT_BLOCK * YH_ICRT_MQ_COEFS;
-    YH_LAUNCH(yhl::icrt_modq(LQ, ia.mode, dim3((ia.ncoef + per - 1) / per, batch), dim3(YH_ICRT_BLOCK), r->stream,
+    const int per = YH_ICRT_BLOCK * YH_ICRT_PIPE_COEFS;
+    YH_LAUNCH(yhl::icrt_pipe(LQ, ia.mode, dim3((ia.ncoef + per - 1) / per, batch), dim3(YH_ICRT_BLOCK), r->stream,
                              r->rd, ia));
     return YH_OK;
   }
@@ -342,7 +342,7 @@ int launch_icrt(yh_ring* r, const yhk::IcrtArgs& ia, int batch) {
   // mul+keyswitch/s on 255- and 511-bit rings); k_icrt keeps the raw integer
   if (ia.mode == yhk::EPI_DECRYPT || ((ia.mode == yhk::EPI_MODQ || ia.mode == yhk::EPI_SCALE) &&
                                        !yhk::icrt_pipelined(LQ, r->rd.set_R[ia.set], ia.mode))) {
-    YH_LAUNCH(yhl::icrt_wide(LQ, ia.mode, dim3((ia.ncoef + YH_ICRT_BLOCK - 1) / YH_ICRT_BLOCK, batch),
+    YH_LAUNCH(yhl::icrt_epi(LQ, ia.mode, dim3((ia.ncoef + YH_ICRT_BLOCK - 1) / YH_ICRT_BLOCK, batch),
                              dim3(YH_ICRT_BLOCK), r->stream, r->rd, ia));
     return YH_OK;
   }
