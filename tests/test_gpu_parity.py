@@ -443,6 +443,40 @@ def test_mul_keyswitch_fused_shapes(gpu_ctx, n, q, wb):
     ring.close()
 
 
+def test_mul_keyswitch_list_equals_arrays(gpu_ctx):
+    # yh_yashe_mul_keyswitch_list (the serving loop's entry: per-element
+    # addresses, out of order) equals the array form, with the scale-and-round
+    # fused into the digit transforms and with separate kernels
+    rng = random.Random(31)
+    n, q, t, batch = 1024, Q127, 1024, 5
+    ring = ya.Ring(gpu_ctx, n, q, t=t, w_bits=32)
+    gammas = [rand_poly(rng, n, q) for _ in range(ring.lw)]
+    evk = ring.evk(gpu_ctx.to_device(ring.pack(gammas)))
+    C1 = [rand_poly(rng, n, q) for _ in range(batch)]
+    C2 = [rand_poly(rng, n, q) for _ in range(batch)]
+    d1, d2 = upload_polys(gpu_ctx, ring, C1), upload_polys(gpu_ctx, ring, C2)
+    ref = gpu_ctx.alloc(batch * ring.L * n * 8)
+    ring.mul_keyswitch(evk, d1, d2, ref, batch)
+    want = download_polys(ref, ring, batch)
+    elt = ring.L * n * 8
+    perm = [3, 0, 4, 1, 2]  # element k of the lists is pair perm[k], written to slot k
+    out = gpu_ctx.alloc(batch * elt)
+    l1 = gpu_ctx.to_device(np.array([d1.ptr + p * elt for p in perm], dtype=np.uint64))
+    l2 = gpu_ctx.to_device(np.array([d2.ptr + p * elt for p in perm], dtype=np.uint64))
+    lo = gpu_ctx.to_device(np.array([out.ptr + k * elt for k in range(batch)], dtype=np.uint64))
+    try:
+        for min_batch in (1, 1 << 30):
+            gpu_ctx.set_fuse_min_batch(min_batch)
+            ya.lib().yh_memset(gpu_ctx.h, out.ptr, 0, out.nbytes)
+            ring.mul_keyswitch_list(evk, l1, l2, lo, batch)
+            got = download_polys(out, ring, batch)
+            assert got == [want[p] for p in perm], min_batch
+    finally:
+        gpu_ctx.set_fuse_min_batch(0)
+    evk.close()
+    ring.close()
+
+
 def test_fuse_min_batch_rejects_negative(gpu_ctx):
     with pytest.raises(ya.YashError, match="bad argument"):
         gpu_ctx.set_fuse_min_batch(-1)
