@@ -95,7 +95,8 @@ struct yh_ctx {
   // mul+keyswitch batches of at least this many ciphertexts run the fused
   // scale-and-digit kernel (one workgroup per ciphertext); smaller ones the
   // separate reconstruction and digit kernels, which spread one ciphertext
-  // over more workgroups (0: the device's CU count)
+  // over more workgroups (0: eight per CU, two full rounds of the fused
+  // kernel's four resident workgroups per CU)
   int fuse_min_batch = 0;
   std::string arch;      // gcnArchName
   bool capturing = false;
@@ -459,10 +460,13 @@ int keyswitch_impl(yh_ring* r, const Scratch& sc, const yh_evk* k, const uint64_
 // mul+keyswitch calls whose scale-and-round runs inside the digit
 // transforms (k_ks_scale_digits): q of one or two limbs, w = 2^32, n >= 1024,
 // product rows that the reconstruction holds in registers, and a batch that
-// fills the device with one workgroup per ciphertext (a single ciphertext
-// finishes sooner on the separate kernels: 0.09 vs 0.15 ms at n = 4096).
+// fills the device twice over with one workgroup per ciphertext. Measured at
+// n = 4096, q = 2^127 - 1 against the separate kernels (the single-epilogue
+// scale-and-round and k_ks_digits): slower below 1536 ciphertexts (512: 0.50
+// vs 0.49 ms, one ciphertext 0.15 vs 0.09 ms), equal at 2048 (1.83 ms),
+// about 1 % faster at 4096.
 bool fusable_mulks(const yh_ring* r, int batch) {
-  const int min_batch = r->ctx->fuse_min_batch > 0 ? r->ctx->fuse_min_batch : r->ctx->cus;
+  const int min_batch = r->ctx->fuse_min_batch > 0 ? r->ctx->fuse_min_batch : 8 * r->ctx->cus;
   return batch >= min_batch && r->rd.logn >= yhl::FUSED_MIN_LOGN && r->rd.LQ <= 2 && r->rd.w_bits == 32 &&
          yhk::icrt_pipelined(r->rd.LQ, r->rd.set_R[0], yhk::EPI_SCALE);
 }

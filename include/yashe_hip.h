@@ -376,11 +376,11 @@ int yh_ctx_reset_timing(yh_ctx* ctx);
 
 /* Batch size from which yh_yashe_mul_keyswitch (and its _list and _static
  * forms) runs the multiply's scale-and-round inside the keyswitch digit
- * transforms, one workgroup per ciphertext (q of one or two limbs, w = 2^32);
- * smaller batches take the separate reconstruction and digit kernels, which
- * give one ciphertext more parallelism. 0 (the default): the device's compute
- * unit count. Results are identical either way. A tuning knob with no
- * counterpart in the reference. */
+ * transforms, one workgroup per ciphertext (n >= 1024, q of one or two limbs,
+ * w = 2^32); smaller batches take the separate reconstruction and digit
+ * kernels, which give one ciphertext more parallelism. 0 (the default): eight
+ * per compute unit (2048 on MI355X). Results are identical either way. A
+ * tuning knob with no counterpart in the reference. */
 int yh_ctx_set_fuse_min_batch(yh_ctx* ctx, int min_batch);
 
 /* ------------------------------------------------------------ wire format
