@@ -162,11 +162,11 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   xsync(wave);
   ntt16_inv_rt(xa, my, tab.inv, g, logn, wave);
   if (active) {
-
+    const uint64_t inv0 = rd.invMi[0][0];
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int j = g + c * T;
-      store_y(in.Y, N, i, j, finish_any(rd, 0, i, gl::mont_mul(xa[c], tab.untwist_pw[j])));
+      store_y(in.Y, N, i, j, finish_any(gold, sk, inv0, gl::mont_mul(xa[c], tab.untwist_pw[j])));
     }
   }
   // Publish the residues: every wave waits until its stores have reached L2

@@ -372,12 +372,14 @@ __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int l
   }
 }
 
-// y of residue i of set `set` after the inverse transform (v already
-// scaled/untwisted): modulus 0 keeps the exact residue mod P, the others
-// reduce the signed convolution value mod p_i (finish_small); both are
-// multiplied by (M/m_i)^-1 mod m_i.
-__device__ __forceinline__ uint64_t finish_any(const RingDev& rd, int set, int i, uint64_t v) {
-  return i == 0 ? gl::mul(v, rd.invMi[set][0]) : (uint64_t)finish_small(v, rd.sp[set][i]);
+// y of a residue after the inverse transform (v already scaled/untwisted):
+// modulus 0 (gold) keeps the exact residue mod P, the others reduce the
+// signed convolution value mod p_i (finish_small); both are multiplied by
+// (M/m_i)^-1 mod m_i. The constants come in registers (sk = the prime's row,
+// inv0 = (M/P)^-1 mod P): the rows' stores may alias the tables, so reading
+// the tables here would reload them after every store.
+__device__ __forceinline__ uint64_t finish_any(bool gold, const SmallPrime& sk, uint64_t inv0, uint64_t v) {
+  return gold ? gl::mul(v, inv0) : (uint64_t)finish_small(v, sk);
 }
 
 template <int LOGN, bool NEGA>
@@ -418,13 +420,24 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   ntt16_inv<LOGN, M>(xa, lds, pa.tab.inv, g3, pa.tab.inv0_pw);
   uint32_t* y = pa.Y + (size_t)pair * y_rows(rd.set_R[pa.set]) * N;
   constexpr int stride = N >> 4;
+  auto scaled = [&](int c, int j) {
+    return M ? xa[c] : NEGA ? gl::mont_mul(xa[c], pa.tab.untwist_pw[j]) : gl::mont_mul(xa[c], pa.tab.n_inv_pw);
+  };
+  // one straight-line loop per modulus kind (i is uniform), with the
+  // constants read once
+  if (gold) {
+    const uint64_t inv0 = rd.invMi[pa.set][0];
 #pragma unroll
-  for (int c = 0; c < E; ++c) {
-    const int j = g3 + c * stride;
-    const uint64_t v = M      ? xa[c]
-                       : NEGA ? gl::mont_mul(xa[c], pa.tab.untwist_pw[j])
-                              : gl::mont_mul(xa[c], pa.tab.n_inv_pw);
-    store_y(y, N, i, j, finish_any(rd, pa.set, i, v));
+    for (int c = 0; c < E; ++c) {
+      const int j = g3 + c * stride;
+      store_y(y, N, 0, j, gl::mul(scaled(c, j), inv0));
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      const int j = g3 + c * stride;
+      store_y(y, N, i, j, finish_small(scaled(c, j), sk));
+    }
   }
 }
 
@@ -464,7 +477,8 @@ static __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
       rc[k] = gl::add(rc[k], prod);
     }
   uint32_t* y = pa.Y + (size_t)pair * y_rows(rd.set_R[pa.set]) * N;
-  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(rd, pa.set, i, rc[j]));
+  const uint64_t inv0 = rd.invMi[pa.set][0];
+  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(gold, sk, inv0, rc[j]));
 }
 
 // ---------------------------------------------------------------- keyswitch
@@ -637,7 +651,9 @@ static __global__ void k_keyswitch_small(RingDev rd, KsArgs ka, const uint64_t* 
       }
   }
   uint32_t* y = ka.Y + (size_t)ct * y_rows(Rks) * N;
-  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(rd, 1, i, rc[j]));
+  const SmallPrime sk = rd.sp[1][i];
+  const uint64_t inv0 = rd.invMi[1][0];
+  for (int j = 0; j < N; ++j) store_y(y, N, i, j, finish_any(gold, sk, inv0, rc[j]));
 }
 
 // Pre-transform of the keyswitch keys: gamma_k (coefficients, [lw][LQ][n]) ->
