@@ -97,15 +97,15 @@ __device__ __forceinline__ void batch_load(uint64_t* x, const uint64_t* a, int L
       for (int l = LQ - 1; l >= 0; --l) {
         const uint64_t* al = a + (size_t)l * N + g;
 #pragma unroll
-        for (int c = 0; c < G; ++c) acc[c] = gl::reduce128(acc[c], al[(h + c) * T]);
+        for (int c = 0; c < G; ++c) acc[c] = gl::reduce128(acc[c], ldg(al + (h + c) * T));
       }
     } else {
       for (int l = 0; l < LQ; ++l) {
-        const uint32_t w0 = (uint32_t)pw[2 * l], w1 = (uint32_t)pw[2 * l + 1];
+        const uint32_t w0 = (uint32_t)ldg(pw + 2 * l), w1 = (uint32_t)ldg(pw + 2 * l + 1);
         const uint64_t* al = a + (size_t)l * N + g;
 #pragma unroll
         for (int c = 0; c < G; ++c) {
-          const uint64_t v = al[(h + c) * T];
+          const uint64_t v = ldg(al + (h + c) * T);
           acc[c] += (uint64_t)(uint32_t)v * w0 + (uint64_t)(uint32_t)(v >> 32) * w1;
         }
         if ((l & 3) == 3) {
@@ -117,7 +117,7 @@ __device__ __forceinline__ void batch_load(uint64_t* x, const uint64_t* a, int L
 #pragma unroll
     for (int c = 0; c < G; ++c) {
       const int j = g + (h + c) * T;
-      x[h + c] = GOLD ? gl::mul(acc[c], twist[j]) : twist_centered(mod_small(acc[c], sk), sk.p, twist[j]);
+      x[h + c] = GOLD ? gl::mul(acc[c], ldg(twist + j)) : twist_centered(mod_small(acc[c], sk), sk.p, ldg(twist + j));
     }
     if (h + G < E) group_fence<G>(x + h);
   }
@@ -163,10 +163,16 @@ __device__ __forceinline__ void batch_bundle(const BatchArgs& ba, const BatchBun
   ntt16_inv_rt(xa, my, tab.inv, g, logn, wave);
   if (active) {
     const uint64_t inv0 = rd.invMi[0][0];
+    // The untwist factors are read before the first row is stored: the
+    // stores may alias the table, so a read issued after a store would wait
+    // for it (one memory round trip per coefficient).
+    uint64_t uw[E];
+#pragma unroll
+    for (int c = 0; c < E; ++c) uw[c] = ldg(tab.untwist_pw + g + c * T);
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int j = g + c * T;
-      store_y(in.Y, N, i, j, finish_any(gold, sk, inv0, gl::mont_mul(xa[c], tab.untwist_pw[j])));
+      store_y(in.Y, N, i, j, finish_any(gold, sk, inv0, gl::mont_mul(xa[c], uw[c])));
     }
   }
   // Publish the residues: every wave waits until its stores have reached L2

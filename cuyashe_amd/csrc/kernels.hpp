@@ -250,10 +250,10 @@ __host__ __device__ constexpr int y_rows(int R) { return R + 1; }
 // Stores y of modulus i at coefficient j; Yp is the operand's first row.
 __device__ __forceinline__ void store_y(uint32_t* Yp, size_t N, int i, int j, uint64_t y) {
   if (i == 0) {
-    Yp[j] = (uint32_t)y;
-    Yp[N + j] = (uint32_t)(y >> 32);
+    stg(Yp + j, (uint32_t)y);
+    stg(Yp + N + j, (uint32_t)(y >> 32));
   } else {
-    Yp[(size_t)(i + 1) * N + j] = (uint32_t)y;
+    stg(Yp + (size_t)(i + 1) * N + j, (uint32_t)y);
   }
 }
 
@@ -324,7 +324,7 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         const int j = g + (h + c) * stride;
-        const uint64_t v = j < ncoef ? al[j] : 0;
+        const uint64_t v = j < ncoef ? ldg(al + j) : 0;
         // each product < 2^60; reducing every 4 limbs keeps the sum < 2^63
         acc[c] += (uint64_t)(uint32_t)v * w0 + (uint64_t)(uint32_t)(v >> 32) * w1;
       }
@@ -337,7 +337,7 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
     for (int c = h; c < h + G; ++c) {
       const int j = g + c * stride;
       const uint32_t r = j < ncoef ? mod_small(acc[c - h], k) : 0u;
-      x[c] = MERGED ? twist_slot(r, k.p, c) : NEGA ? twist_centered(r, k.p, twist[j]) : center(r, k.p);
+      x[c] = MERGED ? twist_slot(r, k.p, c) : NEGA ? twist_centered(r, k.p, ldg(twist + j)) : center(r, k.p);
     }
     if (h + G < E) group_fence<G>(x + h);
   }
@@ -360,13 +360,13 @@ __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int l
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         const int j = g + (h + c) * stride;
-        r[c] = gl::reduce128(r[c], j < ncoef ? al[j] : 0);
+        r[c] = gl::reduce128(r[c], j < ncoef ? ldg(al + j) : 0);
       }
     }
 #pragma unroll
     for (int c = h; c < h + G; ++c) {
       const int j = g + c * stride;
-      x[c] = MERGED ? gl::mul_pow2(r[c - h], (78 * c) % 192) : NEGA ? gl::mul(r[c - h], twist[j]) : r[c - h];
+      x[c] = MERGED ? gl::mul_pow2(r[c - h], (78 * c) % 192) : NEGA ? gl::mul(r[c - h], ldg(twist + j)) : r[c - h];
     }
     if (h + G < E) group_fence<G>(x + h);
   }

@@ -63,6 +63,21 @@ __host__ __device__ constexpr int pass_offset(int logn, int p) {
 // x * 2^e mod P for a compile-time e (gl::mul_pow2).
 __device__ __forceinline__ uint64_t mul_pow2(uint64_t x, int e) { return gl::mul_pow2(x, e); }
 
+// Loads and stores through pointers that the compiler cannot place in an
+// address space (pointers read from memory, such as a ring's tables in the
+// persistent batch kernel): the cast makes them global_* instead of flat_*
+// instructions. A flat access also counts against the LDS counter, so every
+// wait for an LDS exchange would wait for it as well. The pointers must
+// address global memory.
+template <class T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ void stg(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+
 // a * b mod P for a < 2^32 (the high word of the 96-bit product is < 2^32).
 __device__ __forceinline__ uint64_t mul_small(uint32_t a, uint64_t b) {
   const uint64_t p0 = (uint64_t)a * (uint32_t)b;
@@ -184,12 +199,12 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
     if (MERGED && p == 0) {
       const uint64_t* t = tw0 + g;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[c << b0]);
+      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (c << b0)));
     } else if (b0 > 0) {
       const int s = 1 << b0;
       const uint64_t* t = tw + pass_offset(LOGN, p) + (g & (s - 1));
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[(brev4(c) - 1) * s]);
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (brev4(c) - 1) * s));
     }
     if (p < NFULL - 1 || SHORT) {
       st16(lds, x, base, b0);
@@ -233,12 +248,12 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
     if (MERGED && p == 0) {
       const uint64_t* t = twi0 + g;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[c << b0]);
+      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (c << b0)));
     } else if (b0 > 0) {
       const int s = 1 << b0;
       const uint64_t* t = twi + pass_offset(LOGN, p) + (g & (s - 1));
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[(brev4(c) - 1) * s]);
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (brev4(c) - 1) * s));
     }
     dit_r<16>(x);
     if (p > 0) {
@@ -295,7 +310,7 @@ __device__ __forceinline__ void ntt16_fwd_rt(uint64_t* x, uint64_t* lds, const u
       const int s = 1 << b0;
       const uint64_t* t = tw + off + (g & (s - 1));
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[(brev4(c) - 1) * s]);
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (brev4(c) - 1) * s));
       off += 15 * s;
     }
     if (p < nfull - 1 || sh) {
@@ -333,7 +348,7 @@ __device__ __forceinline__ void ntt16_inv_rt(uint64_t* x, uint64_t* lds, const u
       off -= 15 * s;
       const uint64_t* t = twi + off + (g & (s - 1));
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], t[(brev4(c) - 1) * s]);
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (brev4(c) - 1) * s));
     }
     dit_r<16>(x);
     if (p > 0) {
