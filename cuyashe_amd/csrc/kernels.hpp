@@ -308,7 +308,10 @@ __device__ __forceinline__ void group_fence(uint64_t* x) {
 // Within a group the limbs are the outer loop, so the group's eight loads of
 // one limb are in flight together (crt_residue's steps, eight coefficients
 // side by side).
-template <int LOGN, bool NEGA, bool MERGED = false, int G = 8>
+// FULL: the operand has all n coefficients (ncoef >= n), so no load needs
+// the guard j < ncoef (its exec-mask branches cost about 4 % of k_prod's
+// instructions).
+template <int LOGN, bool NEGA, bool MERGED = false, int G = 8, bool FULL = false>
 __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                          const uint64_t* pw, const SmallPrime& k, const uint64_t* twist,
                                          int g = threadIdx.x) {
@@ -324,7 +327,7 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         const int j = g + (h + c) * stride;
-        const uint64_t v = j < ncoef ? ldg(al + j) : 0;
+        const uint64_t v = FULL || j < ncoef ? ldg(al + j) : 0;
         // each product < 2^60; reducing every 4 limbs keeps the sum < 2^63
         acc[c] += (uint64_t)(uint32_t)v * w0 + (uint64_t)(uint32_t)(v >> 32) * w1;
       }
@@ -336,7 +339,7 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
 #pragma unroll
     for (int c = h; c < h + G; ++c) {
       const int j = g + c * stride;
-      const uint32_t r = j < ncoef ? mod_small(acc[c - h], k) : 0u;
+      const uint32_t r = FULL || j < ncoef ? mod_small(acc[c - h], k) : 0u;
       x[c] = MERGED ? twist_slot(r, k.p, c) : NEGA ? twist_centered(r, k.p, ldg(twist + j)) : center(r, k.p);
     }
     if (h + G < E) group_fence<G>(x + h);
@@ -345,7 +348,7 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
 
 // Modulus 0 (P): exact residues mod P, twisted by psi^j, not centred
 // (MERGED: by the slot factor 2^(78 c) only).
-template <int LOGN, bool NEGA, bool MERGED = false, int G = 8>
+template <int LOGN, bool NEGA, bool MERGED = false, int G = 8, bool FULL = false>
 __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                            const uint64_t* twist, int g) {
   constexpr int stride = 1 << (LOGN - 4);
@@ -360,7 +363,7 @@ __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int l
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         const int j = g + (h + c) * stride;
-        r[c] = gl::reduce128(r[c], j < ncoef ? ldg(al + j) : 0);
+        r[c] = gl::reduce128(r[c], FULL || j < ncoef ? ldg(al + j) : 0);
       }
     }
 #pragma unroll
@@ -398,8 +401,15 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   // untwist into the last inverse pass (ntt16.hpp, MERGED)
   constexpr bool M = NEGA;
   const int g1 = opaque_tid(pair);
-  if (gold) load_crt_P<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.tab.twist, g1);
-  else load_crt<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, sk, pa.tab.twist, g1);
+  // operands of all n coefficients (every ciphertext) load without guards
+  constexpr int GB = YH_LOAD_GROUP_B;
+  if (gold) {
+    if (pa.ncoef_a >= N) load_crt_P<LOGN, NEGA, M, 8, true>(xa, A, pa.lin_a, N, pa.lstride_a, pa.tab.twist, g1);
+    else load_crt_P<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.tab.twist, g1);
+  } else {
+    if (pa.ncoef_a >= N) load_crt<LOGN, NEGA, M, 8, true>(xa, A, pa.lin_a, N, pa.lstride_a, pw, sk, pa.tab.twist, g1);
+    else load_crt<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, sk, pa.tab.twist, g1);
+  }
   ntt16_fwd<LOGN, M>(xa, lds, pa.tab.fwd, g1, pa.tab.fwd0);
   // The second operand re-reads the tables (L1 hits) rather than keeping the
   // first transform's twiddles live in registers (the empty asm hides that
@@ -409,8 +419,13 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   const uint64_t* fwd02 = pa.tab.fwd0;
   asm volatile("" : "+s"(twist2), "+s"(fwd2), "+s"(fwd02));
   const int g2 = opaque_tid(xa[E - 1]);
-  if (gold) load_crt_P<LOGN, NEGA, M, YH_LOAD_GROUP_B>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
-  else load_crt<LOGN, NEGA, M, YH_LOAD_GROUP_B>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, sk, twist2, g2);
+  if (gold) {
+    if (pa.ncoef_b >= N) load_crt_P<LOGN, NEGA, M, GB, true>(xb, Bp, pa.lin_b, N, pa.lstride_b, twist2, g2);
+    else load_crt_P<LOGN, NEGA, M, GB>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
+  } else {
+    if (pa.ncoef_b >= N) load_crt<LOGN, NEGA, M, GB, true>(xb, Bp, pa.lin_b, N, pa.lstride_b, pw, sk, twist2, g2);
+    else load_crt<LOGN, NEGA, M, GB>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, sk, twist2, g2);
+  }
   __syncthreads();  // every thread has left the LDS of the first transform
   ntt16_fwd<LOGN, M>(xb, lds, fwd2, g2, fwd02);
 #pragma unroll
@@ -670,8 +685,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_evk_prep(Rin
   const uint64_t* g = G + (size_t)k * rd.LQ * N;
   uint64_t* o = out + ((size_t)k * Rks + i) * N;
   uint64_t x[E];
-  if (i == 0) load_crt_P<LOGN, true>(x, g, rd.LQ, N, N, rd.tab.twist, threadIdx.x);
-  else load_crt<LOGN, true>(x, g, rd.LQ, N, N, pw, sk, rd.tab.twist);
+  if (i == 0) load_crt_P<LOGN, true, false, 8, true>(x, g, rd.LQ, N, N, rd.tab.twist, threadIdx.x);
+  else load_crt<LOGN, true, false, 8, true>(x, g, rd.LQ, N, N, pw, sk, rd.tab.twist);
   ntt16_fwd<LOGN>(x, lds, rd.tab.fwd);
 #pragma unroll
   for (int c = 0; c < E; ++c) o[c * (N >> 4) + threadIdx.x] = x[c];
