@@ -1345,19 +1345,32 @@ __device__ __forceinline__ void ks_digit_transform(const RingDev& rd, const KsAr
   for (int c = 0; c < E; ++c) o[c * stride] = x[c];
 }
 
+// 32-bit rows of one operand through a buffer resource: the workgroup-
+// uniform base and row offset sit in scalar registers and the thread's byte
+// offset in one vector register, so a row load takes no vector address
+// arithmetic. The resource spans `bytes` from `base`, which covers every
+// offset read through it (the range check applies to the vector offset).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const uint32_t* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0);
+}
+
 // Coefficients C, C + 1, ... E - 1 of k_ks_scale_digits' reconstruction:
 // round(t c / q) mod q from the rows in rows[C mod (AH + 1)] into the words
 // w[.][C], after requesting the rows of coefficient C + AH. Recursion on a
 // compile-time C keeps every index static, so w and rows stay in registers.
 template <int LOGN, int LQ, int AH, int C>
-__device__ __forceinline__ void scale_words(const RingDev& rd, const uint32_t* Y, int nrow,
+__device__ __forceinline__ void scale_words(const RingDev& rd, __amdgpu_buffer_rsrc_t Y, uint32_t voff, int nrow,
                                             uint32_t (&rows)[AH + 1][ICRT_PIPE_ROWS], uint32_t (&w)[2 * LQ][E]) {
   constexpr int N = 1 << LOGN;
   constexpr int stride = N >> 4;
   constexpr int MR = ICRT_PIPE_ROWS;
   if constexpr (C + AH < E) {
 #pragma unroll
-    for (int r = 0; r < MR; ++r) rows[(C + AH) % (AH + 1)][r] = r < nrow ? Y[(size_t)r * N + (C + AH) * stride] : 0u;
+    for (int r = 0; r < MR; ++r)
+      rows[(C + AH) % (AH + 1)][r] = r < nrow ? row_load(Y, voff, (uint32_t)(r * N + (C + AH) * stride) * 4u) : 0u;
   }
   const uint64_t* X[2] = {rd.beta, rd.alpha};
   uint64_t yP, S[2][2 * LQ], rem[LQ];
@@ -1365,7 +1378,7 @@ __device__ __forceinline__ void scale_words(const RingDev& rd, const uint32_t* Y
   scale_value<LQ>(rd, 0, S, yP, k, rem);
 #pragma unroll
   for (int v = 0; v < 2 * LQ; ++v) w[v][C] = (uint32_t)(rem[v >> 1] >> (32 * (v & 1)));
-  if constexpr (C + 1 < E) scale_words<LOGN, LQ, AH, C + 1>(rd, Y, nrow, rows, w);
+  if constexpr (C + 1 < E) scale_words<LOGN, LQ, AH, C + 1>(rd, Y, voff, nrow, rows, w);
 }
 
 // The YASHE multiply's scale-and-round fused with the keyswitch digit
@@ -1389,7 +1402,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSSD_WAVES) k_ks_scale_di
   if (!xcd_map_rev(blockIdx.x, 1, ka.batch, ct, unit)) return;
   const int g = threadIdx.x;
   const int nrow = y_rows(rd.set_R[0]);
-  const uint32_t* Y = ka.P + (size_t)ct * nrow * N + g;
+  const __amdgpu_buffer_rsrc_t Y = rows_rsrc(ka.P + (size_t)ct * nrow * N, (uint32_t)(nrow * N) * 4u);
+  const uint32_t voff = (uint32_t)g * 4u;
   uint32_t w[NW][E];
   // rows of coefficient g + c stride; the rows of the next AH coefficients
   // are requested before the current one is computed
@@ -1398,8 +1412,8 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSSD_WAVES) k_ks_scale_di
 #pragma unroll
   for (int c = 0; c < AH; ++c)
 #pragma unroll
-    for (int r = 0; r < MR; ++r) rows[c][r] = r < nrow ? Y[(size_t)r * N + c * stride] : 0u;
-  scale_words<LOGN, LQ, AH, 0>(rd, Y, nrow, rows, w);
+    for (int r = 0; r < MR; ++r) rows[c][r] = r < nrow ? row_load(Y, voff, (uint32_t)(r * N + c * stride) * 4u) : 0u;
+  scale_words<LOGN, LQ, AH, 0>(rd, Y, voff, nrow, rows, w);
   // one call per digit with a compile-time index, so w stays in registers
   ks_digit_transform<LOGN>(rd, ka, ct, 0, w[0]);
   if (rd.lw > 1) ks_digit_transform<LOGN>(rd, ka, ct, 1, w[1], true);
@@ -1436,14 +1450,25 @@ k_icrt_pipe(RingDev rd, IcrtArgs ia) {
   const int RS = rd.set_R[ia.set];
   const int nrow = y_rows(RS);
   uint64_t* o = ia.out_list ? ia.out_list[pair] : ia.out + pair * ia.out_pstride;
-  const uint32_t* Y = ia.Y + (size_t)pair * nrow * ia.N;
+  const __amdgpu_buffer_rsrc_t Y = rows_rsrc(ia.Y + (size_t)pair * nrow * ia.N, (uint32_t)(nrow * ia.N) * 4u);
   const int j0 = blockIdx.x * K * blockDim.x + threadIdx.x;
   uint32_t rows[K][MR];
+  // a block whose coefficients all exist loads without the per-thread guard
+  if ((blockIdx.x + 1) * K * (int)blockDim.x <= ia.ncoef) {
 #pragma unroll
-  for (int u = 0; u < K; ++u) {
-    const int j = j0 + u * blockDim.x;
+    for (int u = 0; u < K; ++u) {
+      const uint32_t voff = (uint32_t)(j0 + u * (int)blockDim.x) * 4u;
 #pragma unroll
-    for (int r = 0; r < MR; ++r) rows[u][r] = (r < nrow && j < ia.ncoef) ? Y[(size_t)r * ia.N + j] : 0u;
+      for (int r = 0; r < MR; ++r) rows[u][r] = r < nrow ? row_load(Y, voff, (uint32_t)(r * ia.N) * 4u) : 0u;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      const int j = j0 + u * blockDim.x;
+#pragma unroll
+      for (int r = 0; r < MR; ++r)
+        rows[u][r] = (r < nrow && j < ia.ncoef) ? row_load(Y, (uint32_t)j * 4u, (uint32_t)(r * ia.N) * 4u) : 0u;
+    }
   }
 #pragma unroll
   for (int u = 0; u < K; ++u) {
