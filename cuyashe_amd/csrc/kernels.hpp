@@ -513,6 +513,19 @@ static __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
 // admits primes twice as large and saves one keyswitch modulus on the common
 // rings. The reference (src/ciphertext.cpp:81-130) runs one full CRT product
 // per digit.
+// 64-bit words through a buffer resource: the workgroup-uniform base and
+// word offset sit in scalar registers and the thread's byte offset in one
+// vector register, so a load takes no vector address arithmetic. The
+// resource spans `bytes` from `base`: every load's scalar plus vector offset
+// must lie below it (reads beyond the range return 0).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t words_rsrc(const uint64_t* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint64_t word_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0);
+  return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+}
+
 struct KsArgs {
   const uint64_t* C;        // [ct][LQ][n]
   size_t ostride;
@@ -580,8 +593,12 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
   const bool mine = ka.reverse ? xcd_map_rev(blockIdx.x, Rks, ka.batch, ct, i) : xcd_map(blockIdx.x, Rks, ka.batch, ct, i);
   if (!mine) return;
   const int g = threadIdx.x;
-  const uint64_t* dg = ka.D + (size_t)ct * rd.lw * N + g;
-  const uint64_t* ev = ka.evk + (size_t)i * N + g;
+  // the ciphertext's digit transforms and this modulus' keys through buffer
+  // resources: uniform bases and offsets in scalar registers, one vector
+  // offset per thread (no vector address arithmetic per load)
+  const __amdgpu_buffer_rsrc_t dg = words_rsrc(ka.D + (size_t)ct * rd.lw * N, (uint32_t)(rd.lw * N) * 8u);
+  const __amdgpu_buffer_rsrc_t ev = words_rsrc(ka.evk + (size_t)i * N, (uint32_t)(((rd.lw - 1) * Rks + 1) * N) * 8u);
+  const uint32_t voff = (uint32_t)g * 8u;
   // The l_w products of a slot are summed exactly as 128-bit values (top
   // counts the carries out of bit 128, < l_w <= 16) and reduced once.
   uint64_t acc[E];
@@ -596,12 +613,13 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_KSMAC_WAVES) k_ks_mac(Rin
       top[c] = 0;
     }
     for (int k = 0; k < rd.lw; ++k) {
-      const uint64_t* dk = dg + (size_t)k * N;
-      const uint64_t* ek = ev + (size_t)k * Rks * N;
+      const uint32_t dk = (uint32_t)(k * N) * 8u;
+      const uint32_t ek = (uint32_t)(k * Rks * N) * 8u;
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         uint64_t ph, pl;
-        gl::mul128(dk[(h + c) * stride], ek[(h + c) * stride], ph, pl);
+        const uint32_t co = (uint32_t)((h + c) * stride) * 8u;
+        gl::mul128(word_load(dg, voff, dk + co), word_load(ev, voff, ek + co), ph, pl);
         lo[c] += pl;
         const uint64_t t = ph + (lo[c] < pl);  // ph <= 2^64 - 2
         hi[c] += t;
@@ -1349,7 +1367,7 @@ __device__ __forceinline__ void ks_digit_transform(const RingDev& rd, const KsAr
 // uniform base and row offset sit in scalar registers and the thread's byte
 // offset in one vector register, so a row load takes no vector address
 // arithmetic. The resource spans `bytes` from `base`, which covers every
-// offset read through it (the range check applies to the vector offset).
+// offset (scalar plus vector) read through it; reads beyond it return 0.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const uint32_t* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(base), (short)0, (int)bytes, 0x00020000);
 }
