@@ -78,6 +78,19 @@ __device__ __forceinline__ void stg(T* p, T v) {
   *(__attribute__((address_space(1))) T*)p = v;
 }
 
+// Twiddle-table reads through a buffer resource: the table's base and the
+// compile-time element offset sit in scalar registers and the thread's
+// offset in one vector register, so the fifteen or sixteen reads of a pass
+// take no vector address arithmetic. The resource's range is not used as a
+// bound (every offset read lies inside the table).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(const uint64_t* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(base), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+__device__ __forceinline__ uint64_t table_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t elem) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, elem * 8u, 0);
+  return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+}
+
 // a * b mod P for a < 2^32 (the high word of the 96-bit product is < 2^32).
 __device__ __forceinline__ uint64_t mul_small(uint32_t a, uint64_t b) {
   const uint64_t p0 = (uint64_t)a * (uint32_t)b;
@@ -197,14 +210,17 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
     if (p > 0) ld16(lds, x, base, b0);
     dif_r<16>(x);
     if (MERGED && p == 0) {
-      const uint64_t* t = tw0 + g;
+      const __amdgpu_buffer_rsrc_t t = table_rsrc(tw0);
+      const uint32_t vo = (uint32_t)g * 8u;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (c << b0)));
+      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], table_load(t, vo, (uint32_t)(c << b0)));
     } else if (b0 > 0) {
       const int s = 1 << b0;
-      const uint64_t* t = tw + pass_offset(LOGN, p) + (g & (s - 1));
+      const __amdgpu_buffer_rsrc_t t = table_rsrc(tw);
+      const uint32_t vo = (uint32_t)(g & (s - 1)) * 8u;
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (brev4(c) - 1) * s));
+      for (int c = 1; c < 16; ++c)
+        x[c] = gl::mont_mul(x[c], table_load(t, vo, (uint32_t)(pass_offset(LOGN, p) + (brev4(c) - 1) * s)));
     }
     if (p < NFULL - 1 || SHORT) {
       st16(lds, x, base, b0);
@@ -246,14 +262,17 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
     const int base = pbase(g, b0);
     if (p < NFULL - 1 || SHORT) ld16(lds, x, base, b0);
     if (MERGED && p == 0) {
-      const uint64_t* t = twi0 + g;
+      const __amdgpu_buffer_rsrc_t t = table_rsrc(twi0);
+      const uint32_t vo = (uint32_t)g * 8u;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (c << b0)));
+      for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], table_load(t, vo, (uint32_t)(c << b0)));
     } else if (b0 > 0) {
       const int s = 1 << b0;
-      const uint64_t* t = twi + pass_offset(LOGN, p) + (g & (s - 1));
+      const __amdgpu_buffer_rsrc_t t = table_rsrc(twi);
+      const uint32_t vo = (uint32_t)(g & (s - 1)) * 8u;
 #pragma unroll
-      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (brev4(c) - 1) * s));
+      for (int c = 1; c < 16; ++c)
+        x[c] = gl::mont_mul(x[c], table_load(t, vo, (uint32_t)(pass_offset(LOGN, p) + (brev4(c) - 1) * s)));
     }
     dit_r<16>(x);
     if (p > 0) {
