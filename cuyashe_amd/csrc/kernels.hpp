@@ -257,6 +257,24 @@ __device__ __forceinline__ void store_y(uint32_t* Yp, size_t N, int i, int j, ui
   }
 }
 
+// 64-bit words through a buffer resource: the workgroup-uniform base and
+// word offset sit in scalar registers and the thread's byte offset in one
+// vector register, so a load takes no vector address arithmetic. The
+// resource spans `bytes` from `base`: every load's scalar plus vector offset
+// must lie below it (reads beyond the range return 0).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t words_rsrc(const uint64_t* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(base), (short)0, (int)bytes, 0x00020000);
+}
+// An operand of `lin` limbs at stride `ls` words, n words each, spans less
+// than 2^31 bytes (it fits a buffer resource and its 32-bit offsets).
+__device__ __forceinline__ bool full_extent(int lin, size_t ls, int n) {
+  return ((size_t)(lin - 1) * ls + (size_t)n) * 8 < ((size_t)1 << 31);
+}
+__device__ __forceinline__ uint64_t word_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0);
+  return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+}
+
 // ---------------------------------------------------------------- fused products
 //
 // k_prod: one workgroup per (pair, prime). CRT both operands on load, forward
@@ -310,12 +328,17 @@ __device__ __forceinline__ void group_fence(uint64_t* x) {
 // side by side).
 // FULL: the operand has all n coefficients (ncoef >= n), so no load needs
 // the guard j < ncoef (its exec-mask branches cost about 4 % of k_prod's
-// instructions).
+// instructions), and its limbs span less than 2^31 bytes, so they are read
+// through a buffer resource.
 template <int LOGN, bool NEGA, bool MERGED = false, int G = 8, bool FULL = false>
 __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                          const uint64_t* pw, const SmallPrime& k, const uint64_t* twist,
                                          int g = threadIdx.x) {
   constexpr int stride = 1 << (LOGN - 4);
+  // FULL: the operand's limbs through a buffer resource (scalar base and
+  // offsets, one vector offset per thread); the caller checks that its
+  // extent fits the resource's 32-bit range
+  const __amdgpu_buffer_rsrc_t ra = words_rsrc(a, FULL ? (uint32_t)(((size_t)(lin - 1) * ls + (1 << LOGN)) * 8) : 0u);
 #pragma unroll
   for (int h = 0; h < E; h += G) {
     uint64_t acc[G];
@@ -327,7 +350,8 @@ __device__ __forceinline__ void load_crt(uint64_t* x, const uint64_t* a, int lin
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         const int j = g + (h + c) * stride;
-        const uint64_t v = FULL || j < ncoef ? ldg(al + j) : 0;
+        const uint64_t v = FULL ? word_load(ra, (uint32_t)g * 8u, (uint32_t)(l * ls + (h + c) * stride) * 8u)
+                                : j < ncoef ? ldg(al + j) : 0;
         // each product < 2^60; reducing every 4 limbs keeps the sum < 2^63
         acc[c] += (uint64_t)(uint32_t)v * w0 + (uint64_t)(uint32_t)(v >> 32) * w1;
       }
@@ -352,6 +376,10 @@ template <int LOGN, bool NEGA, bool MERGED = false, int G = 8, bool FULL = false
 __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int lin, int ncoef, size_t ls,
                                            const uint64_t* twist, int g) {
   constexpr int stride = 1 << (LOGN - 4);
+  // FULL: the operand's limbs through a buffer resource (scalar base and
+  // offsets, one vector offset per thread); the caller checks that its
+  // extent fits the resource's 32-bit range
+  const __amdgpu_buffer_rsrc_t ra = words_rsrc(a, FULL ? (uint32_t)(((size_t)(lin - 1) * ls + (1 << LOGN)) * 8) : 0u);
 #pragma unroll
   for (int h = 0; h < E; h += G) {
     // crt_residue_P's Horner steps, limbs outer
@@ -363,7 +391,8 @@ __device__ __forceinline__ void load_crt_P(uint64_t* x, const uint64_t* a, int l
 #pragma unroll
       for (int c = 0; c < G; ++c) {
         const int j = g + (h + c) * stride;
-        r[c] = gl::reduce128(r[c], FULL || j < ncoef ? ldg(al + j) : 0);
+        r[c] = gl::reduce128(r[c], FULL ? word_load(ra, (uint32_t)g * 8u, (uint32_t)(l * ls + (h + c) * stride) * 8u)
+                                        : j < ncoef ? ldg(al + j) : 0);
       }
     }
 #pragma unroll
@@ -403,11 +432,13 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   const int g1 = opaque_tid(pair);
   // operands of all n coefficients (every ciphertext) load without guards
   constexpr int GB = YH_LOAD_GROUP_B;
+  const bool full_a = pa.ncoef_a >= N && full_extent(pa.lin_a, pa.lstride_a, N);
+  const bool full_b = pa.ncoef_b >= N && full_extent(pa.lin_b, pa.lstride_b, N);
   if (gold) {
-    if (pa.ncoef_a >= N) load_crt_P<LOGN, NEGA, M, 8, true>(xa, A, pa.lin_a, N, pa.lstride_a, pa.tab.twist, g1);
+    if (full_a) load_crt_P<LOGN, NEGA, M, 8, true>(xa, A, pa.lin_a, N, pa.lstride_a, pa.tab.twist, g1);
     else load_crt_P<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pa.tab.twist, g1);
   } else {
-    if (pa.ncoef_a >= N) load_crt<LOGN, NEGA, M, 8, true>(xa, A, pa.lin_a, N, pa.lstride_a, pw, sk, pa.tab.twist, g1);
+    if (full_a) load_crt<LOGN, NEGA, M, 8, true>(xa, A, pa.lin_a, N, pa.lstride_a, pw, sk, pa.tab.twist, g1);
     else load_crt<LOGN, NEGA, M>(xa, A, pa.lin_a, pa.ncoef_a, pa.lstride_a, pw, sk, pa.tab.twist, g1);
   }
   ntt16_fwd<LOGN, M>(xa, lds, pa.tab.fwd, g1, pa.tab.fwd0);
@@ -420,10 +451,10 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   asm volatile("" : "+s"(twist2), "+s"(fwd2), "+s"(fwd02));
   const int g2 = opaque_tid(xa[E - 1]);
   if (gold) {
-    if (pa.ncoef_b >= N) load_crt_P<LOGN, NEGA, M, GB, true>(xb, Bp, pa.lin_b, N, pa.lstride_b, twist2, g2);
+    if (full_b) load_crt_P<LOGN, NEGA, M, GB, true>(xb, Bp, pa.lin_b, N, pa.lstride_b, twist2, g2);
     else load_crt_P<LOGN, NEGA, M, GB>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, twist2, g2);
   } else {
-    if (pa.ncoef_b >= N) load_crt<LOGN, NEGA, M, GB, true>(xb, Bp, pa.lin_b, N, pa.lstride_b, pw, sk, twist2, g2);
+    if (full_b) load_crt<LOGN, NEGA, M, GB, true>(xb, Bp, pa.lin_b, N, pa.lstride_b, pw, sk, twist2, g2);
     else load_crt<LOGN, NEGA, M, GB>(xb, Bp, pa.lin_b, pa.ncoef_b, pa.lstride_b, pw, sk, twist2, g2);
   }
   __syncthreads();  // every thread has left the LDS of the first transform
@@ -513,19 +544,6 @@ static __global__ void k_prod_small(RingDev rd, ProdArgs pa, int negacyclic) {
 // admits primes twice as large and saves one keyswitch modulus on the common
 // rings. The reference (src/ciphertext.cpp:81-130) runs one full CRT product
 // per digit.
-// 64-bit words through a buffer resource: the workgroup-uniform base and
-// word offset sit in scalar registers and the thread's byte offset in one
-// vector register, so a load takes no vector address arithmetic. The
-// resource spans `bytes` from `base`: every load's scalar plus vector offset
-// must lie below it (reads beyond the range return 0).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t words_rsrc(const uint64_t* base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(base), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ uint64_t word_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0);
-  return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
-}
-
 struct KsArgs {
   const uint64_t* C;        // [ct][LQ][n]
   size_t ostride;
