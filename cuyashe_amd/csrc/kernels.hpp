@@ -274,6 +274,25 @@ __device__ __forceinline__ uint64_t word_load(__amdgpu_buffer_rsrc_t rs, uint32_
   const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0);
   return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
 }
+__device__ __forceinline__ void word_store(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff, uint64_t w) {
+  const __attribute__((ext_vector_type(2))) unsigned int d = {(unsigned)w, (unsigned)(w >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b64(d, rs, voff, soff, 0);
+}
+
+// 32-bit rows of one operand through a buffer resource: the workgroup-
+// uniform base and row offset sit in scalar registers and the thread's byte
+// offset in one vector register, so a row load takes no vector address
+// arithmetic. The resource spans `bytes` from `base`, which covers every
+// offset (scalar plus vector) read through it; reads beyond it return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const uint32_t* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0);
+}
+__device__ __forceinline__ void row_store(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, rs, voff, soff, 0);
+}
 
 // ---------------------------------------------------------------- fused products
 //
@@ -464,26 +483,29 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_PROD_WAVES) k_prod(RingDe
   __syncthreads();
   const int g3 = opaque_tid(xa[0]);
   ntt16_inv<LOGN, M>(xa, lds, pa.tab.inv, g3, pa.tab.inv0_pw);
-  uint32_t* y = pa.Y + (size_t)pair * y_rows(rd.set_R[pa.set]) * N;
+  const int nrow = y_rows(rd.set_R[pa.set]);
+  // the pair's rows through a buffer resource (scalar base and row offset)
+  const __amdgpu_buffer_rsrc_t y = rows_rsrc(pa.Y + (size_t)pair * nrow * N, (uint32_t)(nrow * N) * 4u);
+  const uint32_t yv = (uint32_t)g3 * 4u;
   constexpr int stride = N >> 4;
   auto scaled = [&](int c, int j) {
     return M ? xa[c] : NEGA ? gl::mont_mul(xa[c], pa.tab.untwist_pw[j]) : gl::mont_mul(xa[c], pa.tab.n_inv_pw);
   };
   // one straight-line loop per modulus kind (i is uniform), with the
-  // constants read once
+  // constants read once; modulus 0 keeps its 64-bit y in rows 0 and 1,
+  // modulus i >= 1 in row i + 1 (store_y's layout)
   if (gold) {
     const uint64_t inv0 = rd.invMi[pa.set][0];
 #pragma unroll
     for (int c = 0; c < E; ++c) {
-      const int j = g3 + c * stride;
-      store_y(y, N, 0, j, gl::mul(scaled(c, j), inv0));
+      const uint64_t v = gl::mul(scaled(c, g3 + c * stride), inv0);
+      row_store(y, yv, (uint32_t)(c * stride) * 4u, (uint32_t)v);
+      row_store(y, yv, (uint32_t)(N + c * stride) * 4u, (uint32_t)(v >> 32));
     }
   } else {
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      const int j = g3 + c * stride;
-      store_y(y, N, i, j, finish_small(scaled(c, j), sk));
-    }
+    for (int c = 0; c < E; ++c)
+      row_store(y, yv, (uint32_t)((i + 1) * N + c * stride) * 4u, finish_small(scaled(c, g3 + c * stride), sk));
   }
 }
 
@@ -1376,22 +1398,12 @@ __device__ __forceinline__ void ks_digit_transform(const RingDev& rd, const KsAr
   if (after_other) __syncthreads();  // every thread has left the previous digit's LDS
   const int g = opaque_tid(x[E - 1]);
   ntt16_fwd<LOGN, true>(x, lds, rd.tab.fwd, g, rd.tab.fwd0);
-  uint64_t* o = ka.D + ((size_t)ct * rd.lw + d) * N + g;
+  // the digit's transform through a buffer resource over its n words
+  const __amdgpu_buffer_rsrc_t o = words_rsrc(ka.D + ((size_t)ct * rd.lw + d) * N, (uint32_t)N * 8u);
 #pragma unroll
-  for (int c = 0; c < E; ++c) o[c * stride] = x[c];
+  for (int c = 0; c < E; ++c) word_store(o, (uint32_t)g * 8u, (uint32_t)(c * stride) * 8u, x[c]);
 }
 
-// 32-bit rows of one operand through a buffer resource: the workgroup-
-// uniform base and row offset sit in scalar registers and the thread's byte
-// offset in one vector register, so a row load takes no vector address
-// arithmetic. The resource spans `bytes` from `base`, which covers every
-// offset (scalar plus vector) read through it; reads beyond it return 0.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const uint32_t* base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(base), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ uint32_t row_load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
-  return __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0);
-}
 
 // Coefficients C, C + 1, ... E - 1 of k_ks_scale_digits' reconstruction:
 // round(t c / q) mod q from the rows in rows[C mod (AH + 1)] into the words
