@@ -1700,7 +1700,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ntt_cyclic(u
   if (!inverse) {
 #pragma unroll
     for (int c = 0; c < E; ++c) x[c] = d[g + c * stride];
-    ntt16_fwd<LOGN>(x, lds, tab.fwd, g);
+    ntt16_fwd<LOGN, false, false>(x, lds, tab.fwd, g);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < E; ++c) lds[lds_pad(g * E + c)] = x[c];
@@ -1720,7 +1720,7 @@ __global__ void __launch_bounds__((1 << LOGN) / 16, YH_MIN_WAVES) k_ntt_cyclic(u
 #pragma unroll
     for (int c = 0; c < E; ++c) x[c] = lds[lds_pad(g * E + c)];
     __syncthreads();
-    ntt16_inv<LOGN>(x, lds, tab.inv, g);
+    ntt16_inv<LOGN, false, false>(x, lds, tab.inv, g);
 #pragma unroll
     for (int c = 0; c < E; ++c) d[g + c * stride] = gl::mul(x[c], Ninv);
   }

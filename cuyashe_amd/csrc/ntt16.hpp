@@ -198,7 +198,13 @@ __device__ __forceinline__ void xsync(bool wave) {
 // 2^(78 c)) and passes through the 16-point DFT, so the result is the twisted
 // transform, bit for bit. (N = 16: one thread, psi^g = 1 and the table holds
 // 1; psi itself is 2^78.)
-template <int LOGN, bool MERGED = false>
+// TBUF: twiddles through a buffer resource (table_load), the default for
+// n <= 1024, where it frees registers (the n = 1024 product kernel stopped
+// spilling); otherwise plain global loads, which the compiler may move
+// across the LDS exchanges: measured faster for n >= 8192 products (2 %),
+// the circuit of n = 8192 (4 %) and the stand-alone transform kernel (17 %),
+// and even at n = 4096.
+template <int LOGN, bool MERGED = false, bool TBUF = (LOGN <= 10)>
 __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ tw,
                                           int g = threadIdx.x, const uint64_t* __restrict__ tw0 = nullptr) {
   constexpr int NFULL = LOGN / 4;
@@ -214,13 +220,18 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
       const uint32_t vo = (uint32_t)g * 8u;
 #pragma unroll
       for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], table_load(t, vo, (uint32_t)(c << b0)));
-    } else if (b0 > 0) {
+    } else if (b0 > 0 && TBUF) {
       const int s = 1 << b0;
       const __amdgpu_buffer_rsrc_t t = table_rsrc(tw);
       const uint32_t vo = (uint32_t)(g & (s - 1)) * 8u;
 #pragma unroll
       for (int c = 1; c < 16; ++c)
         x[c] = gl::mont_mul(x[c], table_load(t, vo, (uint32_t)(pass_offset(LOGN, p) + (brev4(c) - 1) * s)));
+    } else if (b0 > 0) {
+      const int s = 1 << b0;
+      const uint64_t* t = tw + pass_offset(LOGN, p) + (g & (s - 1));
+#pragma unroll
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (brev4(c) - 1) * s));
     }
     if (p < NFULL - 1 || SHORT) {
       st16(lds, x, base, b0);
@@ -244,7 +255,7 @@ __device__ __forceinline__ void ntt16_fwd(uint64_t* x, uint64_t* lds, const uint
 // by twi0[c N/16 + g] = w_N^(-brev4(c) g) psi^-g N^-1 (times the caller's
 // Montgomery constant) and output slot c (element g + c N/16) by 2^(-78 c):
 // the exit values are untwisted and scaled by N^-1.
-template <int LOGN, bool MERGED = false>
+template <int LOGN, bool MERGED = false, bool TBUF = (LOGN <= 10)>
 __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint64_t* __restrict__ twi,
                                           int g = threadIdx.x, const uint64_t* __restrict__ twi0 = nullptr) {
   constexpr int NFULL = LOGN / 4;
@@ -266,13 +277,18 @@ __device__ __forceinline__ void ntt16_inv(uint64_t* x, uint64_t* lds, const uint
       const uint32_t vo = (uint32_t)g * 8u;
 #pragma unroll
       for (int c = 0; c < 16; ++c) x[c] = gl::mont_mul(x[c], table_load(t, vo, (uint32_t)(c << b0)));
-    } else if (b0 > 0) {
+    } else if (b0 > 0 && TBUF) {
       const int s = 1 << b0;
       const __amdgpu_buffer_rsrc_t t = table_rsrc(twi);
       const uint32_t vo = (uint32_t)(g & (s - 1)) * 8u;
 #pragma unroll
       for (int c = 1; c < 16; ++c)
         x[c] = gl::mont_mul(x[c], table_load(t, vo, (uint32_t)(pass_offset(LOGN, p) + (brev4(c) - 1) * s)));
+    } else if (b0 > 0) {
+      const int s = 1 << b0;
+      const uint64_t* t = twi + pass_offset(LOGN, p) + (g & (s - 1));
+#pragma unroll
+      for (int c = 1; c < 16; ++c) x[c] = gl::mont_mul(x[c], ldg(t + (brev4(c) - 1) * s));
     }
     dit_r<16>(x);
     if (p > 0) {
